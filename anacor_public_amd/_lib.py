@@ -1,0 +1,117 @@
+"""Loader and ctypes prototypes of libanacor_mi355x.so (include/anacor_mi355x.h).
+
+The library is built in-tree by ``python -m anacor_public_amd.build`` (or
+``__graft_entry__.build()``).  There is no fallback: if the library is missing
+``load()`` raises, so nothing silently runs a CPU path in its place.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import threading
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libanacor_mi355x.so")
+
+ANACOR_OK = 0
+ANACOR_ERR_ARG = 1
+ANACOR_ERR_NO_DEVICE = 2
+ANACOR_ERR_HIP = 3
+ANACOR_ERR_GEOMETRY = 4
+FLAG_F64_MEAN = 1
+FLAG_NO_EXP = 2
+RAY_ERR_FACE = 1
+RAY_ERR_ASSERT = 2
+
+# Every symbol the header declares (checked by the CPU test suite).
+EXPORTED = (
+    "ray_tracing_overall", "ray_tracing_gpu_overall", "ray_tracing_single",
+    "ray_tracing_single_mp", "anacor_last_error", "anacor_device_count", "anacor_ctx_create",
+    "anacor_ctx_destroy", "anacor_ctx_set_tile", "anacor_ctx_run", "anacor_ctx_paths",
+    "anacor_ctx_stats", "anacor_reflection_geometry",
+)
+
+PATH_DTYPE = np.dtype([("face", np.int32, 2), ("len", np.int32, 2), ("nseg", np.int32, 2),
+                       ("err", np.int32), ("pad", np.int32), ("last", np.int32, (2, 3)),
+                       ("L", np.float64, (2, 4))])
+assert PATH_DTYPE.itemsize == 120
+
+
+class Stats(ct.Structure):
+    _fields_ = [("kernel_ms", ct.c_double), ("voxel_steps", ct.c_uint64),
+                ("workgroups", ct.c_int64), ("n_voxels", ct.c_int64)]
+
+
+class AnacorError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"anacor error {code}: {message}")
+        self.code = code
+
+
+_lock = threading.Lock()
+_lib = None
+
+
+def _prototypes(lib):
+    vp, i64, i32, dbl = ct.c_void_p, ct.c_int64, ct.c_int, ct.c_double
+    lib.anacor_last_error.restype = ct.c_int
+    lib.anacor_last_error.argtypes = [ct.c_char_p, ct.c_size_t]
+    lib.anacor_device_count.restype = ct.c_int
+    lib.anacor_device_count.argtypes = []
+    lib.anacor_ctx_create.restype = vp
+    lib.anacor_ctx_create.argtypes = [vp, vp, vp, i64, vp, vp, i32, i32]
+    lib.anacor_ctx_destroy.restype = None
+    lib.anacor_ctx_destroy.argtypes = [vp]
+    lib.anacor_ctx_set_tile.restype = ct.c_int
+    lib.anacor_ctx_set_tile.argtypes = [vp, i32]
+    lib.anacor_ctx_run.restype = ct.c_int
+    lib.anacor_ctx_run.argtypes = [vp, vp, vp, i64, vp, vp, vp, i32, vp]
+    lib.anacor_ctx_paths.restype = ct.c_int
+    lib.anacor_ctx_paths.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp]
+    lib.anacor_ctx_stats.restype = ct.c_int
+    lib.anacor_ctx_stats.argtypes = [vp, ct.POINTER(Stats)]
+    lib.anacor_reflection_geometry.restype = ct.c_int
+    lib.anacor_reflection_geometry.argtypes = [vp, vp, i64, vp, vp, vp, i64, vp, vp]
+    # drop-in entry points, with the argtypes the reference's binding uses
+    # (utils/utils_mp.py:59-79, :82-94, :128-148)
+    lib.ray_tracing_overall.restype = ct.POINTER(dbl)
+    lib.ray_tracing_overall.argtypes = [i64, i64, vp, i64, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp,
+                                        i32, i32, i32, i32]
+    lib.ray_tracing_gpu_overall.restype = ct.POINTER(ct.c_float)
+    lib.ray_tracing_gpu_overall.argtypes = [i64, i64, vp, i64, vp, vp, vp, vp, vp, i64, vp, vp, vp,
+                                            vp, vp, i32, i32, i32]
+    lib.ray_tracing_single.restype = dbl
+    lib.ray_tracing_single.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, i32, i64, i32]
+    lib.ray_tracing_single_mp.restype = dbl
+    lib.ray_tracing_single_mp.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, i32, i64, i32, i32]
+    lib.free.restype = None
+    lib.free.argtypes = [vp]
+    return lib
+
+
+def load(path: str = LIB_PATH):
+    """Load (once) and return the library; raise if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise FileNotFoundError(
+                    f"{path} is missing: build it with `python -m anacor_public_amd.build`")
+            _lib = _prototypes(ct.CDLL(path))
+        return _lib
+
+
+def last_error(lib=None):
+    lib = lib or load()
+    buf = ct.create_string_buffer(512)
+    code = lib.anacor_last_error(buf, 512)
+    return code, buf.value.decode(errors="replace")
+
+
+def check(rc: int, lib=None, allow=(ANACOR_OK,)):
+    if rc not in allow:
+        code, msg = last_error(lib)
+        raise AnacorError(rc, msg)
+    return rc

@@ -1,0 +1,53 @@
+// Per-reflection FP64 geometry shared by the host setup (geometry.cpp, built
+// with g++ against glibc libm) and the HIP kernels (trace.hip).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+namespace anacor {
+
+// Bits of RayRule::meta[f] (one word per exit face f = 1..6, index f-1).
+enum : uint32_t {
+  kMetaValid = 1u << 0,      // the reference's precondition for this face holds
+  kMetaRound = 1u << 1,      // round-half-away (else floor)
+  kMetaInclusive = 1u << 2,  // loop bound inclusive
+  kMetaToMax = 1u << 3,      // leaves through the upper plane of `axis`
+  kMetaAxisShift = 4,        // bits 4-5: bounding axis 0 z, 1 y, 2 x
+  kMetaVacShift = 8,         // bits 8-15: full_iteration vacuum-run limit
+};
+
+// One ray direction (incident or diffracted) of one reflection: the
+// cube_face direction and the stepping rule for each possible exit face.
+// 256 bytes, so a reflection's two rays are one 512-byte record.
+struct alignas(16) RayRule {
+  double dir[4];        // myframe (z, y, x) direction, [3] unused
+  double d[6][4];       // per face: per-increment delta (z, y, x), [3] unused
+  uint32_t meta[6];     // per face flags, see above
+  uint32_t neg;         // 1 for the incident ray (cube_face's L1)
+  uint32_t guard;       // 1 when a coordinate can leave the int64 range
+};
+static_assert(sizeof(RayRule) == 256, "RayRule layout");
+
+struct ReflRule {
+  RayRule ray[2];       // [0] incident (traced backwards), [1] diffracted
+};
+
+// Angles and rotated vectors of one reflection (diagnostics / tests).
+struct ReflGeom {
+  double xray[3], s1[3];
+  double theta[2], phi[2];
+  double dir[2][3];
+};
+
+// Fill rules[R] (and geom[R] when non-null) for R reflections, using up to
+// n_threads host threads.  max_dim is the largest volume extent, used to set
+// RayRule::guard.  Thread-safe, allocates nothing.
+void host_setup(const double* s1, const double* omega, int64_t R,
+                const double* raw_xray, const double* omega_axis, const double* F,
+                int64_t max_dim, ReflRule* rules, ReflGeom* geom, int n_threads);
+
+// Same for already-rotated vectors (ray_tracing_single).
+void host_setup_rotated(const double rotated_s1[3], const double xray[3], int64_t max_dim,
+                        ReflRule* rule, ReflGeom* geom);
+
+}  // namespace anacor

@@ -1,0 +1,1048 @@
+// MI355X (gfx950) kernels of the analytical absorption-correction ray tracer
+// and the C-ABI around them.
+//
+// Work item: one (reflection, crystal voxel) pair per lane, both rays of the
+// pair (incident traced backwards, diffracted forwards) on the same lane so
+// cal_rate needs no exchange.  A workgroup covers one reflection and a tile of
+// the Morton-ordered voxel list, so every wave's 64 rays are parallel and start
+// from neighbouring voxels: their label gathers hit the same cache lines.
+//
+// Numerics (bit-exact path lengths against AnACor/src/ray_tracing_basic.c):
+//   * cube_face: six FP64 divisions, the reference's candidate order and ties.
+//   * stepping: coord + (double)inc * d as a separately rounded multiply and
+//     add (this file is built with -ffp-contract=off), then floor or
+//     round-half-away, the clamp to [0, extent-1] and the double->int64 quirk
+//     (values outside int64 become 0) when the host flags a ray as needing it.
+//   * lengths: per-segment terms T * (span / (len-1)) summed in path order
+//     from a per-lane segment list in LDS (exact re-walk when it overflows).
+//   * cal_rate in FP64 with the reference's association; exp in FP32 after an
+//     FP64 range reduction; per-reflection mean by compensated FP32 summation
+//     (Kahan per lane, TwoSum merges across lanes, waves and tiles), or FP64
+//     accumulation when ANACOR_FLAG_F64_MEAN is set.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "geometry.h"
+#include "../../include/anacor_mi355x.h"
+
+namespace anacor {
+
+// ------------------------------------------------------------ error state
+
+static thread_local char g_err[512];
+static std::atomic<int> g_err_code{0};
+
+static void set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  g_err_code.store(code);
+  if (std::getenv("ANACOR_VERBOSE")) fprintf(stderr, "anacor: %s\n", g_err);
+}
+
+static void clear_error() {
+  g_err[0] = 0;
+  g_err_code.store(0);
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_error(ANACOR_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+      return ANACOR_ERR_HIP;                                                       \
+    }                                                                              \
+  } while (0)
+
+// ------------------------------------------------------------ device code
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+
+struct PathRecord {           // per (reflection, voxel): both rays
+  int32_t face[2];
+  int32_t len[2];
+  int32_t nseg[2];
+  int32_t err;
+  int32_t pad;
+  int32_t last[2][3];
+  double L[2][4];
+};
+static_assert(sizeof(PathRecord) == 120, "PathRecord layout");
+
+struct TraceArgs {
+  const uint8_t* vol;
+  int Z, Y, X;
+  const int4* coords;          // (z, y, x, original index)
+  int N;
+  const ReflRule* rules;
+  int R;
+  int tiles;                   // tiles per reflection
+  int vox_per_tile;
+  int n_blocks;                // logical blocks = R * tiles
+  double vs[3];                // voxel size (z, y, x)
+  double mu[4];                // [li, lo, cr, bu]
+  int is_exp;
+  float2* partial32;           // [R * tiles] (sum, compensation)
+  double* partial64;           // [R * tiles]
+  unsigned* err;               // [R]
+  unsigned long long* steps;   // sum of len_path_2
+  PathRecord* paths;           // [R * N] in original voxel order (paths kernel)
+};
+
+// Map hardware workgroup id to a logical one so that consecutive logical
+// blocks (tiles of the same reflection) land on the same XCD: the dispatcher
+// deals workgroups round-robin over the 8 XCDs, each with its own L2.
+__device__ __forceinline__ int xcd_logical_block(int hw, int n_hw) {
+  const int per = n_hw >> 3;  // n_hw is a multiple of 8
+  return (hw & 7) * per + (hw >> 3);
+}
+
+// cube_face (ray_tracing_basic.c:69-156).  0 when no exit plane qualifies.
+__device__ __forceinline__ int exit_face(int z, int y, int x, double dz, double dy, double dx,
+                                         int Z, int Y, int X, bool neg) {
+  double t0 = (double)(0 - x) / dx;   // x_min
+  double t1 = (double)(0 - y) / dy;   // y_min
+  double t2 = (double)(0 - z) / dz;   // z_min
+  double t3 = (double)(X - x) / dx;   // x_max
+  double t4 = (double)(Y - y) / dy;   // y_max
+  double t5 = (double)(Z - z) / dz;   // z_max
+  if (neg) { t0 = -t0; t1 = -t1; t2 = -t2; t3 = -t3; t4 = -t4; t5 = -t5; }
+  const double big = __builtin_huge_val();
+  // minimum over the non-negative candidates (NaN never qualifies)
+  double m = big;
+  bool have = false;
+  if (t0 >= 0) { m = t0; have = true; }
+  if (t1 >= 0) { m = (!have || t1 < m) ? t1 : m; have = true; }
+  if (t2 >= 0) { m = (!have || t2 < m) ? t2 : m; have = true; }
+  if (t3 >= 0) { m = (!have || t3 < m) ? t3 : m; have = true; }
+  if (t4 >= 0) { m = (!have || t4 < m) ? t4 : m; have = true; }
+  if (t5 >= 0) { m = (!have || t5 < m) ? t5 : m; have = true; }
+  if (!have) return 0;
+  if (m == t0) return 6;
+  if (m == t3) return 1;
+  if (m == t1) return 4;
+  if (m == t4) return 5;
+  if (m == t2) return 2;
+  if (m == t5) return 3;
+  return 0;
+}
+
+// One axis of one step: clamp(rnd(base + inc*d)) as an integer-valued double.
+template <bool GUARD>
+__device__ __forceinline__ double axis_step(double base, double incd, double d, bool rnd,
+                                            double hi) {
+  const double w = base + incd * d;
+  double r = __builtin_floor(w);
+  if (rnd && (w - r >= 0.5)) r += 1.0;        // round-half-away for w >= -0.5
+  if (GUARD && !(w < 0x1p63)) r = -1.0;       // x86 double->int64 overflow -> 0
+  r = __builtin_fmax(r, 0.0);                 // NaN -> 0 as well
+  r = __builtin_fmin(r, hi);
+  return r;
+}
+
+struct RayLane {
+  double bz, by, bx;     // origin
+  double dz, dy, dx;     // per-increment deltas
+  double hz, hy, hx;     // extent - 1
+  int iz, iy, ix;        // extent - 1
+  bool rnd;
+  int limit;             // increments tried: inc in [0, limit)
+  int vac;               // full_iteration vacuum-run limit
+};
+
+__device__ __forceinline__ int clamp_int(int v, int hi) { return min(max(v, 0), hi); }
+
+template <bool GUARD>
+__device__ __forceinline__ void coord_at(const RayLane& L, int inc, int& z, int& y, int& x) {
+  const double incd = (double)inc;
+  // the integer clamp keeps every gather in bounds whatever the conversion
+  // of a non-finite value yields
+  z = clamp_int((int)axis_step<GUARD>(L.bz, incd, L.dz, L.rnd, L.hz), L.iz);
+  y = clamp_int((int)axis_step<GUARD>(L.by, incd, L.dy, L.rnd, L.hy), L.iy);
+  x = clamp_int((int)axis_step<GUARD>(L.bx, incd, L.dx, L.rnd, L.hx), L.ix);
+}
+
+// Walk one ray (cal_coord + cal_path2_plus).  Segments beyond the implicit
+// first one (class 3 at 0) go to this lane's column of `seg` (CAP entries,
+// lane-interleaved); `len` is len_path_2; L receives [li, lo, cr, bu].
+template <int CAP, int U, bool GUARD, bool FULL>
+__device__ __forceinline__ void walk_ray(const RayLane& L, const uint8_t* __restrict__ vol,
+                                         int Y, int X, const double vs[3], uint32_t* seg,
+                                         int& len_out, int& nseg_out, int last_xyz[3],
+                                         double out[4]) {
+  int last = -1, prev = 0, nseg = 0, va = 0;
+  bool done = L.limit <= 0;
+  for (int inc0 = 0; !done; inc0 += U) {
+    int lab[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      int z, y, x;
+      coord_at<GUARD>(L, inc0 + u, z, y, x);      // always in bounds
+      lab[u] = vol[(size_t)((z * Y + y) * X + x)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int inc = inc0 + u;
+      if (!done) {
+        if (inc >= L.limit) {
+          done = true;
+        } else {
+          const int lb = (int)(int8_t)lab[u];
+          bool stop;
+          if (FULL) {
+            va = (lb == 0) ? va + 1 : 0;
+            stop = va > L.vac;
+          } else {
+            stop = lb == 0;
+          }
+          if (stop) {
+            done = true;
+          } else {
+            if (inc > 0 && lb != prev && (unsigned)lb <= 4u) {
+              if (nseg < CAP) seg[nseg * kBlock] = ((uint32_t)inc << 3) | (uint32_t)lb;
+              ++nseg;
+            }
+            prev = lb;
+            last = inc;
+          }
+        }
+      }
+    }
+  }
+
+  const int len = last >= 0 ? last + 1 : 1;
+  len_out = len;
+  nseg_out = nseg + 1;
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  last_xyz[0] = last_xyz[1] = last_xyz[2] = 0;
+  if (len == 1) {
+    if (last == 0) {
+      int z, y, x;
+      coord_at<GUARD>(L, 0, z, y, x);
+      last_xyz[0] = z; last_xyz[1] = y; last_xyz[2] = x;
+    }
+    return;
+  }
+  int fz, fy, fx, lz, ly, lx;
+  coord_at<GUARD>(L, 0, fz, fy, fx);
+  coord_at<GUARD>(L, last, lz, ly, lx);
+  last_xyz[0] = lz; last_xyz[1] = ly; last_xyz[2] = lx;
+  const double ey = (double)(ly - fy) * vs[1];
+  const double ez = (double)(lz - fz) * vs[0];
+  const double ex = (double)(lx - fx) * vs[2];
+  const double T = __builtin_sqrt(ey * ey + ez * ez + ex * ex);
+  const double D = (double)(len - 1);
+  double li = 0.0, lo = 0.0, cr = 0.0, bu = 0.0;
+  auto add = [&](int cls, int span) {
+    const double term = T * ((double)span / D);
+    li += (cls == 1) ? term : 0.0;
+    lo += (cls == 2) ? term : 0.0;
+    cr += (cls == 3) ? term : 0.0;
+    bu += (cls == 4) ? term : 0.0;
+  };
+  int cls = 3, start = 0;
+  if (nseg <= CAP) {
+    for (int k = 0; k < nseg; ++k) {
+      const uint32_t e = seg[k * kBlock];
+      const int pos = (int)(e >> 3);
+      add(cls, pos - start);
+      cls = (int)(e & 7u);
+      start = pos;
+    }
+  } else {
+    // segment list overflowed: re-walk the (now known) path in order
+    int pl = 0;
+    {
+      int z, y, x;
+      coord_at<GUARD>(L, 0, z, y, x);
+      pl = (int)(int8_t)vol[(size_t)((z * Y + y) * X + x)];
+    }
+    for (int inc = 1; inc <= last; ++inc) {
+      int z, y, x;
+      coord_at<GUARD>(L, inc, z, y, x);
+      const int lb = (int)(int8_t)vol[(size_t)((z * Y + y) * X + x)];
+      if (lb != pl && (unsigned)lb <= 4u) {
+        add(cls, inc - start);
+        cls = lb;
+        start = inc;
+      }
+      pl = lb;
+    }
+  }
+  add(cls, len - start);
+  out[0] = li; out[1] = lo; out[2] = cr; out[3] = bu;
+}
+
+// Set up the lane's ray for exit face `face` (1..6) of rule `rr`.
+__device__ __forceinline__ bool make_lane(const RayRule* __restrict__ rr, int face, int z, int y,
+                                          int x, int Z, int Y, int X, RayLane& L) {
+  const uint32_t meta = rr->meta[face - 1];
+  L.bz = (double)z; L.by = (double)y; L.bx = (double)x;
+  L.dz = rr->d[face - 1][0];
+  L.dy = rr->d[face - 1][1];
+  L.dx = rr->d[face - 1][2];
+  L.hz = (double)(Z - 1); L.hy = (double)(Y - 1); L.hx = (double)(X - 1);
+  L.iz = Z - 1; L.iy = Y - 1; L.ix = X - 1;
+  L.rnd = (meta & kMetaRound) != 0;
+  L.vac = (int)((meta >> kMetaVacShift) & 0xffu);
+  const int axis = (int)((meta >> kMetaAxisShift) & 3u);
+  const int c = axis == 0 ? z : (axis == 1 ? y : x);
+  const int ext = axis == 0 ? Z : (axis == 1 ? Y : X);
+  L.limit = (meta & kMetaToMax) ? (ext - c) + ((meta & kMetaInclusive) ? 1 : 0) : c + 1;
+  return (meta & kMetaValid) != 0;
+}
+
+// exp(-r) in FP32 after an FP64 range reduction r = k ln2 + f, |f| <= ln2/2.
+__device__ __forceinline__ float exp_neg(double r) {
+  const double t = -r;
+  const double k = __builtin_rint(t * 1.4426950408889634);
+  const double f = (t - k * 0.6931471805599453) - k * 2.3190468138462996e-17;
+  const double kc = __builtin_fmin(__builtin_fmax(k, -300.0), 300.0);
+  return __builtin_ldexpf(expf((float)f), (int)kc);
+}
+
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Kahan pair (s, c) with true value s - c; merge o into it.
+__device__ __forceinline__ void merge_pair(float& s, float& c, float os, float oc) {
+  float t, e;
+  two_sum(s, os, t, e);
+  s = t;
+  c = (c + oc) - e;
+}
+
+template <int CAP, int U, bool GUARD, bool FULL, bool F64, bool PATHS>
+__global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
+  __shared__ uint32_t seg_lds[CAP * kBlock];
+  __shared__ double red64[kBlock / kWave];
+  __shared__ float red32[2][kBlock / kWave];
+  __shared__ unsigned long long red_steps[kBlock / kWave];
+
+  const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
+  if (lb >= a.n_blocks) return;
+  const int refl = lb / a.tiles;
+  const int tile = lb - refl * a.tiles;
+  const ReflRule* __restrict__ rule = a.rules + refl;
+  const int tid = (int)threadIdx.x;
+  uint32_t* seg = seg_lds + tid;
+
+  float ks = 0.f, kc = 0.f;   // Kahan sum (value ks - kc)
+  double acc64 = 0.0;
+  unsigned long long steps = 0;
+  unsigned err = 0;
+
+  const int v_end = min(a.N, (tile + 1) * a.vox_per_tile);
+  for (int v = tile * a.vox_per_tile + tid; v < v_end; v += kBlock) {
+    const int4 co = a.coords[v];
+    double Lr[2][4];
+    int lens[2], nsegs[2], faces[2], lasts[2][3];
+    unsigned verr = 0;
+    // cube_face for both rays first (ray_tracing.c:206-207)
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const RayRule* rr = &rule->ray[k];
+      faces[k] = exit_face(co.x, co.y, co.z, rr->dir[0], rr->dir[1], rr->dir[2], a.Z, a.Y, a.X,
+                           rr->neg != 0);
+    }
+    if (faces[0] == 0 || faces[1] == 0) verr = ANACOR_RAY_ERR_FACE;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      Lr[k][0] = Lr[k][1] = Lr[k][2] = Lr[k][3] = 0.0;
+      lens[k] = 0; nsegs[k] = 0;
+      lasts[k][0] = lasts[k][1] = lasts[k][2] = 0;
+      if (verr) continue;
+      RayLane L;
+      if (!make_lane(&rule->ray[k], faces[k], co.x, co.y, co.z, a.Z, a.Y, a.X, L)) {
+        verr = ANACOR_RAY_ERR_ASSERT;
+        continue;
+      }
+      walk_ray<CAP, U, GUARD, FULL>(L, a.vol, a.Y, a.X, a.vs, seg, lens[k], nsegs[k], lasts[k],
+                                    Lr[k]);
+      steps += (unsigned long long)lens[k];
+    }
+    err |= verr;
+    if (PATHS) {
+      PathRecord& p = a.paths[(size_t)refl * a.N + co.w];
+      p.err = (int)verr;
+      p.pad = 0;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        p.face[k] = faces[k];
+        p.len[k] = lens[k];
+        p.nseg[k] = nsegs[k];
+        p.last[k][0] = lasts[k][0]; p.last[k][1] = lasts[k][1]; p.last[k][2] = lasts[k][2];
+        p.L[k][0] = Lr[k][0]; p.L[k][1] = Lr[k][1]; p.L[k][2] = Lr[k][2]; p.L[k][3] = Lr[k][3];
+      }
+      continue;
+    }
+    // cal_rate (ray_tracing_basic.c:1554-1583), same association order
+    const double r = a.mu[0] * (Lr[0][0] + Lr[1][0]) + a.mu[1] * (Lr[0][1] + Lr[1][1]) +
+                     a.mu[2] * (Lr[0][2] + Lr[1][2]) + a.mu[3] * (Lr[0][3] + Lr[1][3]);
+    if (F64) {
+      acc64 += a.is_exp ? exp(-r) : r;
+    } else {
+      const float val = a.is_exp ? exp_neg(r) : (float)r;
+      const float y = val - kc;
+      const float t = ks + y;
+      kc = (t - ks) - y;
+      ks = t;
+    }
+  }
+
+  // ---- workgroup reduction (wave shuffles, then one LDS round)
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    steps += __shfl_xor(steps, off);
+    err |= __shfl_xor(err, off);
+    if (!PATHS) {
+      if (F64) {
+        acc64 += __shfl_xor(acc64, off);
+      } else {
+        const float os = __shfl_xor(ks, off), oc = __shfl_xor(kc, off);
+        merge_pair(ks, kc, os, oc);
+      }
+    }
+  }
+  if (lane == 0) {
+    red_steps[wave] = steps;
+    if (F64) red64[wave] = acc64;
+    red32[0][wave] = ks;
+    red32[1][wave] = kc;
+    if (err) atomicOr(&a.err[refl], err);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long st = red_steps[0];
+    double s64 = red64[0];
+    float s = red32[0][0], c = red32[1][0];
+    for (int w = 1; w < kBlock / kWave; ++w) {
+      st += red_steps[w];
+      if (F64) s64 += red64[w];
+      merge_pair(s, c, red32[0][w], red32[1][w]);
+    }
+    atomicAdd(a.steps, st);
+    if (!PATHS) {
+      if (F64) a.partial64[lb] = s64;
+      else a.partial32[lb] = make_float2(s, c);
+    }
+  }
+}
+
+// Per reflection: combine the tile partials in tile order and divide by N.
+__global__ void finish_kernel(const float2* __restrict__ p32, const double* __restrict__ p64,
+                              const unsigned* __restrict__ err, int R, int tiles, int N, int f64,
+                              double* __restrict__ out) {
+  const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= R) return;
+  double sum = 0.0;
+  if (f64) {
+    for (int t = 0; t < tiles; ++t) sum += p64[(size_t)r * tiles + t];
+  } else {
+    float s = 0.f, c = 0.f;
+    for (int t = 0; t < tiles; ++t) {
+      const float2 q = p32[(size_t)r * tiles + t];
+      merge_pair(s, c, q.x, q.y);
+    }
+    sum = (double)s - (double)c;
+  }
+  out[r] = err[r] ? __builtin_nan("") : sum / (double)N;
+}
+
+// ------------------------------------------------------------ host side
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint8_t* d_vol = nullptr;
+  int Z = 0, Y = 0, X = 0;
+  int4* d_coords = nullptr;
+  int N = 0;
+  double vs[3] = {1, 1, 1};
+  double mu[4] = {0, 0, 0, 0};
+  int full = 0;
+  int tile_voxels = 0;          // 0: automatic
+  // reusable per-run buffers
+  ReflRule* d_rules = nullptr;
+  size_t rules_cap = 0;
+  void* d_partial = nullptr;
+  size_t partial_cap = 0;
+  unsigned* d_err = nullptr;
+  double* d_out = nullptr;
+  size_t refl_cap = 0;
+  unsigned long long* d_steps = nullptr;
+  std::vector<ReflRule> h_rules;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_kernel_ms = 0.f;
+  unsigned long long last_steps = 0;
+  int last_blocks = 0;
+  std::mutex mu_run;
+};
+
+static uint64_t spread_bits(uint64_t v) {   // 21 bits -> every third bit
+  v &= 0x1fffff;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+
+static int ensure_run_buffers(Context* c, int64_t R, int64_t n_partials, bool f64) {
+  if ((size_t)R > c->rules_cap) {
+    if (c->d_rules) HIP_TRY(hipFree(c->d_rules));
+    c->d_rules = nullptr;
+    HIP_TRY(hipMalloc(&c->d_rules, (size_t)R * sizeof(ReflRule)));
+    c->rules_cap = (size_t)R;
+  }
+  const size_t pbytes = (size_t)n_partials * (f64 ? sizeof(double) : sizeof(float2));
+  if (pbytes > c->partial_cap) {
+    if (c->d_partial) HIP_TRY(hipFree(c->d_partial));
+    c->d_partial = nullptr;
+    HIP_TRY(hipMalloc(&c->d_partial, pbytes));
+    c->partial_cap = pbytes;
+  }
+  if ((size_t)R > c->refl_cap) {
+    if (c->d_err) HIP_TRY(hipFree(c->d_err));
+    if (c->d_out) HIP_TRY(hipFree(c->d_out));
+    c->d_err = nullptr;
+    c->d_out = nullptr;
+    HIP_TRY(hipMalloc(&c->d_err, (size_t)R * sizeof(unsigned)));
+    HIP_TRY(hipMalloc(&c->d_out, (size_t)R * sizeof(double)));
+    c->refl_cap = (size_t)R;
+  }
+  if (!c->d_steps) HIP_TRY(hipMalloc(&c->d_steps, sizeof(unsigned long long)));
+  return ANACOR_OK;
+}
+
+// Voxel tile per workgroup: enough workgroups to fill 256 CUs several times
+// over, at least one voxel per lane, at most 64 voxels per lane.
+static int choose_tile(const Context* c, int64_t R) {
+  if (c->tile_voxels > 0) return c->tile_voxels;
+  const int64_t target_blocks = 256 * 16;
+  int64_t tiles = (target_blocks + R - 1) / std::max<int64_t>(R, 1);
+  tiles = std::max<int64_t>(1, std::min<int64_t>(tiles, (c->N + kBlock - 1) / kBlock));
+  int64_t vpt = (c->N + tiles - 1) / tiles;
+  vpt = std::min<int64_t>(std::max<int64_t>(vpt, kBlock), (int64_t)kBlock * 64);
+  return (int)vpt;
+}
+
+template <bool GUARD, bool FULL, bool F64, bool PATHS>
+static void launch_trace(const TraceArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((trace_kernel<16, 4, GUARD, FULL, F64, PATHS>), dim3(grid), dim3(kBlock), 0,
+                     s, a);
+}
+
+template <bool PATHS>
+static void dispatch_trace(const TraceArgs& a, int grid, bool guard, bool full, bool f64,
+                           hipStream_t s) {
+  if (guard) {
+    if (full) {
+      f64 ? launch_trace<true, true, true, PATHS>(a, grid, s)
+          : launch_trace<true, true, false, PATHS>(a, grid, s);
+    } else {
+      f64 ? launch_trace<true, false, true, PATHS>(a, grid, s)
+          : launch_trace<true, false, false, PATHS>(a, grid, s);
+    }
+  } else {
+    if (full) {
+      f64 ? launch_trace<false, true, true, PATHS>(a, grid, s)
+          : launch_trace<false, true, false, PATHS>(a, grid, s);
+    } else {
+      f64 ? launch_trace<false, false, true, PATHS>(a, grid, s)
+          : launch_trace<false, false, false, PATHS>(a, grid, s);
+    }
+  }
+}
+
+static int run_rules(Context* c, const ReflRule* h_rules, int64_t R, int flags, double* out,
+                     PathRecord* paths_out) {
+  const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
+  const bool paths = paths_out != nullptr;
+  if (R == 0) return ANACOR_OK;
+  if (c->N == 0) {
+    for (int64_t i = 0; i < R; ++i) out ? (void)(out[i] = __builtin_nan("")) : (void)0;
+    return ANACOR_OK;
+  }
+  const int vpt = paths ? kBlock : choose_tile(c, R);
+  const int64_t tiles = (c->N + vpt - 1) / vpt;
+  const int64_t nblk = R * tiles;
+  if (nblk > (int64_t)0x7fffffff - 8) {
+    set_error(ANACOR_ERR_ARG, "too many workgroups (%lld)", (long long)nblk);
+    return ANACOR_ERR_ARG;
+  }
+  int rc = ensure_run_buffers(c, R, nblk, f64);
+  if (rc) return rc;
+  bool guard = false;
+  for (int64_t i = 0; i < R; ++i)
+    guard = guard || h_rules[i].ray[0].guard || h_rules[i].ray[1].guard;
+
+  PathRecord* d_paths = nullptr;
+  if (paths) HIP_TRY(hipMalloc(&d_paths, (size_t)R * c->N * sizeof(PathRecord)));
+
+  HIP_TRY(hipMemcpyAsync(c->d_rules, h_rules, (size_t)R * sizeof(ReflRule),
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemsetAsync(c->d_err, 0, (size_t)R * sizeof(unsigned), c->stream));
+  HIP_TRY(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
+
+  TraceArgs a;
+  a.vol = c->d_vol;
+  a.Z = c->Z; a.Y = c->Y; a.X = c->X;
+  a.coords = c->d_coords;
+  a.N = c->N;
+  a.rules = c->d_rules;
+  a.R = (int)R;
+  a.tiles = (int)tiles;
+  a.vox_per_tile = vpt;
+  a.n_blocks = (int)nblk;
+  for (int i = 0; i < 3; ++i) a.vs[i] = c->vs[i];
+  for (int i = 0; i < 4; ++i) a.mu[i] = c->mu[i];
+  a.is_exp = (flags & ANACOR_FLAG_NO_EXP) ? 0 : 1;
+  a.partial32 = (float2*)c->d_partial;
+  a.partial64 = (double*)c->d_partial;
+  a.err = c->d_err;
+  a.steps = c->d_steps;
+  a.paths = d_paths;
+  const int grid = (int)((nblk + 7) / 8 * 8);
+
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  if (paths) dispatch_trace<true>(a, grid, guard, c->full != 0, f64, c->stream);
+  else dispatch_trace<false>(a, grid, guard, c->full != 0, f64, c->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  if (!paths) {
+    hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, c->stream,
+                       (const float2*)c->d_partial, (const double*)c->d_partial, c->d_err, (int)R,
+                       (int)tiles, c->N, f64 ? 1 : 0, c->d_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, c->d_out, (size_t)R * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  } else {
+    HIP_TRY(hipMemcpyAsync(paths_out, d_paths, (size_t)R * c->N * sizeof(PathRecord),
+                           hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(&c->last_steps, c->d_steps, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipEventElapsedTime(&c->last_kernel_ms, c->ev0, c->ev1));
+  c->last_blocks = (int)nblk;
+  if (d_paths) HIP_TRY(hipFree(d_paths));
+  std::vector<unsigned> herr;
+  if (!paths) {
+    // report the first failing reflection (results are already NaN there)
+    herr.resize((size_t)R);
+    HIP_TRY(hipMemcpy(herr.data(), c->d_err, (size_t)R * sizeof(unsigned),
+                      hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < R; ++i)
+      if (herr[i]) {
+        set_error(ANACOR_ERR_GEOMETRY,
+                  "reflection %lld: %s (the reference aborts here); result set to NaN",
+                  (long long)i,
+                  (herr[i] & ANACOR_RAY_ERR_FACE) ? "no exit face (cube_face)"
+                                                  : "exit face inconsistent with beam angles");
+        return ANACOR_ERR_GEOMETRY;
+      }
+  }
+  return ANACOR_OK;
+}
+
+static int ctx_create(const uint8_t* vol, const int64_t shape[3], const int64_t* coords,
+                      const int32_t* coords32, int64_t n, const double vs[3], const double mu[4],
+                      int full, int device, Context** out) {
+  *out = nullptr;
+  if (!vol || !shape) {
+    set_error(ANACOR_ERR_ARG, "null volume or shape");
+    return ANACOR_ERR_ARG;
+  }
+  for (int i = 0; i < 3; ++i)
+    if (shape[i] <= 0 || shape[i] > (1 << 20)) {
+      set_error(ANACOR_ERR_ARG, "bad volume shape %lld", (long long)shape[i]);
+      return ANACOR_ERR_ARG;
+    }
+  const uint64_t nvox = (uint64_t)shape[0] * shape[1] * shape[2];
+  if (nvox >= (1ull << 31)) {
+    set_error(ANACOR_ERR_ARG, "volume of %llu voxels exceeds 2^31", (unsigned long long)nvox);
+    return ANACOR_ERR_ARG;
+  }
+  if (n < 0 || n >= (1ll << 31) || (n > 0 && !coords && !coords32)) {
+    set_error(ANACOR_ERR_ARG, "bad voxel list (n=%lld)", (long long)n);
+    return ANACOR_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error(ANACOR_ERR_NO_DEVICE, "no HIP device visible");
+    return ANACOR_ERR_NO_DEVICE;
+  }
+  if (device < 0 || device >= ndev) device = 0;
+  Context* c = new Context();
+  c->device = device;
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&c->ev0));
+  HIP_TRY(hipEventCreate(&c->ev1));
+  c->Z = (int)shape[0]; c->Y = (int)shape[1]; c->X = (int)shape[2];
+  for (int i = 0; i < 3; ++i) c->vs[i] = vs ? vs[i] : 1.0;
+  for (int i = 0; i < 4; ++i) c->mu[i] = mu ? mu[i] : 0.0;
+  c->full = full;
+  c->N = (int)n;
+  HIP_TRY(hipMalloc(&c->d_vol, nvox));
+  HIP_TRY(hipMemcpy(c->d_vol, vol, nvox, hipMemcpyHostToDevice));
+  if (n > 0) {
+    // Morton order of (z, y, x): lanes of a wave start from neighbouring voxels
+    std::vector<int4> h((size_t)n);
+    std::vector<uint64_t> key((size_t)n);
+    std::vector<int32_t> idx((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t z = coords ? coords[3 * i] : coords32[3 * i];
+      const int64_t y = coords ? coords[3 * i + 1] : coords32[3 * i + 1];
+      const int64_t x = coords ? coords[3 * i + 2] : coords32[3 * i + 2];
+      if (z < INT32_MIN / 2 || z > INT32_MAX / 2 || y < INT32_MIN / 2 || y > INT32_MAX / 2 ||
+          x < INT32_MIN / 2 || x > INT32_MAX / 2) {
+        set_error(ANACOR_ERR_ARG, "voxel %lld coordinate out of range", (long long)i);
+        delete c;
+        return ANACOR_ERR_ARG;
+      }
+      h[(size_t)i] = make_int4((int)z, (int)y, (int)x, (int)i);
+      key[(size_t)i] = (spread_bits((uint64_t)z) << 2) | (spread_bits((uint64_t)y) << 1) |
+                       spread_bits((uint64_t)x);
+      idx[(size_t)i] = (int32_t)i;
+    }
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int32_t a, int32_t b) { return key[(size_t)a] < key[(size_t)b]; });
+    std::vector<int4> sorted((size_t)n);
+    for (int64_t i = 0; i < n; ++i) sorted[(size_t)i] = h[(size_t)idx[(size_t)i]];
+    HIP_TRY(hipMalloc(&c->d_coords, (size_t)n * sizeof(int4)));
+    HIP_TRY(hipMemcpy(c->d_coords, sorted.data(), (size_t)n * sizeof(int4),
+                      hipMemcpyHostToDevice));
+  }
+  *out = c;
+  return ANACOR_OK;
+}
+
+static void ctx_destroy(Context* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->d_vol) (void)hipFree(c->d_vol);
+  if (c->d_coords) (void)hipFree(c->d_coords);
+  if (c->d_rules) (void)hipFree(c->d_rules);
+  if (c->d_partial) (void)hipFree(c->d_partial);
+  if (c->d_err) (void)hipFree(c->d_err);
+  if (c->d_out) (void)hipFree(c->d_out);
+  if (c->d_steps) (void)hipFree(c->d_steps);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static int host_threads(int requested) {
+  if (requested > 0) return std::min(requested, 64);
+  const char* e = std::getenv("ANACOR_HOST_THREADS");
+  if (e && std::atoi(e) > 0) return std::min(std::atoi(e), 64);
+  return 8;
+}
+
+static int max_dim(const Context* c) { return std::max(c->Z, std::max(c->Y, c->X)); }
+
+static int ctx_run(Context* c, const double* s1, const double* omega, int64_t R,
+                   const double* raw_xray, const double* omega_axis, const double* F, int flags,
+                   int n_threads, double* out, PathRecord* paths) {
+  if (!c) {
+    set_error(ANACOR_ERR_ARG, "null context");
+    return ANACOR_ERR_ARG;
+  }
+  if (R < 0 || (R > 0 && (!s1 || !omega || !raw_xray || !omega_axis || !F))) {
+    set_error(ANACOR_ERR_ARG, "bad reflection arguments");
+    return ANACOR_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(c->mu_run);
+  HIP_TRY(hipSetDevice(c->device));
+  c->h_rules.resize((size_t)std::max<int64_t>(R, 1));
+  host_setup(s1, omega, R, raw_xray, omega_axis, F, max_dim(c), c->h_rules.data(), nullptr,
+             host_threads(n_threads));
+  return run_rules(c, c->h_rules.data(), R, flags, out, paths);
+}
+
+// Build a contiguous copy of an int8*** row-pointer table (python_2_c_3d,
+// utils/utils_os.py:45-68) unless its rows already are one block.
+static const uint8_t* flatten_table(int8_t*** table, const int64_t shape[3],
+                                    std::vector<uint8_t>& storage) {
+  const int64_t Z = shape[0], Y = shape[1], X = shape[2];
+  const int8_t* base = table[0][0];
+  bool contiguous = true;
+  for (int64_t z = 0; z < Z && contiguous; ++z)
+    for (int64_t y = 0; y < Y; ++y)
+      if (table[z][y] != base + (z * Y + y) * X) {
+        contiguous = false;
+        break;
+      }
+  if (contiguous) return reinterpret_cast<const uint8_t*>(base);
+  storage.resize((size_t)(Z * Y * X));
+  for (int64_t z = 0; z < Z; ++z)
+    for (int64_t y = 0; y < Y; ++y)
+      std::memcpy(&storage[(size_t)((z * Y + y) * X)], table[z][y], (size_t)X);
+  return storage.data();
+}
+
+}  // namespace anacor
+
+// ===================================================================== C-ABI
+
+using namespace anacor;
+
+struct anacor_ctx {
+  Context* c;
+};
+
+extern "C" {
+
+int anacor_last_error(char* buf, size_t n) {
+  if (buf && n) {
+    std::strncpy(buf, g_err, n - 1);
+    buf[n - 1] = 0;
+  }
+  return g_err_code.load();
+}
+
+int anacor_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+anacor_ctx* anacor_ctx_create(const uint8_t* volume, const int64_t shape[3],
+                              const int64_t* coords, int64_t n_coords,
+                              const double voxel_size[3], const double coefficients[4],
+                              int full_iteration, int device) {
+  clear_error();
+  Context* c = nullptr;
+  if (ctx_create(volume, shape, coords, nullptr, n_coords, voxel_size, coefficients,
+                 full_iteration, device, &c) != ANACOR_OK)
+    return nullptr;
+  return new anacor_ctx{c};
+}
+
+void anacor_ctx_destroy(anacor_ctx* ctx) {
+  if (!ctx) return;
+  ctx_destroy(ctx->c);
+  delete ctx;
+}
+
+int anacor_ctx_set_tile(anacor_ctx* ctx, int voxels_per_tile) {
+  if (!ctx) return ANACOR_ERR_ARG;
+  ctx->c->tile_voxels = voxels_per_tile > 0 ? voxels_per_tile : 0;
+  return ANACOR_OK;
+}
+
+int anacor_ctx_run(anacor_ctx* ctx, const double* s1, const double* omega, int64_t R,
+                   const double raw_xray[3], const double omega_axis[3], const double F[9],
+                   int flags, double* out) {
+  clear_error();
+  if (!ctx || (R > 0 && !out)) {
+    set_error(ANACOR_ERR_ARG, "null context or output");
+    return ANACOR_ERR_ARG;
+  }
+  return ctx_run(ctx->c, s1, omega, R, raw_xray, omega_axis, F, flags, 0, out, nullptr);
+}
+
+int anacor_ctx_paths(anacor_ctx* ctx, const double* s1, const double* omega, int64_t R,
+                     const double raw_xray[3], const double omega_axis[3], const double F[9],
+                     anacor_path_t* out) {
+  clear_error();
+  if (!ctx || (R > 0 && !out)) {
+    set_error(ANACOR_ERR_ARG, "null context or output");
+    return ANACOR_ERR_ARG;
+  }
+  static_assert(sizeof(anacor_path_t) == sizeof(PathRecord), "path record ABI");
+  return ctx_run(ctx->c, s1, omega, R, raw_xray, omega_axis, F, 0, 0, nullptr,
+                 reinterpret_cast<PathRecord*>(out));
+}
+
+int anacor_ctx_stats(const anacor_ctx* ctx, anacor_stats_t* st) {
+  if (!ctx || !st) return ANACOR_ERR_ARG;
+  st->kernel_ms = ctx->c->last_kernel_ms;
+  st->voxel_steps = ctx->c->last_steps;
+  st->workgroups = ctx->c->last_blocks;
+  st->n_voxels = ctx->c->N;
+  return ANACOR_OK;
+}
+
+int anacor_reflection_geometry(const double* s1, const double* omega, int64_t R,
+                               const double raw_xray[3], const double omega_axis[3],
+                               const double F[9], int64_t max_dim_, double* theta_phi,
+                               double* d_table) {
+  clear_error();
+  if (R < 0) return ANACOR_ERR_ARG;
+  std::vector<ReflRule> rules((size_t)std::max<int64_t>(R, 1));
+  std::vector<ReflGeom> geom((size_t)std::max<int64_t>(R, 1));
+  host_setup(s1, omega, R, raw_xray, omega_axis, F, max_dim_, rules.data(), geom.data(), 1);
+  for (int64_t i = 0; i < R; ++i) {
+    if (theta_phi) {
+      theta_phi[i * 4 + 0] = geom[(size_t)i].theta[0];
+      theta_phi[i * 4 + 1] = geom[(size_t)i].phi[0];
+      theta_phi[i * 4 + 2] = geom[(size_t)i].theta[1];
+      theta_phi[i * 4 + 3] = geom[(size_t)i].phi[1];
+    }
+    if (d_table)
+      for (int k = 0; k < 2; ++k)
+        for (int f = 0; f < 6; ++f)
+          for (int a = 0; a < 3; ++a)
+            d_table[((i * 2 + k) * 6 + f) * 3 + a] = rules[(size_t)i].ray[k].d[f][a];
+  }
+  return ANACOR_OK;
+}
+
+// ---- drop-in entry points with the reference's signatures
+
+double* ray_tracing_overall(int64_t low, int64_t up, int64_t* coord_list, int64_t len_coord_list,
+                            const double* scattering_vector_list, const double* omega_list,
+                            const double* raw_xray, const double* omega_axis,
+                            const double* kp_rotation_matrix, int64_t len_result,
+                            double* voxel_size, double* coefficients, int8_t*** label_list,
+                            int64_t* shape, int full_iteration, int store_paths, int num_workers,
+                            int IsExp) {
+  (void)low; (void)up; (void)store_paths;
+  clear_error();
+  const int64_t R = len_result > 0 ? len_result : 0;
+  double* result = (double*)std::malloc((size_t)std::max<int64_t>(R, 1) * sizeof(double));
+  if (!result) return nullptr;
+  for (int64_t i = 0; i < R; ++i) result[i] = __builtin_nan("");
+  if (!label_list || !shape) {
+    set_error(ANACOR_ERR_ARG, "null label_list or shape");
+    return result;
+  }
+  std::vector<uint8_t> storage;
+  const uint8_t* vol = flatten_table(label_list, shape, storage);
+  Context* c = nullptr;
+  if (ctx_create(vol, shape, coord_list, nullptr, len_coord_list, voxel_size, coefficients,
+                 full_iteration, 0, &c) != ANACOR_OK) {
+    fprintf(stderr, "anacor: ray_tracing_overall: %s\n", g_err);
+    return result;
+  }
+  const int flags = ((char)IsExp == 1) ? 0 : ANACOR_FLAG_NO_EXP;
+  const int rc = ctx_run(c, scattering_vector_list, omega_list, R, raw_xray, omega_axis,
+                         kp_rotation_matrix, flags, num_workers, result, nullptr);
+  if (rc != ANACOR_OK && rc != ANACOR_ERR_GEOMETRY)
+    fprintf(stderr, "anacor: ray_tracing_overall: %s\n", g_err);
+  ctx_destroy(c);
+  return result;
+}
+
+float* ray_tracing_gpu_overall(size_t low, size_t up, int* coord_list, size_t len_coord_list,
+                               const float* scattering_vector_list, const float* omega_list,
+                               const float* raw_xray, const float* omega_axis,
+                               const float* kp_rotation_matrix, size_t len_result,
+                               float* voxel_size, float* coefficients, int8_t*** label_list,
+                               int8_t* label_list_1d, int* shape, int full_iteration,
+                               int store_paths, int gpumethod) {
+  (void)low; (void)up; (void)store_paths; (void)gpumethod;
+  clear_error();
+  const int64_t R = (int64_t)len_result;
+  float* result = (float*)std::malloc((size_t)std::max<int64_t>(R, 1) * sizeof(float));
+  if (!result) return nullptr;
+  for (int64_t i = 0; i < R; ++i) result[i] = __builtin_nanf("");
+  if (!shape || (!label_list_1d && !label_list)) {
+    set_error(ANACOR_ERR_ARG, "null volume or shape");
+    return result;
+  }
+  const int64_t shp[3] = {shape[0], shape[1], shape[2]};
+  std::vector<uint8_t> storage;
+  const uint8_t* vol = label_list_1d ? reinterpret_cast<const uint8_t*>(label_list_1d)
+                                     : flatten_table(label_list, shp, storage);
+  // float inputs are widened to double and traced with the FP64 rules
+  std::vector<double> s1d((size_t)R * 3), omd((size_t)R);
+  for (int64_t i = 0; i < R * 3; ++i) s1d[(size_t)i] = scattering_vector_list[i];
+  for (int64_t i = 0; i < R; ++i) omd[(size_t)i] = omega_list[i];
+  double xr[3], ax[3], Fm[9], vs[3], mu[4];
+  for (int i = 0; i < 3; ++i) {
+    xr[i] = raw_xray[i];
+    ax[i] = omega_axis[i];
+    vs[i] = voxel_size[i];
+  }
+  for (int i = 0; i < 9; ++i) Fm[i] = kp_rotation_matrix[i];
+  for (int i = 0; i < 4; ++i) mu[i] = coefficients[i];
+  Context* c = nullptr;
+  if (ctx_create(vol, shp, nullptr, coord_list, (int64_t)len_coord_list, vs, mu, full_iteration,
+                 0, &c) != ANACOR_OK) {
+    fprintf(stderr, "anacor: ray_tracing_gpu_overall: %s\n", g_err);
+    return result;
+  }
+  std::vector<double> tmp((size_t)std::max<int64_t>(R, 1));
+  const int rc = ctx_run(c, s1d.data(), omd.data(), R, xr, ax, Fm, 0, 0, tmp.data(), nullptr);
+  if (rc == ANACOR_OK || rc == ANACOR_ERR_GEOMETRY)
+    for (int64_t i = 0; i < R; ++i) result[i] = (float)tmp[(size_t)i];
+  else
+    fprintf(stderr, "anacor: ray_tracing_gpu_overall: %s\n", g_err);
+  ctx_destroy(c);
+  return result;
+}
+
+static double single_impl(int64_t* coord_list, int64_t len_coord_list, const double* rotated_s1,
+                          const double* xray, double* voxel_size, double* coefficients,
+                          int8_t*** label_list, int64_t* shape, int full_iteration, int IsExp) {
+  clear_error();
+  if (!label_list || !shape || !rotated_s1 || !xray) {
+    set_error(ANACOR_ERR_ARG, "null argument");
+    return __builtin_nan("");
+  }
+  std::vector<uint8_t> storage;
+  const uint8_t* vol = flatten_table(label_list, shape, storage);
+  Context* c = nullptr;
+  if (ctx_create(vol, shape, coord_list, nullptr, len_coord_list, voxel_size, coefficients,
+                 full_iteration, 0, &c) != ANACOR_OK) {
+    fprintf(stderr, "anacor: ray_tracing_single: %s\n", g_err);
+    return __builtin_nan("");
+  }
+  ReflRule rule;
+  host_setup_rotated(rotated_s1, xray, max_dim(c), &rule, nullptr);
+  double out = __builtin_nan("");
+  const int flags = ((char)IsExp == 1) ? 0 : ANACOR_FLAG_NO_EXP;
+  {
+    std::lock_guard<std::mutex> lk(c->mu_run);
+    if (hipSetDevice(c->device) == hipSuccess) {
+      const int rc = run_rules(c, &rule, 1, flags, &out, nullptr);
+      if (rc != ANACOR_OK && rc != ANACOR_ERR_GEOMETRY)
+        fprintf(stderr, "anacor: ray_tracing_single: %s\n", g_err);
+    }
+  }
+  ctx_destroy(c);
+  return out;
+}
+
+double ray_tracing_single(int64_t* coord_list, int64_t len_coord_list, const double* rotated_s1,
+                          const double* xray, double* voxel_size, double* coefficients,
+                          int8_t*** label_list, int64_t* shape, int full_iteration,
+                          int64_t store_paths, int IsExp) {
+  (void)store_paths;
+  return single_impl(coord_list, len_coord_list, rotated_s1, xray, voxel_size, coefficients,
+                     label_list, shape, full_iteration, IsExp);
+}
+
+double ray_tracing_single_mp(int64_t* coord_list, int64_t len_coord_list,
+                             const double* rotated_s1, const double* xray, double* voxel_size,
+                             double* coefficients, int8_t*** label_list, int64_t* shape,
+                             int full_iteration, int64_t store_paths, int IsExp,
+                             int num_workers) {
+  (void)store_paths; (void)num_workers;
+  return single_impl(coord_list, len_coord_list, rotated_s1, xray, voxel_size, coefficients,
+                     label_list, shape, full_iteration, IsExp);
+}
+
+}  // extern "C"

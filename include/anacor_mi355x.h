@@ -1,0 +1,154 @@
+/*
+ * anacor_mi355x.h -- C-ABI of the MI355X (gfx950) absorption-correction ray
+ * tracer, libanacor_mi355x.so.
+ *
+ * The first four entry points keep the exact signatures of the reference's
+ * ctypes targets, so AnACor's Python drivers can load this library in place of
+ * src/ray_tracing_cpu.so / src/ray_tracing_gpu_{card}.so.  The rest is the
+ * library's own extended interface (resident volume, diagnostics).
+ *
+ * Error convention: no entry point aborts.  A reflection on which the
+ * reference would abort (cube_face exit(), cal_coord assert) yields NaN;
+ * anacor_last_error() then returns a nonzero code and a message.  Results of
+ * the drop-in entry points are allocated with libc malloc(), so callers free
+ * them with libc free() exactly as utils/utils_mp.py:212 does.
+ */
+#ifndef ANACOR_MI355X_H
+#define ANACOR_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define ANACOR_OK            0
+#define ANACOR_ERR_ARG       1   /* invalid argument */
+#define ANACOR_ERR_NO_DEVICE 2   /* no HIP device visible */
+#define ANACOR_ERR_HIP       3   /* HIP runtime failure */
+#define ANACOR_ERR_GEOMETRY  4   /* some reflection hit a reference abort case (result NaN) */
+
+/* per-ray error bits in anacor_path_t.err */
+#define ANACOR_RAY_ERR_FACE   1u  /* cube_face found no exit plane */
+#define ANACOR_RAY_ERR_ASSERT 2u  /* exit face inconsistent with the beam angles */
+
+/* flags of anacor_ctx_run */
+#define ANACOR_FLAG_F64_MEAN 1   /* FP64 exp and accumulation instead of FP32 exp + compensated sum */
+#define ANACOR_FLAG_NO_EXP   2   /* average sum(mu*L) instead of exp(-sum(mu*L)) (IsExp = 0) */
+
+/* ------------------------------------------------------------------ drop-in */
+
+/* Replaces ray_tracing_overall, AnACor/src/ray_tracing.c:365-374 (ctypes
+ * binding utils/utils_mp.py:59-79, call :197-204).  label_list is the int8
+ * row-pointer table of python_2_c_3d (utils/utils_os.py:45-68); low, up and
+ * store_paths are accepted and unused as in the reference; num_workers sets the
+ * host threads of the O(R) setup.  Returns malloc'd double[len_result]. */
+double *ray_tracing_overall(int64_t low, int64_t up, int64_t *coord_list, int64_t len_coord_list,
+                            const double *scattering_vector_list, const double *omega_list,
+                            const double *raw_xray, const double *omega_axis,
+                            const double *kp_rotation_matrix, int64_t len_result,
+                            double *voxel_size, double *coefficients, int8_t ***label_list,
+                            int64_t *shape, int full_iteration, int store_paths, int num_workers,
+                            int IsExp);
+
+/* Replaces ray_tracing_gpu_overall, AnACor/src/ray_tracing_gpu.c:33-40
+ * (ctypes binding utils/utils_mp.py:128-148, call :166-187).  float inputs are
+ * widened to double and traced with the same FP64 rules; label_list_1d is
+ * used when non-null; gpumethod is accepted and ignored.  Returns malloc'd
+ * float[len_result]. */
+float *ray_tracing_gpu_overall(size_t low, size_t up, int *coord_list, size_t len_coord_list,
+                               const float *scattering_vector_list, const float *omega_list,
+                               const float *raw_xray, const float *omega_axis,
+                               const float *kp_rotation_matrix, size_t len_result,
+                               float *voxel_size, float *coefficients, int8_t ***label_list,
+                               int8_t *label_list_1d, int *shape, int full_iteration,
+                               int store_paths, int gpumethod);
+
+/* Replaces ray_tracing_single, AnACor/src/ray_tracing.c:166-172 (ctypes
+ * binding utils/utils_mp.py:82-94): one reflection given already rotated
+ * vectors; returns the mean absorption factor. */
+double ray_tracing_single(int64_t *coord_list, int64_t len_coord_list, const double *rotated_s1,
+                          const double *xray, double *voxel_size, double *coefficients,
+                          int8_t ***label_list, int64_t *shape, int full_iteration,
+                          int64_t store_paths, int IsExp);
+
+/* Replaces ray_tracing_single_mp, AnACor/src/ray_tracing.c:294-300 (called by
+ * verification/main_verification.py:213-248); num_workers is ignored. */
+double ray_tracing_single_mp(int64_t *coord_list, int64_t len_coord_list,
+                             const double *rotated_s1, const double *xray, double *voxel_size,
+                             double *coefficients, int8_t ***label_list, int64_t *shape,
+                             int full_iteration, int64_t store_paths, int IsExp, int num_workers);
+
+/* ------------------------------------------------------- extended interface */
+
+/* Last error of the calling thread's most recent call: returns the code and
+ * copies the message into buf (no counterpart in the reference, which aborts;
+ * replaces the printf/exit of ray_tracing_basic.c:150-155). */
+int anacor_last_error(char *buf, size_t n);
+
+/* Number of visible HIP devices. */
+int anacor_device_count(void);
+
+/* A device-resident problem: the label volume [z][y][x] (uint8, contiguous)
+ * and the crystal-voxel list (int64 [n][3], z y x) are uploaded once;
+ * voxel_size (z, y, x) and coefficients [li, lo, cr, bu] as in
+ * ray_tracing_overall.  Returns NULL on failure (see anacor_last_error). */
+typedef struct anacor_ctx anacor_ctx;
+anacor_ctx *anacor_ctx_create(const uint8_t *volume, const int64_t shape[3],
+                              const int64_t *coords, int64_t n_coords,
+                              const double voxel_size[3], const double coefficients[4],
+                              int full_iteration, int device);
+void anacor_ctx_destroy(anacor_ctx *ctx);
+
+/* Voxels per workgroup tile (0 = automatic). */
+int anacor_ctx_set_tile(anacor_ctx *ctx, int voxels_per_tile);
+
+/* Mean absorption factor of R reflections (s1 [R][3], omega [R], DIALS frame)
+ * into out[R] (host memory).  The same computation as ray_tracing_overall on
+ * a resident volume. */
+int anacor_ctx_run(anacor_ctx *ctx, const double *s1, const double *omega, int64_t R,
+                   const double raw_xray[3], const double omega_axis[3], const double F[9],
+                   int flags, double *out);
+
+/* Per (reflection, voxel) path record, both rays ([0] incident, [1]
+ * diffracted): exit face, len_path_2, boundary count, last path coordinate
+ * (z, y, x) and the four lengths [li, lo, cr, bu] of cal_path2_plus. */
+typedef struct {
+    int32_t face[2];
+    int32_t len[2];
+    int32_t nseg[2];
+    int32_t err;
+    int32_t pad;
+    int32_t last[2][3];
+    double  L[2][4];
+} anacor_path_t;
+
+/* Path records of R reflections into out[R * n_coords] (voxels in the order
+ * given to anacor_ctx_create). */
+int anacor_ctx_paths(anacor_ctx *ctx, const double *s1, const double *omega, int64_t R,
+                     const double raw_xray[3], const double omega_axis[3], const double F[9],
+                     anacor_path_t *out);
+
+/* Statistics of the most recent run on ctx. */
+typedef struct {
+    double   kernel_ms;      /* trace kernel duration (HIP events on the ctx stream) */
+    uint64_t voxel_steps;    /* sum of len_path_2 over all rays */
+    int64_t  workgroups;     /* workgroups of the trace launch */
+    int64_t  n_voxels;
+} anacor_stats_t;
+int anacor_ctx_stats(const anacor_ctx *ctx, anacor_stats_t *stats);
+
+/* Host-side geometry of R reflections (diagnostics): theta_phi[R][4] =
+ * (theta, phi) of the incident then the diffracted ray; d_table[R][2][6][3] =
+ * per-face per-increment deltas (z, y, x).  Either output may be NULL. */
+int anacor_reflection_geometry(const double *s1, const double *omega, int64_t R,
+                               const double raw_xray[3], const double omega_axis[3],
+                               const double F[9], int64_t max_dim, double *theta_phi,
+                               double *d_table);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,516 @@
+/*
+ * anacor_oracle.c -- CPU restatement (FP64) of AnACor's analytical
+ * absorption-correction ray tracer.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Only tests/,
+ * __graft_entry__.smoke() and the CPU-baseline leg of bench.py load it.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fopenmp (oracle/Makefile).  No FMA
+ * contraction, glibc libm, so every FP64 operation rounds exactly as in the
+ * reference binary built with AnACor/src/Makefile:36,47-60 (gcc -O3, x86-64
+ * baseline ISA, which has no FMA instruction to contract into).
+ *
+ * Arithmetic details that matter for bit-exactness, all checked against the
+ * reference build (oracle/_ref) by tests/test_oracle_reference.py:
+ *   - pow(v, 2) in the reference is folded to v*v by gcc at -O3; glibc pow(v, 2)
+ *     differs from v*v in ~0.1% of inputs, so the restatement writes v*v.
+ *   - double -> int64 conversion of an out-of-range or NaN value yields
+ *     INT64_MIN on x86-64 (cvttsd2si); it is then clamped to 0.
+ *   - "x + -1*inc*r" (int64 -1*inc, then *r) equals x + inc*(-r) and
+ *     "y - inc*r" equals y + inc*(-r) in IEEE round-to-nearest, so every axis
+ *     is evaluated as  coord + (double)inc * d  with the sign folded into d.
+ *
+ * Pinning status (what checks each function besides the reference build):
+ *   oracle_kp_rotation   closed form (Rodrigues) + golden bits
+ *   oracle_thetaphi      golden bits from the reference build
+ *   oracle_myframe       closed form (permutation, signed-zero behaviour)
+ *   oracle_cube_face     brute-force slab test + golden faces
+ *   oracle_face_rule     golden per-ray paths (no closed form exists)
+ *   walk / lengths       golden per-ray (face, len, last coord, lengths);
+ *                        single-material cube closed form
+ *   oracle_trace         golden A_h bits; mu=0 => 1; analytic sphere/cylinder
+ *                        tables at percent level
+ */
+#include "anacor_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* The reference's own literal (ray_tracing.h:10). */
+#define ORACLE_PI 3.14159265358979323846
+
+/* ---------------------------------------------------------------- rotation */
+
+/* matrices.h:49-68 (Rodrigues rotation, same expression trees). */
+void oracle_kp_rotation(const double axis[3], double theta, double out[9])
+{
+    const double x = axis[0], y = axis[1], z = axis[2];
+    const double c = cos(theta);
+    const double s = sin(theta);
+    out[0] = c + (x * x) * (1 - c);
+    out[1] = x * y * (1 - c) - z * s;
+    out[2] = y * s + x * z * (1 - c);
+    out[3] = z * s + x * y * (1 - c);
+    out[4] = c + (y * y) * (1 - c);
+    out[5] = -x * s + y * z * (1 - c);
+    out[6] = -y * s + x * z * (1 - c);
+    out[7] = x * s + y * z * (1 - c);
+    out[8] = c + (z * z) * (1 - c);
+}
+
+/* matrices.h:14-27: C[m x p] = A[m x n] B[n x p], each entry accumulated from
+ * 0.0 in k order. */
+static void mat_mul(const double *A, const double *B, double *C, int m, int n, int p)
+{
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < p; j++) {
+            double acc = 0.0;
+            for (int k = 0; k < n; k++)
+                acc += A[i * n + k] * B[k * p + j];
+            C[i * p + j] = acc;
+        }
+}
+
+/* ------------------------------------------------------------------ angles */
+
+/* ray_tracing_basic.c:13-50 (dials_2_thetaphi_22).  The incident ray (L1) is
+ * negated first.  pow(v,2) is written v*v (see header comment). */
+void oracle_thetaphi(const double v_in[3], int64_t L1, double *theta, double *phi)
+{
+    double v[3] = {v_in[0], v_in[1], v_in[2]};
+    if (L1 == 1) {
+        v[0] = -v[0];
+        v[1] = -v[1];
+        v[2] = -v[2];
+    }
+    if (v[1] == 0) {
+        *theta = atan(-v[2] / (-sqrt(v[0] * v[0] + v[1] * v[1]) + 0.001));
+        *phi = atan(-v[0] / (v[1] + 0.001));
+    } else if (v[1] < 0) {
+        *theta = atan(-v[2] / sqrt(v[0] * v[0] + v[1] * v[1]));
+        *phi = atan(-v[0] / (v[1]));
+    } else {
+        const double a = atan(-v[2] / sqrt(v[0] * v[0] + v[1] * v[1]));
+        *theta = (v[2] < 0) ? ORACLE_PI - a : -ORACLE_PI - a;
+        *phi = -atan(-v[0] / (-v[1]));
+    }
+}
+
+/* ray_tracing_basic.c:52-67: permutation (v0, v2, v1) evaluated as a 3x3
+ * product accumulated from 0.0, which turns -0.0 into +0.0 and 0*inf into
+ * NaN exactly as the reference does. */
+void oracle_myframe(const double v[3], double out[3])
+{
+    static const double P[3][3] = {{1, 0, 0}, {0, 0, 1}, {0, 1, 0}};
+    for (int i = 0; i < 3; i++) {
+        double acc = 0.0;
+        for (int j = 0; j < 3; j++)
+            acc += P[i][j] * v[j];
+        out[i] = acc;
+    }
+}
+
+/* ------------------------------------------------------------- exit plane */
+
+/* ray_tracing_basic.c:69-156 (cube_face).  Slab distances to the six planes
+ * 0 and shape (not shape-1); negated for the incident ray; the smallest
+ * non-negative distance wins, ties resolved in the order
+ * x_min(6), x_max(1), y_min(4), y_max(5), z_min(2), z_max(3).
+ * Returns 0 where the reference calls exit(EXIT_FAILURE). */
+int64_t oracle_cube_face(const int64_t origin[3], const double dir[3],
+                         const int64_t shape[3], int L1)
+{
+    double t[6];
+    /* storage order of the reference's candidate list: x_min y_min z_min x_max y_max z_max */
+    t[0] = (0 - origin[2]) / dir[2];
+    t[1] = (0 - origin[1]) / dir[1];
+    t[2] = (0 - origin[0]) / dir[0];
+    t[3] = (shape[2] - origin[2]) / dir[2];
+    t[4] = (shape[1] - origin[1]) / dir[1];
+    t[5] = (shape[0] - origin[0]) / dir[0];
+    if (L1)
+        for (int i = 0; i < 6; i++)
+            t[i] = -t[i];
+
+    int have = 0;
+    double tmin = 0.0;
+    for (int i = 0; i < 6; i++) {
+        if (t[i] >= 0) {
+            if (!have || t[i] < tmin)
+                tmin = t[i];
+            have = 1;
+        }
+    }
+    if (!have)
+        return 0; /* reference reads an uninitialised value and exits */
+    if (tmin == t[0]) return 6;
+    if (tmin == t[3]) return 1;
+    if (tmin == t[1]) return 4;
+    if (tmin == t[4]) return 5;
+    if (tmin == t[2]) return 2;
+    if (tmin == t[5]) return 3;
+    return 0;
+}
+
+/* ------------------------------------------------------------ face rules */
+
+/* The per-face stepping rules of cal_coord, ray_tracing_basic.c:418-1377.
+ * Each axis advances as coord + inc*d (d carries the reference's sign), the
+ * rounding is floor or round-half-away, and the loop runs to the exit plane
+ * with the reference's inclusive/exclusive bound. */
+void oracle_face_rule(double theta, double phi, int64_t face, oracle_rule_t *r)
+{
+    const double half_pi = ORACLE_PI / 2;
+    const int narrow = fabs(theta) < half_pi;   /* |theta| < pi/2 */
+    memset(r, 0, sizeof(*r));
+    r->vacuum_run = 50;                          /* va_threshold, :12 */
+    r->rounding = (theta > 0) ? 0 : 1;           /* floor when theta > 0 */
+    switch (face) {
+    case 6: { /* x -> 0, :418-511 */
+        const double ry = tan(theta) / cos(phi);
+        const double rz = tan(phi);
+        r->valid = fabs(theta) <= half_pi;
+        r->d[0] = rz; r->d[1] = -ry; r->d[2] = -1.0;
+        r->axis = 2; r->to_max = 0; r->inclusive = 1;
+        break;
+    }
+    case 1: { /* x -> x_max, :1280-1377 */
+        const double ry = tan(ORACLE_PI - theta) / cos(fabs(phi));
+        const double rz = tan(phi);
+        r->valid = fabs(theta) >= half_pi;
+        r->d[0] = -rz; r->d[1] = -ry; r->d[2] = 1.0;
+        r->axis = 2; r->to_max = 1; r->inclusive = 0;
+        r->vacuum_run = 10;                      /* :1352 */
+        break;
+    }
+    case 2: { /* z -> 0, :512-710 */
+        const double rx = 1 / tan(fabs(phi));
+        if (narrow) {
+            const double ry = tan(theta) / sin(fabs(phi));
+            r->valid = phi < 0;
+            r->d[0] = -1.0; r->d[1] = -ry; r->d[2] = -rx;
+        } else {
+            const double ry = tan(ORACLE_PI - theta) / sin(fabs(phi));
+            r->valid = phi > 0;
+            r->d[0] = -1.0; r->d[1] = -ry; r->d[2] = rx;
+        }
+        r->axis = 0; r->to_max = 0; r->inclusive = 1;
+        break;
+    }
+    case 3: { /* z -> z_max, :712-897 */
+        const double rx = 1 / tan(fabs(phi));
+        if (narrow) {
+            const double ry = tan(theta) / sin(fabs(phi));
+            r->valid = phi > 0;
+            r->d[0] = 1.0; r->d[1] = -ry; r->d[2] = -rx;
+            r->inclusive = 0;
+        } else {
+            const double ry = tan(ORACLE_PI - theta) / sin(fabs(phi));
+            r->valid = phi < 0;
+            r->d[0] = 1.0; r->d[1] = -ry; r->d[2] = rx;
+            r->inclusive = 1;
+        }
+        r->axis = 0; r->to_max = 1;
+        break;
+    }
+    case 4: { /* y -> 0, always floor, :899-1102 */
+        r->valid = theta > 0;
+        r->rounding = 0;
+        if (narrow) {
+            const double rx = cos(fabs(phi)) / tan(fabs(theta));
+            const double rz = sin(phi) / tan(fabs(theta));
+            r->d[0] = rz; r->d[1] = -1.0; r->d[2] = -rx;
+        } else {
+            const double rx = cos(fabs(phi)) / (tan((ORACLE_PI - fabs(theta))));
+            const double rz = sin(-phi) / (tan((ORACLE_PI - fabs(theta))));
+            r->d[0] = rz; r->d[1] = -1.0; r->d[2] = rx;
+        }
+        r->axis = 1; r->to_max = 0; r->inclusive = 1;
+        break;
+    }
+    case 5: { /* y -> y_max, always round, :1104-1278 */
+        r->valid = theta < 0;
+        r->rounding = 1;
+        if (narrow) {
+            const double rx = cos(fabs(phi)) / (tan(fabs(theta)));
+            const double rz = sin(phi) / (tan(fabs(theta)));
+            r->d[0] = rz; r->d[1] = 1.0; r->d[2] = -rx;
+            r->inclusive = 0;
+        } else {
+            const double rx = cos(fabs(phi)) / (tan(ORACLE_PI - fabs(theta)));
+            const double rz = sin(phi) / (tan(ORACLE_PI - fabs(theta)));
+            r->d[0] = -rz; r->d[1] = 1.0; r->d[2] = rx;
+            r->inclusive = 1;
+        }
+        r->axis = 1; r->to_max = 1;
+        break;
+    }
+    default:
+        r->valid = 0;
+        break;
+    }
+}
+
+/* ------------------------------------------------------------------- walk */
+
+/* double -> int64 as x86-64 cvttsd2si converts it (INT64_MIN when the value
+ * does not fit), then the clamp to [0, max-1] of every cal_coord branch. */
+static inline int64_t clamp_index(double w, int64_t max)
+{
+    int64_t v;
+    if (!(w >= -0x1p63 && w < 0x1p63))
+        v = INT64_MIN;
+    else
+        v = (int64_t)w;
+    if (v >= max)
+        v = max - 1;
+    else if (v < 0)
+        v = 0;
+    return v;
+}
+
+typedef struct {
+    int64_t *posi;
+    int8_t *cls;
+    int64_t cap;
+} seg_buf;
+
+static int seg_reserve(seg_buf *b, int64_t need)
+{
+    if (need <= b->cap)
+        return 1;
+    int64_t cap = b->cap ? b->cap : 64;
+    while (cap < need)
+        cap *= 2;
+    int64_t *p = (int64_t *)realloc(b->posi, (size_t)cap * sizeof(int64_t));
+    if (!p) return 0;
+    b->posi = p;
+    int8_t *c = (int8_t *)realloc(b->cls, (size_t)cap);
+    if (!c) return 0;
+    b->cls = c;
+    b->cap = cap;
+    return 1;
+}
+
+/* One ray: cal_coord (ray_tracing_basic.c:396-1460, with appending :343-394)
+ * followed by cal_path2_plus (:1462-1552).  Writes the four lengths. */
+static void trace_ray(const int64_t c0[3], const oracle_rule_t *rule, int64_t face,
+                      const int8_t *vol, const int64_t shape[3],
+                      const double *voxel_size, int full_iteration,
+                      seg_buf *segs, oracle_ray_t *out)
+{
+    const int64_t Z = shape[0], Y = shape[1], X = shape[2];
+    const int64_t ext[3] = {Z, Y, X};
+    const int64_t ax = rule->axis;
+    const int64_t limit = rule->to_max ? (ext[ax] - c0[ax]) + rule->inclusive
+                                       : c0[ax] + 1;
+    const double base[3] = {(double)c0[0], (double)c0[1], (double)c0[2]};
+
+    int64_t len = 1, nseg = 1, va = 0;
+    int64_t first[3] = {0, 0, 0}, last[3] = {0, 0, 0};
+    int prev = 0;
+
+    seg_reserve(segs, limit > 0 ? limit + 2 : 2);
+    segs->posi[0] = 0;      /* classes_posi[0] = 0, :407-408 */
+    segs->cls[0] = 3;       /* classes[0] = 3 */
+
+    for (int64_t inc = 0; inc < limit; inc++) {
+        int64_t c[3];
+        for (int a = 0; a < 3; a++) {
+            double w = base[a] + (double)inc * rule->d[a];
+            w = rule->rounding ? round(w) : floor(w);
+            c[a] = clamp_index(w, ext[a]);
+        }
+        const int lab = (int)vol[(c[0] * Y + c[1]) * X + c[2]];
+        if (!full_iteration) {
+            if (lab == 0)
+                break;
+        } else {
+            va = (lab == 0) ? va + 1 : 0;
+            if (va > rule->vacuum_run)
+                break;
+        }
+        if (inc == 0) {
+            first[0] = c[0]; first[1] = c[1]; first[2] = c[2];
+            last[0] = c[0]; last[1] = c[1]; last[2] = c[2];
+            prev = lab;
+            continue;
+        }
+        if (lab != prev && lab >= 0 && lab <= 4) {
+            segs->posi[nseg] = inc;
+            segs->cls[nseg] = (int8_t)lab;
+            nseg++;
+        }
+        last[0] = c[0]; last[1] = c[1]; last[2] = c[2];
+        prev = lab;
+        len++;
+    }
+
+    double L[4] = {0.0, 0.0, 0.0, 0.0}; /* li, lo, cr, bu */
+    if (len != 1) {
+        const double dy = (double)(last[1] - first[1]) * voxel_size[1];
+        const double dz = (double)(last[0] - first[0]) * voxel_size[0];
+        const double dx = (double)(last[2] - first[2]) * voxel_size[2];
+        const double T = sqrt(dy * dy + dz * dz + dx * dx);
+        for (int64_t j = 0; j < nseg; j++) {
+            const int64_t span = (j < nseg - 1) ? segs->posi[j + 1] - segs->posi[j]
+                                                : len - segs->posi[j];
+            const double term = T * ((double)span / (double)(len - 1));
+            switch (segs->cls[j]) {
+            case 1: L[0] += term; break;
+            case 2: L[1] += term; break;
+            case 3: L[2] += term; break;
+            case 4: L[3] += term; break;
+            default: break;
+            }
+        }
+    }
+    if (out) {
+        out->face = (int32_t)face;
+        out->len = len;
+        out->nseg = nseg;
+        memcpy(out->first, first, sizeof(first));
+        memcpy(out->last, last, sizeof(last));
+        memcpy(out->L, L, sizeof(L));
+    }
+}
+
+/* cal_rate, ray_tracing_basic.c:1554-1583 (same association order). */
+static double rate(const double *n1, const double *n2, const double *mu, int is_exp)
+{
+    double r = (mu[0] * (n1[0] + n2[0]) +
+                mu[1] * (n1[1] + n2[1]) +
+                mu[2] * (n1[2] + n2[2]) +
+                mu[3] * (n1[3] + n2[3]));
+    if ((char)is_exp == 1)
+        r = exp(-r);
+    return r;
+}
+
+/* ray_tracing_single, ray_tracing.c:166-292, on already computed angles and
+ * directions.  Returns the mean; *err receives the first error code. */
+static double single_core(const int64_t *coords, int64_t n, const double theta[2],
+                          const double phi[2], const double dir[2][3],
+                          const double *voxel_size, const double *mu,
+                          const int8_t *vol, const int64_t shape[3],
+                          int full_iteration, int is_exp, seg_buf *segs,
+                          oracle_ray_t *rays, int64_t *steps, int *err)
+{
+    double sum = 0.0;
+    int64_t st = 0;
+    *err = ORACLE_OK;
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t c[3] = {coords[i * 3], coords[i * 3 + 1], coords[i * 3 + 2]};
+        oracle_ray_t rr[2];
+        memset(rr, 0, sizeof(rr));
+        /* cube_face for both rays runs before either walk (:206-207) */
+        const int64_t face[2] = {oracle_cube_face(c, dir[0], shape, 1),
+                                 oracle_cube_face(c, dir[1], shape, 0)};
+        if (face[0] == 0 || face[1] == 0) {
+            *err = ORACLE_ERR_FACE;
+            if (rays) {
+                rays[i * 2].err = rays[i * 2 + 1].err = ORACLE_ERR_FACE;
+            }
+            return NAN;
+        }
+        for (int k = 0; k < 2; k++) {
+            oracle_rule_t rule;
+            oracle_face_rule(theta[k], phi[k], face[k], &rule);
+            if (!rule.valid) {
+                *err = ORACLE_ERR_ASSERT;
+                if (rays) rays[i * 2 + k].err = ORACLE_ERR_ASSERT;
+                return NAN;
+            }
+            trace_ray(c, &rule, face[k], vol, shape, voxel_size, full_iteration, segs, &rr[k]);
+            st += rr[k].len;
+        }
+        sum += rate(rr[0].L, rr[1].L, mu, is_exp);
+        if (rays) {
+            rays[i * 2] = rr[0];
+            rays[i * 2 + 1] = rr[1];
+        }
+    }
+    if (steps) *steps = st;
+    return sum / n;
+}
+
+/* ray_tracing.c:393-407 (rotation) then :182-197 (angles, directions). */
+static void setup_one(const double s1_in[3], double omega, const double *raw_xray,
+                      const double *omega_axis, const double *F, oracle_refl_t *o)
+{
+    double Rw[9], M[9], T[9];
+    oracle_kp_rotation(omega_axis, omega, Rw);
+    mat_mul(Rw, F, M, 3, 3, 3);
+    for (int i = 0; i < 3; i++)          /* transpose, matrices.h:6-12 */
+        for (int j = 0; j < 3; j++)
+            T[j * 3 + i] = M[i * 3 + j];
+    mat_mul(T, raw_xray, o->xray, 3, 3, 1);
+    mat_mul(T, s1_in, o->s1, 3, 3, 1);
+    oracle_thetaphi(o->xray, 1, &o->theta[0], &o->phi[0]);
+    oracle_thetaphi(o->s1, 0, &o->theta[1], &o->phi[1]);
+    oracle_myframe(o->xray, o->dir[0]);
+    oracle_myframe(o->s1, o->dir[1]);
+}
+
+void oracle_reflection_setup(const double *s1, const double *omega, int64_t R,
+                             const double *raw_xray, const double *omega_axis,
+                             const double *F, oracle_refl_t *out)
+{
+    for (int64_t i = 0; i < R; i++)
+        setup_one(&s1[i * 3], omega[i], raw_xray, omega_axis, F, &out[i]);
+}
+
+int64_t oracle_trace(const int64_t *coords, int64_t n_coords,
+                     const double *s1, const double *omega, int64_t R,
+                     const double *raw_xray, const double *omega_axis, const double *F,
+                     const double *voxel_size, const double *coefficients,
+                     const uint8_t *volume, const int64_t shape[3],
+                     int full_iteration, int is_exp, int n_threads,
+                     double *result, oracle_ray_t *rays, int64_t *steps)
+{
+    int64_t failed = 0, total = 0;
+    if (n_threads < 1) n_threads = 1;
+#pragma omp parallel num_threads(n_threads) reduction(+ : failed, total)
+    {
+        seg_buf segs = {NULL, NULL, 0};
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t i = 0; i < R; i++) {
+            oracle_refl_t g;
+            setup_one(&s1[i * 3], omega[i], raw_xray, omega_axis, F, &g);
+            int err = 0;
+            int64_t st = 0;
+            result[i] = single_core(coords, n_coords, g.theta, g.phi,
+                                    (const double(*)[3])g.dir, voxel_size, coefficients,
+                                    (const int8_t *)volume, shape, full_iteration, is_exp,
+                                    &segs, rays ? rays + i * n_coords * 2 : NULL, &st, &err);
+            if (err) failed++;
+            total += st;
+        }
+        free(segs.posi);
+        free(segs.cls);
+    }
+    if (steps) *steps = total;
+    return failed;
+}
+
+double oracle_single(const int64_t *coords, int64_t n_coords,
+                     const double rotated_s1[3], const double xray[3],
+                     const double *voxel_size, const double *coefficients,
+                     const uint8_t *volume, const int64_t shape[3],
+                     int full_iteration, int is_exp, oracle_ray_t *rays, int64_t *steps)
+{
+    double theta[2], phi[2], dir[2][3];
+    oracle_thetaphi(xray, 1, &theta[0], &phi[0]);
+    oracle_thetaphi(rotated_s1, 0, &theta[1], &phi[1]);
+    oracle_myframe(xray, dir[0]);
+    oracle_myframe(rotated_s1, dir[1]);
+    seg_buf segs = {NULL, NULL, 0};
+    int err = 0;
+    double m = single_core(coords, n_coords, theta, phi, (const double(*)[3])dir,
+                           voxel_size, coefficients, (const int8_t *)volume, shape,
+                           full_iteration, is_exp, &segs, rays, steps, &err);
+    free(segs.posi);
+    free(segs.cls);
+    return m;
+}
