@@ -137,18 +137,21 @@ void face_rules(double t, double p, RayRule& r) {
   r.meta[3] = pack_meta(t > 0, false, true, false, 1, 50);
 }
 
-// True when some clamped coordinate could come from a value outside the
-// int64 range (or a non-finite one), which the kernels then map to 0 as the
-// reference's double->int64 conversion does.
+// True when, on some face the ray may walk (its precondition holds), a
+// clamped coordinate could come from a value outside the int64 range (or a
+// non-finite one), which the kernels then map to 0 as the reference's
+// double->int64 conversion does.  Faces whose precondition fails never walk.
 bool needs_guard(const RayRule& r, int64_t max_dim) {
   const double lim = 0x1p62;
   const double m = double(max_dim) + 1.0;
-  for (int f = 0; f < 6; ++f)
+  for (int f = 0; f < 6; ++f) {
+    if (!(r.meta[f] & kMetaValid)) continue;
     for (int a = 0; a < 3; ++a) {
       const double d = r.d[f][a];
       if (!std::isfinite(d)) return true;
       if (std::fabs(d) * m + m >= lim) return true;
     }
+  }
   return false;
 }
 

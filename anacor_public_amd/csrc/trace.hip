@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <utility>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
@@ -138,86 +140,147 @@ __device__ __forceinline__ int exit_face(int z, int y, int x, double dz, double 
   return 0;
 }
 
-// One axis of one step: clamp(rnd(base + inc*d)) as an integer-valued double.
-template <bool GUARD>
-__device__ __forceinline__ double axis_step(double base, double incd, double d, bool rnd,
-                                            double hi) {
-  const double w = base + incd * d;
-  double r = __builtin_floor(w);
-  if (rnd && (w - r >= 0.5)) r += 1.0;        // round-half-away for w >= -0.5
-  if (GUARD && !(w < 0x1p63)) r = -1.0;       // x86 double->int64 overflow -> 0
-  r = __builtin_fmax(r, 0.0);                 // NaN -> 0 as well
-  r = __builtin_fmin(r, hi);
+// ---- one step of a ray
+//
+// The reference evaluates each axis as  idx = clamp(rnd(c + inc*d), 0, n-1)
+// with rnd = floor or round-half-away and a double->int64 conversion that
+// maps values outside int64 to INT64_MIN (then clamped to 0).  Here:
+//   * the exit face's own axis has d = +-1, so it is an exact integer walk;
+//   * for the other two axes, w = c + inc*d is evaluated exactly as the
+//     reference does (separately rounded multiply and add), then
+//       rnd(w) clamped == clamp(trunc_sat(w + bias)),
+//     bias = 0 for floor and 0.5 - 2^-54 for round-half-away: negative values
+//     clamp to 0 either way, and for w >= 0, floor(w + (0.5 - 2^-54)) equals
+//     round-half-away(w) for every double (0.5 itself would round
+//     0.49999999999999994 up).  v_cvt_i32_f64 truncates, saturates and maps
+//     NaN to 0; values >= 2^63 (only possible when the host sets GUARD) are
+//     mapped to 0 as the reference's conversion does.
+constexpr double kRoundBias = 0x1.fffffffffffffp-2;  // 0.5 - 2^-54
+
+__device__ __forceinline__ int cvt_trunc_sat(double x) {
+  int r;
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(x));
   return r;
 }
 
+// median(v, 0, hi) == clamp to [0, hi] for hi >= 0, one v_med3_i32
+__device__ __forceinline__ int clamp_int(int v, int hi) {
+  int r;
+  asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(v), "v"(hi));
+  return r;
+}
+
+template <bool GUARD>
+__device__ __forceinline__ int axis_index(double base, double incd, double d, double bias,
+                                          int hi) {
+  const double w = base + incd * d;
+  int v = cvt_trunc_sat(w + bias);
+  if (GUARD && !(w < 0x1p63)) v = 0;
+  return clamp_int(v, hi);
+}
+
+// A ray of one lane, with its axes permuted so that `p` is the exit face's
+// axis and a, b are the other two (in z, y, x order).
 struct RayLane {
-  double bz, by, bx;     // origin
-  double dz, dy, dx;     // per-increment deltas
-  double hz, hy, hx;     // extent - 1
-  int iz, iy, ix;        // extent - 1
-  bool rnd;
+  double ba, bb;         // origins of the stepped axes a, b
+  double da, db;         // per-increment deltas of a, b
+  double bias;           // 0 (floor) or kRoundBias (round-half-away)
+  int sa, sb, sp;        // strides (elements) of a, b, p in the volume
+  int ha, hb, hp;        // extent - 1 of a, b, p
+  int p0, pdir;          // origin and direction (+1/-1) of the face axis
+  int paxis;             // the face axis: 0 z, 1 y, 2 x
   int limit;             // increments tried: inc in [0, limit)
   int vac;               // full_iteration vacuum-run limit
 };
 
-__device__ __forceinline__ int clamp_int(int v, int hi) { return min(max(v, 0), hi); }
-
 template <bool GUARD>
-__device__ __forceinline__ void coord_at(const RayLane& L, int inc, int& z, int& y, int& x) {
+__device__ __forceinline__ void lane_coords(const RayLane& L, int inc, int& ia, int& ib, int& ip) {
   const double incd = (double)inc;
-  // the integer clamp keeps every gather in bounds whatever the conversion
-  // of a non-finite value yields
-  z = clamp_int((int)axis_step<GUARD>(L.bz, incd, L.dz, L.rnd, L.hz), L.iz);
-  y = clamp_int((int)axis_step<GUARD>(L.by, incd, L.dy, L.rnd, L.hy), L.iy);
-  x = clamp_int((int)axis_step<GUARD>(L.bx, incd, L.dx, L.rnd, L.hx), L.ix);
+  ia = axis_index<GUARD>(L.ba, incd, L.da, L.bias, L.ha);
+  ib = axis_index<GUARD>(L.bb, incd, L.db, L.bias, L.hb);
+  ip = clamp_int(L.p0 + L.pdir * inc, L.hp);
 }
 
-// Walk one ray (cal_coord + cal_path2_plus).  Segments beyond the implicit
-// first one (class 3 at 0) go to this lane's column of `seg` (CAP entries,
-// lane-interleaved); `len` is len_path_2; L receives [li, lo, cr, bu].
-template <int CAP, int U, bool GUARD, bool FULL>
-__device__ __forceinline__ void walk_ray(const RayLane& L, const uint8_t* __restrict__ vol,
-                                         int Y, int X, const double vs[3], uint32_t* seg,
-                                         int& len_out, int& nseg_out, int last_xyz[3],
-                                         double out[4]) {
+template <bool WIDE>
+__device__ __forceinline__ int lane_offset(const RayLane& L, int ia, int ib, int ip) {
+  if (WIDE) return ia * L.sa + ib * L.sb + ip * L.sp;
+  // all factors < 2^24 (checked on the host): full-rate 24-bit multiplies
+  return __mul24(ia, L.sa) + __mul24(ib, L.sb) +
+         __mul24(ip, L.sp);
+}
+
+template <bool GUARD>
+__device__ __forceinline__ void lane_zyx(const RayLane& L, int inc, int& z, int& y, int& x) {
+  int ia, ib, ip;
+  lane_coords<GUARD>(L, inc, ia, ib, ip);
+  if (L.paxis == 0) { z = ip; y = ia; x = ib; }
+  else if (L.paxis == 1) { z = ia; y = ip; x = ib; }
+  else { z = ia; y = ib; x = ip; }
+}
+
+struct Volume {
+  __amdgpu_buffer_rsrc_t rsrc;  // range-checked: offsets past the end read 0
+  int Z, Y, X;
+};
+
+__device__ __forceinline__ int load_label(const Volume& V, int off) {
+  return __builtin_amdgcn_raw_buffer_load_b8(V.rsrc, off, 0, 0);
+}
+
+// Walk one ray (cal_coord + cal_path2_plus).  Boundaries beyond the implicit
+// first one (class 3 at increment 0) are kept in this lane's column of `seg`
+// (CAP entries, lane-interleaved, plus one scratch slot so the store needs no
+// branch); len is len_path_2; out receives [li, lo, cr, bu].
+template <int CAP, int U, bool GUARD, bool FULL, bool WIDE>
+__device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, const double vs[3],
+                                         uint32_t* seg, int& len_out, int& nseg_out,
+                                         int last_zyx[3], double out[4]) {
   int last = -1, prev = 0, nseg = 0, va = 0;
   bool done = L.limit <= 0;
-  for (int inc0 = 0; !done; inc0 += U) {
+  if (!done) {  // increment 0: its label opens no boundary
+    int ia, ib, ip;
+    lane_coords<GUARD>(L, 0, ia, ib, ip);
+    const int lab = load_label(V, lane_offset<WIDE>(L, ia, ib, ip));
+    bool stop;
+    if (FULL) {
+      va = (lab == 0) ? 1 : 0;
+      stop = va > L.vac;
+    } else {
+      stop = lab == 0;
+    }
+    if (stop) done = true;
+    else { prev = lab; last = 0; }
+  }
+  for (int inc0 = 1; !done; inc0 += U) {
     int lab[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      int z, y, x;
-      coord_at<GUARD>(L, inc0 + u, z, y, x);      // always in bounds
-      lab[u] = vol[(size_t)((z * Y + y) * X + x)];
+    for (int u = 0; u < U; ++u) {  // all offsets are in range (clamped)
+      int ia, ib, ip;
+      lane_coords<GUARD>(L, inc0 + u, ia, ib, ip);
+      lab[u] = load_label(V, lane_offset<WIDE>(L, ia, ib, ip));
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int inc = inc0 + u;
-      if (!done) {
-        if (inc >= L.limit) {
-          done = true;
-        } else {
-          const int lb = (int)(int8_t)lab[u];
-          bool stop;
-          if (FULL) {
-            va = (lb == 0) ? va + 1 : 0;
-            stop = va > L.vac;
-          } else {
-            stop = lb == 0;
-          }
-          if (stop) {
-            done = true;
-          } else {
-            if (inc > 0 && lb != prev && (unsigned)lb <= 4u) {
-              if (nseg < CAP) seg[nseg * kBlock] = ((uint32_t)inc << 3) | (uint32_t)lb;
-              ++nseg;
-            }
-            prev = lb;
-            last = inc;
-          }
-        }
+      const int lb = lab[u];
+      const bool act = !done && inc < L.limit;
+      bool stop;
+      if (FULL) {
+        const int va_n = (lb == 0) ? va + 1 : 0;
+        va = act ? va_n : va;
+        stop = va_n > L.vac;
+      } else {
+        stop = lb == 0;
       }
+      const bool take = act && !stop;
+      // int8 labels: only values 0..4 open a boundary (appending, :343-394)
+      const bool chg = take && lb != prev && (unsigned)lb <= 4u;
+      const int slot = (chg && nseg < CAP) ? nseg : CAP;
+      seg[slot * kBlock] = ((uint32_t)inc << 3) | (uint32_t)lb;
+      nseg += chg ? 1 : 0;
+      prev = take ? lb : prev;
+      last = take ? inc : last;
+      done = done || !take;
     }
   }
 
@@ -225,19 +288,14 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const uint8_t* __rest
   len_out = len;
   nseg_out = nseg + 1;
   out[0] = out[1] = out[2] = out[3] = 0.0;
-  last_xyz[0] = last_xyz[1] = last_xyz[2] = 0;
-  if (len == 1) {
-    if (last == 0) {
-      int z, y, x;
-      coord_at<GUARD>(L, 0, z, y, x);
-      last_xyz[0] = z; last_xyz[1] = y; last_xyz[2] = x;
-    }
-    return;
-  }
+  last_zyx[0] = last_zyx[1] = last_zyx[2] = 0;
+  if (last < 0) return;
   int fz, fy, fx, lz, ly, lx;
-  coord_at<GUARD>(L, 0, fz, fy, fx);
-  coord_at<GUARD>(L, last, lz, ly, lx);
-  last_xyz[0] = lz; last_xyz[1] = ly; last_xyz[2] = lx;
+  lane_zyx<GUARD>(L, 0, fz, fy, fx);
+  lane_zyx<GUARD>(L, last, lz, ly, lx);
+  last_zyx[0] = lz; last_zyx[1] = ly; last_zyx[2] = lx;
+  if (len == 1) return;
+  // total length first -> last (y, z, x order as in :1480-1482)
   const double ey = (double)(ly - fy) * vs[1];
   const double ez = (double)(lz - fz) * vs[0];
   const double ex = (double)(lx - fx) * vs[2];
@@ -261,17 +319,13 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const uint8_t* __rest
       start = pos;
     }
   } else {
-    // segment list overflowed: re-walk the (now known) path in order
-    int pl = 0;
-    {
-      int z, y, x;
-      coord_at<GUARD>(L, 0, z, y, x);
-      pl = (int)(int8_t)vol[(size_t)((z * Y + y) * X + x)];
-    }
+    // the boundary list overflowed: walk the (now known) path again in order
+    int ia, ib, ip;
+    lane_coords<GUARD>(L, 0, ia, ib, ip);
+    int pl = load_label(V, lane_offset<WIDE>(L, ia, ib, ip));
     for (int inc = 1; inc <= last; ++inc) {
-      int z, y, x;
-      coord_at<GUARD>(L, inc, z, y, x);
-      const int lb = (int)(int8_t)vol[(size_t)((z * Y + y) * X + x)];
+      lane_coords<GUARD>(L, inc, ia, ib, ip);
+      const int lb = load_label(V, lane_offset<WIDE>(L, ia, ib, ip));
       if (lb != pl && (unsigned)lb <= 4u) {
         add(cls, inc - start);
         cls = lb;
@@ -284,22 +338,39 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const uint8_t* __rest
   out[0] = li; out[1] = lo; out[2] = cr; out[3] = bu;
 }
 
-// Set up the lane's ray for exit face `face` (1..6) of rule `rr`.
+// Set up the lane's ray for exit face `face` (1..6) of rule `rr`; false when
+// the reference's precondition for that face fails.
 __device__ __forceinline__ bool make_lane(const RayRule* __restrict__ rr, int face, int z, int y,
-                                          int x, int Z, int Y, int X, RayLane& L) {
+                                          int x, const Volume& V, RayLane& L) {
   const uint32_t meta = rr->meta[face - 1];
-  L.bz = (double)z; L.by = (double)y; L.bx = (double)x;
-  L.dz = rr->d[face - 1][0];
-  L.dy = rr->d[face - 1][1];
-  L.dx = rr->d[face - 1][2];
-  L.hz = (double)(Z - 1); L.hy = (double)(Y - 1); L.hx = (double)(X - 1);
-  L.iz = Z - 1; L.iy = Y - 1; L.ix = X - 1;
-  L.rnd = (meta & kMetaRound) != 0;
-  L.vac = (int)((meta >> kMetaVacShift) & 0xffu);
+  const double* d = rr->d[face - 1];
   const int axis = (int)((meta >> kMetaAxisShift) & 3u);
-  const int c = axis == 0 ? z : (axis == 1 ? y : x);
-  const int ext = axis == 0 ? Z : (axis == 1 ? Y : X);
-  L.limit = (meta & kMetaToMax) ? (ext - c) + ((meta & kMetaInclusive) ? 1 : 0) : c + 1;
+  const int sz = V.Y * V.X, sy = V.X;
+  L.bias = (meta & kMetaRound) ? kRoundBias : 0.0;
+  L.vac = (int)((meta >> kMetaVacShift) & 0xffu);
+  L.paxis = axis;
+  int c, ext;
+  if (axis == 0) {          // z: a = y, b = x
+    L.ba = (double)y; L.da = d[1]; L.sa = sy; L.ha = V.Y - 1;
+    L.bb = (double)x; L.db = d[2]; L.sb = 1;  L.hb = V.X - 1;
+    L.sp = sz; L.hp = V.Z - 1; c = z; ext = V.Z;
+  } else if (axis == 1) {   // y: a = z, b = x
+    L.ba = (double)z; L.da = d[0]; L.sa = sz; L.ha = V.Z - 1;
+    L.bb = (double)x; L.db = d[2]; L.sb = 1;  L.hb = V.X - 1;
+    L.sp = sy; L.hp = V.Y - 1; c = y; ext = V.Y;
+  } else {                  // x: a = z, b = y
+    L.ba = (double)z; L.da = d[0]; L.sa = sz; L.ha = V.Z - 1;
+    L.bb = (double)y; L.db = d[1]; L.sb = sy; L.hb = V.Y - 1;
+    L.sp = 1; L.hp = V.X - 1; c = x; ext = V.X;
+  }
+  L.p0 = c;
+  if (meta & kMetaToMax) {
+    L.pdir = 1;
+    L.limit = (ext - c) + ((meta & kMetaInclusive) ? 1 : 0);
+  } else {
+    L.pdir = -1;
+    L.limit = c + 1;
+  }
   return (meta & kMetaValid) != 0;
 }
 
@@ -326,9 +397,9 @@ __device__ __forceinline__ void merge_pair(float& s, float& c, float os, float o
   c = (c + oc) - e;
 }
 
-template <int CAP, int U, bool GUARD, bool FULL, bool F64, bool PATHS>
+template <int CAP, int U, bool GUARD, bool FULL, bool WIDE, bool F64, bool PATHS>
 __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
-  __shared__ uint32_t seg_lds[CAP * kBlock];
+  __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
   __shared__ double red64[kBlock / kWave];
   __shared__ float red32[2][kBlock / kWave];
   __shared__ unsigned long long red_steps[kBlock / kWave];
@@ -340,6 +411,10 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
   const ReflRule* __restrict__ rule = a.rules + refl;
   const int tid = (int)threadIdx.x;
   uint32_t* seg = seg_lds + tid;
+  Volume V;
+  V.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.vol, (short)0,
+                                             a.Z * a.Y * a.X, 0x00020000);
+  V.Z = a.Z; V.Y = a.Y; V.X = a.X;
 
   float ks = 0.f, kc = 0.f;   // Kahan sum (value ks - kc)
   double acc64 = 0.0;
@@ -367,12 +442,11 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
       lasts[k][0] = lasts[k][1] = lasts[k][2] = 0;
       if (verr) continue;
       RayLane L;
-      if (!make_lane(&rule->ray[k], faces[k], co.x, co.y, co.z, a.Z, a.Y, a.X, L)) {
+      if (!make_lane(&rule->ray[k], faces[k], co.x, co.y, co.z, V, L)) {
         verr = ANACOR_RAY_ERR_ASSERT;
         continue;
       }
-      walk_ray<CAP, U, GUARD, FULL>(L, a.vol, a.Y, a.X, a.vs, seg, lens[k], nsegs[k], lasts[k],
-                                    Lr[k]);
+      walk_ray<CAP, U, GUARD, FULL, WIDE>(L, V, a.vs, seg, lens[k], nsegs[k], lasts[k], Lr[k]);
       steps += (unsigned long long)lens[k];
     }
     err |= verr;
@@ -543,32 +617,34 @@ static int choose_tile(const Context* c, int64_t R) {
   return (int)vpt;
 }
 
-template <bool GUARD, bool FULL, bool F64, bool PATHS>
+constexpr int kCap = 16;     // boundaries kept per lane before the re-walk fallback
+constexpr int kUnroll = 4;   // label gathers in flight per lane
+
+template <bool GUARD, bool FULL, bool WIDE, bool F64, bool PATHS>
 static void launch_trace(const TraceArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((trace_kernel<16, 4, GUARD, FULL, F64, PATHS>), dim3(grid), dim3(kBlock), 0,
-                     s, a);
+  hipLaunchKernelGGL((trace_kernel<kCap, kUnroll, GUARD, FULL, WIDE, F64, PATHS>), dim3(grid),
+                     dim3(kBlock), 0, s, a);
 }
 
-template <bool PATHS>
-static void dispatch_trace(const TraceArgs& a, int grid, bool guard, bool full, bool f64,
-                           hipStream_t s) {
-  if (guard) {
-    if (full) {
-      f64 ? launch_trace<true, true, true, PATHS>(a, grid, s)
-          : launch_trace<true, true, false, PATHS>(a, grid, s);
-    } else {
-      f64 ? launch_trace<true, false, true, PATHS>(a, grid, s)
-          : launch_trace<true, false, false, PATHS>(a, grid, s);
-    }
-  } else {
-    if (full) {
-      f64 ? launch_trace<false, true, true, PATHS>(a, grid, s)
-          : launch_trace<false, true, false, PATHS>(a, grid, s);
-    } else {
-      f64 ? launch_trace<false, false, true, PATHS>(a, grid, s)
-          : launch_trace<false, false, false, PATHS>(a, grid, s);
-    }
-  }
+using LaunchFn = void (*)(const TraceArgs&, int, hipStream_t);
+
+template <int I>
+static constexpr LaunchFn launcher() {
+  return &launch_trace<(I & 1) != 0, (I & 2) != 0, (I & 4) != 0, (I & 8) != 0, (I & 16) != 0>;
+}
+
+template <int... I>
+static constexpr std::array<LaunchFn, sizeof...(I)> launch_table(std::integer_sequence<int, I...>) {
+  return {launcher<I>()...};
+}
+
+static const std::array<LaunchFn, 32> kLaunch = launch_table(std::make_integer_sequence<int, 32>{});
+
+static void dispatch_trace(const TraceArgs& a, int grid, bool guard, bool full, bool wide,
+                           bool f64, bool paths, hipStream_t s) {
+  const int i = (guard ? 1 : 0) | (full ? 2 : 0) | (wide ? 4 : 0) | (f64 ? 8 : 0) |
+                (paths ? 16 : 0);
+  kLaunch[i](a, grid, s);
 }
 
 static int run_rules(Context* c, const ReflRule* h_rules, int64_t R, int flags, double* out,
@@ -622,8 +698,9 @@ static int run_rules(Context* c, const ReflRule* h_rules, int64_t R, int flags, 
   const int grid = (int)((nblk + 7) / 8 * 8);
 
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  if (paths) dispatch_trace<true>(a, grid, guard, c->full != 0, f64, c->stream);
-  else dispatch_trace<false>(a, grid, guard, c->full != 0, f64, c->stream);
+  // 24-bit address arithmetic unless a z-plane or the volume is too large for it
+  const bool wide = (int64_t)c->Y * c->X >= (1 << 23) || c->Z >= (1 << 23);
+  dispatch_trace(a, grid, guard, c->full != 0, wide, f64, paths, c->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   if (!paths) {

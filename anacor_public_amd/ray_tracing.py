@@ -69,7 +69,7 @@ def ray_tracing_overall(low, up, coord_list, len_coord_list, scattering_vector_l
     ``label_list`` may be a LabelTable or a 3-D array.  Returns float64[len_result];
     reflections the reference would abort on are NaN (see ``last_error``)."""
     lib = _lib.load()
-    table = label_list if isinstance(label_list, LabelTable) else LabelTable(label_list)
+    table = label_list if hasattr(label_list, "ptr") else LabelTable(label_list)
     coords = np.ascontiguousarray(coord_list, dtype=np.int64)
     s1 = _f64(scattering_vector_list)
     om = _f64(omega_list)
@@ -96,7 +96,7 @@ def ray_tracing_gpu_overall(low, up, coord_list, len_coord_list, scattering_vect
     """Drop-in for the reference's FP32 GPU entry (ray_tracing_gpu.c:33-62):
     float32 inputs, int32 coordinates and shape.  Returns float32[len_result]."""
     lib = _lib.load()
-    table = label_list if isinstance(label_list, LabelTable) else LabelTable(label_list)
+    table = label_list if hasattr(label_list, "ptr") else LabelTable(label_list)
     f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32).reshape(-1)  # noqa: E731
     coords = np.ascontiguousarray(coord_list, dtype=np.int32)
     p = lib.ray_tracing_gpu_overall(int(low), int(up), _p(coords), int(len_coord_list),
@@ -120,7 +120,7 @@ def ray_tracing_single(coord_list, len_coord_list, rotated_s1, xray, voxel_size,
                        label_list, shape, full_iteration=0, store_paths=0, IsExp=1) -> float:
     """Drop-in for ray_tracing_single (ray_tracing.c:166-292)."""
     lib = _lib.load()
-    table = label_list if isinstance(label_list, LabelTable) else LabelTable(label_list)
+    table = label_list if hasattr(label_list, "ptr") else LabelTable(label_list)
     coords = np.ascontiguousarray(coord_list, dtype=np.int64)
     v = lib.ray_tracing_single(_p(coords), int(len_coord_list), _p(_f64(rotated_s1)), _p(_f64(xray)),
                                _p(_f64(voxel_size)), _p(_f64(coefficients)), table.ptr,
@@ -137,7 +137,7 @@ def ray_tracing_single_mp(coord_list, len_coord_list, rotated_s1, xray, voxel_si
                           num_workers=1) -> float:
     """Drop-in for ray_tracing_single_mp (ray_tracing.c:294-364)."""
     lib = _lib.load()
-    table = label_list if isinstance(label_list, LabelTable) else LabelTable(label_list)
+    table = label_list if hasattr(label_list, "ptr") else LabelTable(label_list)
     coords = np.ascontiguousarray(coord_list, dtype=np.int64)
     v = lib.ray_tracing_single_mp(_p(coords), int(len_coord_list), _p(_f64(rotated_s1)),
                                   _p(_f64(xray)), _p(_f64(voxel_size)), _p(_f64(coefficients)),
