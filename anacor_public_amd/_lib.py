@@ -29,7 +29,7 @@ RAY_ERR_ASSERT = 2
 EXPORTED = (
     "ray_tracing_overall", "ray_tracing_gpu_overall", "ray_tracing_single",
     "ray_tracing_single_mp", "anacor_last_error", "anacor_device_count", "anacor_ctx_create",
-    "anacor_ctx_destroy", "anacor_ctx_set_tile", "anacor_ctx_run", "anacor_ctx_paths",
+    "anacor_ctx_destroy", "anacor_ctx_set_tile", "anacor_ctx_set_voxel_block", "anacor_ctx_run", "anacor_ctx_paths",
     "anacor_ctx_stats", "anacor_reflection_geometry",
 )
 
@@ -66,6 +66,8 @@ def _prototypes(lib):
     lib.anacor_ctx_destroy.argtypes = [vp]
     lib.anacor_ctx_set_tile.restype = ct.c_int
     lib.anacor_ctx_set_tile.argtypes = [vp, i32]
+    lib.anacor_ctx_set_voxel_block.restype = ct.c_int
+    lib.anacor_ctx_set_voxel_block.argtypes = [vp, i32, i32, i32]
     lib.anacor_ctx_run.restype = ct.c_int
     lib.anacor_ctx_run.argtypes = [vp, vp, vp, i64, vp, vp, vp, i32, vp]
     lib.anacor_ctx_paths.restype = ct.c_int

@@ -100,6 +100,7 @@ struct TraceArgs {
   unsigned* err;               // [R]
   unsigned long long* steps;   // sum of len_path_2
   PathRecord* paths;           // [R * N] in original voxel order (paths kernel)
+  const int* order;            // [R] caller's index of each (sorted) reflection
 };
 
 // Map hardware workgroup id to a logical one so that consecutive logical
@@ -180,39 +181,41 @@ __device__ __forceinline__ int axis_index(double base, double incd, double d, do
 }
 
 // A ray of one lane, with its axes permuted so that `p` is the exit face's
-// axis and a, b are the other two (in z, y, x order).
+// axis and a, b are the other two (in z, y, x order).  Origins are inside the
+// volume (checked on the host), so the face axis walks p0 + pdir*inc without
+// leaving [0, extent-1] except on the last step of an inclusive bound, where
+// the reference's clamp keeps it on the last plane: that is a clamp of the
+// increment at incmax.
 struct RayLane {
   double ba, bb;         // origins of the stepped axes a, b
   double da, db;         // per-increment deltas of a, b
   double bias;           // 0 (floor) or kRoundBias (round-half-away)
-  int sa, sb, sp;        // strides (elements) of a, b, p in the volume
-  int ha, hb, hp;        // extent - 1 of a, b, p
-  int p0, pdir;          // origin and direction (+1/-1) of the face axis
-  int paxis;             // the face axis: 0 z, 1 y, 2 x
+  int sa, sb;            // strides (elements) of a, b in the volume
+  int ha, hb;            // extent - 1 of a, b
+  int pbase, pstep;      // p0 * stride_p and pdir * stride_p
+  int incmax;            // increment at which the face axis reaches its last plane
+  int p0, pdir, paxis;   // face axis origin, direction (+1/-1), index (0 z, 1 y, 2 x)
   int limit;             // increments tried: inc in [0, limit)
   int vac;               // full_iteration vacuum-run limit
 };
 
-template <bool GUARD>
-__device__ __forceinline__ void lane_coords(const RayLane& L, int inc, int& ia, int& ib, int& ip) {
+template <bool GUARD, bool WIDE>
+__device__ __forceinline__ int lane_offset(const RayLane& L, int inc) {
   const double incd = (double)inc;
-  ia = axis_index<GUARD>(L.ba, incd, L.da, L.bias, L.ha);
-  ib = axis_index<GUARD>(L.bb, incd, L.db, L.bias, L.hb);
-  ip = clamp_int(L.p0 + L.pdir * inc, L.hp);
-}
-
-template <bool WIDE>
-__device__ __forceinline__ int lane_offset(const RayLane& L, int ia, int ib, int ip) {
-  if (WIDE) return ia * L.sa + ib * L.sb + ip * L.sp;
-  // all factors < 2^24 (checked on the host): full-rate 24-bit multiplies
-  return __mul24(ia, L.sa) + __mul24(ib, L.sb) +
-         __mul24(ip, L.sp);
+  const int ia = axis_index<GUARD>(L.ba, incd, L.da, L.bias, L.ha);
+  const int ib = axis_index<GUARD>(L.bb, incd, L.db, L.bias, L.hb);
+  const int ip = min(inc, L.incmax);
+  if (WIDE) return ia * L.sa + ib * L.sb + ip * L.pstep + L.pbase;
+  // every factor fits 24 bits (checked on the host): full-rate v_mad_i32_i24
+  return __mul24(ia, L.sa) + (__mul24(ib, L.sb) + (__mul24(ip, L.pstep) + L.pbase));
 }
 
 template <bool GUARD>
 __device__ __forceinline__ void lane_zyx(const RayLane& L, int inc, int& z, int& y, int& x) {
-  int ia, ib, ip;
-  lane_coords<GUARD>(L, inc, ia, ib, ip);
+  const double incd = (double)inc;
+  const int ia = axis_index<GUARD>(L.ba, incd, L.da, L.bias, L.ha);
+  const int ib = axis_index<GUARD>(L.bb, incd, L.db, L.bias, L.hb);
+  const int ip = L.p0 + L.pdir * min(inc, L.incmax);
   if (L.paxis == 0) { z = ip; y = ia; x = ib; }
   else if (L.paxis == 1) { z = ia; y = ip; x = ib; }
   else { z = ia; y = ib; x = ip; }
@@ -229,18 +232,21 @@ __device__ __forceinline__ int load_label(const Volume& V, int off) {
 
 // Walk one ray (cal_coord + cal_path2_plus).  Boundaries beyond the implicit
 // first one (class 3 at increment 0) are kept in this lane's column of `seg`
-// (CAP entries, lane-interleaved, plus one scratch slot so the store needs no
-// branch); len is len_path_2; out receives [li, lo, cr, bu].
+// (CAP entries, lane-interleaved, plus one scratch slot: every step stores
+// at the write slot, which advances only when a boundary opens); len is
+// len_path_2; out receives [li, lo, cr, bu].
 template <int CAP, int U, bool GUARD, bool FULL, bool WIDE>
 __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, const double vs[3],
                                          uint32_t* seg, int& len_out, int& nseg_out,
                                          int last_zyx[3], double out[4]) {
-  int last = -1, prev = 0, nseg = 0, va = 0;
+  // taken steps (increments 0 .. cnt-1 passed), last label, vacuum run
+  int cnt = 0, prev = 0, va = 0;
   bool done = L.limit <= 0;
+  bool overflow = false;
+  int wi = 0;                           // write slot * kBlock (32-bit LDS index)
+  constexpr int kTrash = CAP * kBlock;  // the scratch slot
   if (!done) {  // increment 0: its label opens no boundary
-    int ia, ib, ip;
-    lane_coords<GUARD>(L, 0, ia, ib, ip);
-    const int lab = load_label(V, lane_offset<WIDE>(L, ia, ib, ip));
+    const int lab = load_label(V, lane_offset<GUARD, WIDE>(L, 0));
     bool stop;
     if (FULL) {
       va = (lab == 0) ? 1 : 0;
@@ -249,42 +255,41 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
       stop = lab == 0;
     }
     if (stop) done = true;
-    else { prev = lab; last = 0; }
+    else { prev = lab; cnt = 1; }
   }
   for (int inc0 = 1; !done; inc0 += U) {
     int lab[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {  // all offsets are in range (clamped)
-      int ia, ib, ip;
-      lane_coords<GUARD>(L, inc0 + u, ia, ib, ip);
-      lab[u] = load_label(V, lane_offset<WIDE>(L, ia, ib, ip));
-    }
+    for (int u = 0; u < U; ++u)  // offsets stay inside the volume (clamped)
+      lab[u] = load_label(V, lane_offset<GUARD, WIDE>(L, inc0 + u));
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int inc = inc0 + u;
       const int lb = lab[u];
-      const bool act = !done && inc < L.limit;
       bool stop;
       if (FULL) {
-        const int va_n = (lb == 0) ? va + 1 : 0;
-        va = act ? va_n : va;
-        stop = va_n > L.vac;
+        va = (lb == 0) ? va + 1 : 0;
+        stop = va > L.vac;
       } else {
         stop = lb == 0;
       }
-      const bool take = act && !stop;
+      // once a lane stops it never takes another step, so prev/va need no select
+      const bool take = !done && inc < L.limit && !stop;
       // int8 labels: only values 0..4 open a boundary (appending, :343-394)
       const bool chg = take && lb != prev && (unsigned)lb <= 4u;
-      const int slot = (chg && nseg < CAP) ? nseg : CAP;
-      seg[slot * kBlock] = ((uint32_t)inc << 3) | (uint32_t)lb;
-      nseg += chg ? 1 : 0;
-      prev = take ? lb : prev;
-      last = take ? inc : last;
+      seg[wi] = ((uint32_t)inc << 3) | (uint32_t)lb;
+      const bool room = wi != kTrash;
+      overflow = overflow || (chg && !room);
+      wi = (chg && room) ? wi + kBlock : wi;
+      cnt += take ? 1 : 0;
+      prev = lb;
       done = done || !take;
     }
   }
 
-  const int len = last >= 0 ? last + 1 : 1;
+  int nseg = wi / kBlock;   // exact unless the list overflowed
+  const int last = cnt - 1;
+  const int len = cnt > 0 ? cnt : 1;
   len_out = len;
   nseg_out = nseg + 1;
   out[0] = out[1] = out[2] = out[3] = 0.0;
@@ -310,7 +315,7 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
     bu += (cls == 4) ? term : 0.0;
   };
   int cls = 3, start = 0;
-  if (nseg <= CAP) {
+  if (!overflow) {
     for (int k = 0; k < nseg; ++k) {
       const uint32_t e = seg[k * kBlock];
       const int pos = (int)(e >> 3);
@@ -320,19 +325,19 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
     }
   } else {
     // the boundary list overflowed: walk the (now known) path again in order
-    int ia, ib, ip;
-    lane_coords<GUARD>(L, 0, ia, ib, ip);
-    int pl = load_label(V, lane_offset<WIDE>(L, ia, ib, ip));
+    int pl = load_label(V, lane_offset<GUARD, WIDE>(L, 0));
+    nseg = 0;
     for (int inc = 1; inc <= last; ++inc) {
-      lane_coords<GUARD>(L, inc, ia, ib, ip);
-      const int lb = load_label(V, lane_offset<WIDE>(L, ia, ib, ip));
+      const int lb = load_label(V, lane_offset<GUARD, WIDE>(L, inc));
       if (lb != pl && (unsigned)lb <= 4u) {
         add(cls, inc - start);
         cls = lb;
         start = inc;
+        ++nseg;
       }
       pl = lb;
     }
+    nseg_out = nseg + 1;
   }
   add(cls, len - start);
   out[0] = li; out[1] = lo; out[2] = cr; out[3] = bu;
@@ -349,28 +354,32 @@ __device__ __forceinline__ bool make_lane(const RayRule* __restrict__ rr, int fa
   L.bias = (meta & kMetaRound) ? kRoundBias : 0.0;
   L.vac = (int)((meta >> kMetaVacShift) & 0xffu);
   L.paxis = axis;
-  int c, ext;
+  int c, ext, sp;
   if (axis == 0) {          // z: a = y, b = x
     L.ba = (double)y; L.da = d[1]; L.sa = sy; L.ha = V.Y - 1;
     L.bb = (double)x; L.db = d[2]; L.sb = 1;  L.hb = V.X - 1;
-    L.sp = sz; L.hp = V.Z - 1; c = z; ext = V.Z;
+    sp = sz; c = z; ext = V.Z;
   } else if (axis == 1) {   // y: a = z, b = x
     L.ba = (double)z; L.da = d[0]; L.sa = sz; L.ha = V.Z - 1;
     L.bb = (double)x; L.db = d[2]; L.sb = 1;  L.hb = V.X - 1;
-    L.sp = sy; L.hp = V.Y - 1; c = y; ext = V.Y;
+    sp = sy; c = y; ext = V.Y;
   } else {                  // x: a = z, b = y
     L.ba = (double)z; L.da = d[0]; L.sa = sz; L.ha = V.Z - 1;
     L.bb = (double)y; L.db = d[1]; L.sb = sy; L.hb = V.Y - 1;
-    L.sp = 1; L.hp = V.X - 1; c = x; ext = V.X;
+    sp = 1; c = x; ext = V.X;
   }
   L.p0 = c;
+  L.pbase = c * sp;
   if (meta & kMetaToMax) {
     L.pdir = 1;
     L.limit = (ext - c) + ((meta & kMetaInclusive) ? 1 : 0);
+    L.incmax = ext - 1 - c;
   } else {
     L.pdir = -1;
     L.limit = c + 1;
+    L.incmax = c;
   }
+  L.pstep = L.pdir * sp;
   return (meta & kMetaValid) != 0;
 }
 
@@ -451,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
     }
     err |= verr;
     if (PATHS) {
-      PathRecord& p = a.paths[(size_t)refl * a.N + co.w];
+      PathRecord& p = a.paths[(size_t)a.order[refl] * a.N + co.w];
       p.err = (int)verr;
       p.pad = 0;
 #pragma unroll
@@ -520,8 +529,8 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
 
 // Per reflection: combine the tile partials in tile order and divide by N.
 __global__ void finish_kernel(const float2* __restrict__ p32, const double* __restrict__ p64,
-                              const unsigned* __restrict__ err, int R, int tiles, int N, int f64,
-                              double* __restrict__ out) {
+                              const unsigned* __restrict__ err, const int* __restrict__ order,
+                              int R, int tiles, int N, int f64, double* __restrict__ out) {
   const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (r >= R) return;
   double sum = 0.0;
@@ -535,7 +544,7 @@ __global__ void finish_kernel(const float2* __restrict__ p32, const double* __re
     }
     sum = (double)s - (double)c;
   }
-  out[r] = err[r] ? __builtin_nan("") : sum / (double)N;
+  out[order[r]] = err[r] ? __builtin_nan("") : sum / (double)N;
 }
 
 // ------------------------------------------------------------ host side
@@ -551,6 +560,8 @@ struct Context {
   double mu[4] = {0, 0, 0, 0};
   int full = 0;
   int tile_voxels = 0;          // 0: automatic
+  std::vector<int4> h_coords;   // (z, y, x, index) in the caller's order
+  int block[3] = {0, 0, 0};     // voxel grouping in use (0,0,0: Morton)
   // reusable per-run buffers
   ReflRule* d_rules = nullptr;
   size_t rules_cap = 0;
@@ -561,6 +572,9 @@ struct Context {
   size_t refl_cap = 0;
   unsigned long long* d_steps = nullptr;
   std::vector<ReflRule> h_rules;
+  std::vector<ReflRule> h_sorted;   // rules in processing order
+  std::vector<int> h_order;         // processing position -> caller's index
+  int* d_order = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_kernel_ms = 0.f;
   unsigned long long last_steps = 0;
@@ -595,10 +609,13 @@ static int ensure_run_buffers(Context* c, int64_t R, int64_t n_partials, bool f6
   if ((size_t)R > c->refl_cap) {
     if (c->d_err) HIP_TRY(hipFree(c->d_err));
     if (c->d_out) HIP_TRY(hipFree(c->d_out));
+    if (c->d_order) HIP_TRY(hipFree(c->d_order));
     c->d_err = nullptr;
     c->d_out = nullptr;
+    c->d_order = nullptr;
     HIP_TRY(hipMalloc(&c->d_err, (size_t)R * sizeof(unsigned)));
     HIP_TRY(hipMalloc(&c->d_out, (size_t)R * sizeof(double)));
+    HIP_TRY(hipMalloc(&c->d_order, (size_t)R * sizeof(int)));
     c->refl_cap = (size_t)R;
   }
   if (!c->d_steps) HIP_TRY(hipMalloc(&c->d_steps, sizeof(unsigned long long)));
@@ -647,7 +664,35 @@ static void dispatch_trace(const TraceArgs& a, int grid, bool guard, bool full, 
   kLaunch[i](a, grid, s);
 }
 
-static int run_rules(Context* c, const ReflRule* h_rules, int64_t R, int flags, double* out,
+// Processing order of the reflections: sorted by the directions of their two
+// rays (7-bit Morton code of each unit vector, incident first), so the
+// workgroups resident on an XCD at one time trace nearly parallel rays whose
+// label gathers share that XCD's L2.  Results are written back through the
+// inverse permutation, so the order is invisible to callers.
+static void order_reflections(Context* c, const ReflRule* rules, int64_t R) {
+  c->h_order.resize((size_t)R);
+  for (int64_t i = 0; i < R; ++i) c->h_order[(size_t)i] = (int)i;
+  const char* env = std::getenv("ANACOR_SORT_REFLECTIONS");
+  if (env && env[0] == '0') return;
+  auto q = [](double v) {
+    double t = (v + 1.0) * 64.0;
+    if (!(t >= 0.0)) t = 0.0;
+    if (t > 127.0) t = 127.0;
+    return (uint64_t)t;
+  };
+  std::vector<uint64_t> key((size_t)R);
+  for (int64_t i = 0; i < R; ++i) {
+    const double* a = rules[i].ray[0].dir;
+    const double* b = rules[i].ray[1].dir;
+    const uint64_t ka = (spread_bits(q(a[0])) << 2) | (spread_bits(q(a[1])) << 1) | spread_bits(q(a[2]));
+    const uint64_t kb = (spread_bits(q(b[0])) << 2) | (spread_bits(q(b[1])) << 1) | spread_bits(q(b[2]));
+    key[(size_t)i] = (ka << 21) | kb;
+  }
+  std::stable_sort(c->h_order.begin(), c->h_order.end(),
+                   [&](int x, int y) { return key[(size_t)x] < key[(size_t)y]; });
+}
+
+static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flags, double* out,
                      PathRecord* paths_out) {
   const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
   const bool paths = paths_out != nullptr;
@@ -665,6 +710,10 @@ static int run_rules(Context* c, const ReflRule* h_rules, int64_t R, int flags, 
   }
   int rc = ensure_run_buffers(c, R, nblk, f64);
   if (rc) return rc;
+  order_reflections(c, h_rules_in, R);
+  c->h_sorted.resize((size_t)R);
+  for (int64_t i = 0; i < R; ++i) c->h_sorted[(size_t)i] = h_rules_in[c->h_order[(size_t)i]];
+  const ReflRule* h_rules = c->h_sorted.data();
   bool guard = false;
   for (int64_t i = 0; i < R; ++i)
     guard = guard || h_rules[i].ray[0].guard || h_rules[i].ray[1].guard;
@@ -673,6 +722,8 @@ static int run_rules(Context* c, const ReflRule* h_rules, int64_t R, int flags, 
   if (paths) HIP_TRY(hipMalloc(&d_paths, (size_t)R * c->N * sizeof(PathRecord)));
 
   HIP_TRY(hipMemcpyAsync(c->d_rules, h_rules, (size_t)R * sizeof(ReflRule),
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order.data(), (size_t)R * sizeof(int),
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemsetAsync(c->d_err, 0, (size_t)R * sizeof(unsigned), c->stream));
   HIP_TRY(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
@@ -695,6 +746,7 @@ static int run_rules(Context* c, const ReflRule* h_rules, int64_t R, int flags, 
   a.err = c->d_err;
   a.steps = c->d_steps;
   a.paths = d_paths;
+  a.order = c->d_order;
   const int grid = (int)((nblk + 7) / 8 * 8);
 
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
@@ -705,8 +757,8 @@ static int run_rules(Context* c, const ReflRule* h_rules, int64_t R, int flags, 
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   if (!paths) {
     hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, c->stream,
-                       (const float2*)c->d_partial, (const double*)c->d_partial, c->d_err, (int)R,
-                       (int)tiles, c->N, f64 ? 1 : 0, c->d_out);
+                       (const float2*)c->d_partial, (const double*)c->d_partial, c->d_err,
+                       c->d_order, (int)R, (int)tiles, c->N, f64 ? 1 : 0, c->d_out);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, c->d_out, (size_t)R * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
@@ -730,12 +782,63 @@ static int run_rules(Context* c, const ReflRule* h_rules, int64_t R, int flags, 
       if (herr[i]) {
         set_error(ANACOR_ERR_GEOMETRY,
                   "reflection %lld: %s (the reference aborts here); result set to NaN",
-                  (long long)i,
+                  (long long)c->h_order[(size_t)i],
                   (herr[i] & ANACOR_RAY_ERR_FACE) ? "no exit face (cube_face)"
                                                   : "exit face inconsistent with beam angles");
         return ANACOR_ERR_GEOMETRY;
       }
   }
+  return ANACOR_OK;
+}
+
+static void ctx_destroy(Context* c);
+
+// Default grouping of crystal voxels into waves (voxel units, z y x): lanes
+// run along x within a block, blocks in z, y, x order.  A wave's rays are
+// parallel, so lanes that share z and y keep sharing them at every step and
+// gather from one row of the [z][y][x] volume.
+constexpr int kDefaultBlock[3] = {1, 2, 64};
+
+// Order the voxel list for the kernels and upload it.  (bz, by, bx) > 0:
+// blocks of that shape in z, y, x order, lexicographic (z, y, x) inside a
+// block; (0, 0, 0): Morton order.
+static int upload_voxel_order(Context* c, int bz, int by, int bx) {
+  const size_t n = c->h_coords.size();
+  if (n == 0) return ANACOR_OK;
+  std::vector<uint64_t> key(n);
+  const bool morton = bz <= 0 || by <= 0 || bx <= 0;
+  for (size_t i = 0; i < n; ++i) {
+    const int4 v = c->h_coords[i];
+    const uint64_t z = (uint64_t)v.x, y = (uint64_t)v.y, x = (uint64_t)v.z;
+    if (morton) {
+      key[i] = (spread_bits(z) << 2) | (spread_bits(y) << 1) | spread_bits(x);
+    } else {
+      // block index in (z, y, x) order
+      key[i] = ((z / bz) << 42) | ((y / by) << 21) | (x / bx);
+    }
+  }
+  std::vector<int32_t> idx(n);
+  for (size_t i = 0; i < n; ++i) idx[i] = (int32_t)i;
+  if (morton) {
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int32_t a, int32_t b) { return key[(size_t)a] < key[(size_t)b]; });
+  } else {
+    // block index, then (z, y, x) inside the block
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) {
+      if (key[(size_t)a] != key[(size_t)b]) return key[(size_t)a] < key[(size_t)b];
+      const int4 u = c->h_coords[(size_t)a], w = c->h_coords[(size_t)b];
+      if (u.x != w.x) return u.x < w.x;
+      if (u.y != w.y) return u.y < w.y;
+      return u.z < w.z;
+    });
+  }
+  std::vector<int4> sorted(n);
+  for (size_t i = 0; i < n; ++i) sorted[i] = c->h_coords[(size_t)idx[i]];
+  if (!c->d_coords) HIP_TRY(hipMalloc(&c->d_coords, n * sizeof(int4)));
+  HIP_TRY(hipMemcpy(c->d_coords, sorted.data(), n * sizeof(int4), hipMemcpyHostToDevice));
+  c->block[0] = morton ? 0 : bz;
+  c->block[1] = morton ? 0 : by;
+  c->block[2] = morton ? 0 : bx;
   return ANACOR_OK;
 }
 
@@ -761,6 +864,18 @@ static int ctx_create(const uint8_t* vol, const int64_t shape[3], const int64_t*
     set_error(ANACOR_ERR_ARG, "bad voxel list (n=%lld)", (long long)n);
     return ANACOR_ERR_ARG;
   }
+  // crystal voxels must lie inside the volume (generate_sampling only yields
+  // such voxels; the kernels rely on it for their face-axis stepping)
+  for (int64_t i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      const int64_t v = coords ? coords[3 * i + a] : coords32[3 * i + a];
+      if (v < 0 || v >= shape[a]) {
+        set_error(ANACOR_ERR_ARG, "voxel %lld: coordinate %lld outside the volume", (long long)i,
+                  (long long)v);
+        return ANACOR_ERR_ARG;
+      }
+    }
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     set_error(ANACOR_ERR_NO_DEVICE, "no HIP device visible");
@@ -780,33 +895,20 @@ static int ctx_create(const uint8_t* vol, const int64_t shape[3], const int64_t*
   c->N = (int)n;
   HIP_TRY(hipMalloc(&c->d_vol, nvox));
   HIP_TRY(hipMemcpy(c->d_vol, vol, nvox, hipMemcpyHostToDevice));
-  if (n > 0) {
-    // Morton order of (z, y, x): lanes of a wave start from neighbouring voxels
-    std::vector<int4> h((size_t)n);
-    std::vector<uint64_t> key((size_t)n);
-    std::vector<int32_t> idx((size_t)n);
-    for (int64_t i = 0; i < n; ++i) {
-      const int64_t z = coords ? coords[3 * i] : coords32[3 * i];
-      const int64_t y = coords ? coords[3 * i + 1] : coords32[3 * i + 1];
-      const int64_t x = coords ? coords[3 * i + 2] : coords32[3 * i + 2];
-      if (z < INT32_MIN / 2 || z > INT32_MAX / 2 || y < INT32_MIN / 2 || y > INT32_MAX / 2 ||
-          x < INT32_MIN / 2 || x > INT32_MAX / 2) {
-        set_error(ANACOR_ERR_ARG, "voxel %lld coordinate out of range", (long long)i);
-        delete c;
-        return ANACOR_ERR_ARG;
-      }
-      h[(size_t)i] = make_int4((int)z, (int)y, (int)x, (int)i);
-      key[(size_t)i] = (spread_bits((uint64_t)z) << 2) | (spread_bits((uint64_t)y) << 1) |
-                       spread_bits((uint64_t)x);
-      idx[(size_t)i] = (int32_t)i;
-    }
-    std::stable_sort(idx.begin(), idx.end(),
-                     [&](int32_t a, int32_t b) { return key[(size_t)a] < key[(size_t)b]; });
-    std::vector<int4> sorted((size_t)n);
-    for (int64_t i = 0; i < n; ++i) sorted[(size_t)i] = h[(size_t)idx[(size_t)i]];
-    HIP_TRY(hipMalloc(&c->d_coords, (size_t)n * sizeof(int4)));
-    HIP_TRY(hipMemcpy(c->d_coords, sorted.data(), (size_t)n * sizeof(int4),
-                      hipMemcpyHostToDevice));
+  c->h_coords.resize((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t z = coords ? coords[3 * i] : coords32[3 * i];
+    const int64_t y = coords ? coords[3 * i + 1] : coords32[3 * i + 1];
+    const int64_t x = coords ? coords[3 * i + 2] : coords32[3 * i + 2];
+    c->h_coords[(size_t)i] = make_int4((int)z, (int)y, (int)x, (int)i);
+  }
+  int blk[3] = {kDefaultBlock[0], kDefaultBlock[1], kDefaultBlock[2]};
+  if (const char* e = std::getenv("ANACOR_VOXEL_BLOCK"))
+    if (std::sscanf(e, "%d,%d,%d", &blk[0], &blk[1], &blk[2]) != 3) blk[0] = blk[1] = blk[2] = 0;
+  const int rc = upload_voxel_order(c, blk[0], blk[1], blk[2]);
+  if (rc != ANACOR_OK) {
+    ctx_destroy(c);
+    return rc;
   }
   *out = c;
   return ANACOR_OK;
@@ -822,6 +924,7 @@ static void ctx_destroy(Context* c) {
   if (c->d_partial) (void)hipFree(c->d_partial);
   if (c->d_err) (void)hipFree(c->d_err);
   if (c->d_out) (void)hipFree(c->d_out);
+  if (c->d_order) (void)hipFree(c->d_order);
   if (c->d_steps) (void)hipFree(c->d_steps);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -920,6 +1023,14 @@ void anacor_ctx_destroy(anacor_ctx* ctx) {
   if (!ctx) return;
   ctx_destroy(ctx->c);
   delete ctx;
+}
+
+int anacor_ctx_set_voxel_block(anacor_ctx* ctx, int bz, int by, int bx) {
+  clear_error();
+  if (!ctx) return ANACOR_ERR_ARG;
+  std::lock_guard<std::mutex> lk(ctx->c->mu_run);
+  HIP_TRY(hipSetDevice(ctx->c->device));
+  return upload_voxel_order(ctx->c, bz, by, bx);
 }
 
 int anacor_ctx_set_tile(anacor_ctx* ctx, int voxels_per_tile) {

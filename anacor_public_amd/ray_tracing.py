@@ -199,6 +199,12 @@ class Tracer:
     def set_tile(self, voxels_per_tile: int):
         self._lib.anacor_ctx_set_tile(self._h, int(voxels_per_tile))
 
+    def set_voxel_block(self, bz: int, by: int, bx: int):
+        """Group crystal voxels into waves by (bz, by, bx) voxel blocks
+        ((0, 0, 0): Morton order)."""
+        _lib.check(self._lib.anacor_ctx_set_voxel_block(self._h, int(bz), int(by), int(bx)),
+                   self._lib)
+
     def run(self, s1, omega, raw_xray, omega_axis, F, f64_mean: bool = False, is_exp: bool = True,
             strict: bool = False) -> np.ndarray:
         s1 = _f64(s1).reshape(-1, 3)

@@ -68,6 +68,23 @@ def cpu_baseline(wl, budget_s: float = 15.0):
                        f"({dt:.1f} s)"), out
 
 
+def calibrate_copy(nbytes: int = 1 << 30) -> None:
+    """A plain device-to-device copy of known size (HIP's blit kernel), run
+    under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE readings."""
+    import ctypes as ct
+    hip = ct.CDLL("libamdhip64.so")
+    a, b = ct.c_void_p(), ct.c_void_p()
+    assert hip.hipMalloc(ct.byref(a), ct.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ct.byref(b), ct.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ct.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ct.c_size_t(nbytes), 3) == 0   # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,7 +94,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--reflections", type=int, default=0,
+                    help="trace only the first R reflections (profiling runs)")
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards "
               "reflections across GPUs by construction but this build runs and reports N=1.",
@@ -94,6 +117,8 @@ def main():
         wl = W.small_config()
     else:
         wl = W.large_config()
+    if args.reflections:
+        wl = wl.subset(args.reflections)
     R, N = len(wl.omega), len(wl.coords)
     print(f"# workload {wl.name}: volume {wl.volume.shape}, N={N}, R={R} "
           f"(built in {time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)

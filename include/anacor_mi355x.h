@@ -105,6 +105,11 @@ void anacor_ctx_destroy(anacor_ctx *ctx);
 /* Voxels per workgroup tile (0 = automatic). */
 int anacor_ctx_set_tile(anacor_ctx *ctx, int voxels_per_tile);
 
+/* Grouping of the crystal voxels into waves: blocks of bz x by x bx voxels in
+ * z, y, x order, lanes along x inside a block; (0, 0, 0) selects Morton
+ * order.  Affects speed only, never results beyond the FP32 summation order. */
+int anacor_ctx_set_voxel_block(anacor_ctx *ctx, int bz, int by, int bx);
+
 /* Mean absorption factor of R reflections (s1 [R][3], omega [R], DIALS frame)
  * into out[R] (host memory).  The same computation as ray_tracing_overall on
  * a resident volume. */
