@@ -164,6 +164,13 @@ __device__ __forceinline__ int cvt_trunc_sat(double x) {
   return r;
 }
 
+// a * b + c on the low 24 bits of a and b (signed), one v_mad_i32_i24
+__device__ __forceinline__ int mad24(int a, int b, int c) {
+  int r;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
 // median(v, 0, hi) == clamp to [0, hi] for hi >= 0, one v_med3_i32
 __device__ __forceinline__ int clamp_int(int v, int hi) {
   int r;
@@ -207,7 +214,7 @@ __device__ __forceinline__ int lane_offset(const RayLane& L, int inc) {
   const int ip = min(inc, L.incmax);
   if (WIDE) return ia * L.sa + ib * L.sb + ip * L.pstep + L.pbase;
   // every factor fits 24 bits (checked on the host): full-rate v_mad_i32_i24
-  return __mul24(ia, L.sa) + (__mul24(ib, L.sb) + (__mul24(ip, L.pstep) + L.pbase));
+  return mad24(ia, L.sa, mad24(ib, L.sb, mad24(ip, L.pstep, L.pbase)));
 }
 
 template <bool GUARD>
@@ -235,16 +242,18 @@ __device__ __forceinline__ int load_label(const Volume& V, int off) {
 // (CAP entries, lane-interleaved, plus one scratch slot: every step stores
 // at the write slot, which advances only when a boundary opens); len is
 // len_path_2; out receives [li, lo, cr, bu].
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 template <int CAP, int U, bool GUARD, bool FULL, bool WIDE>
 __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, const double vs[3],
-                                         uint32_t* seg, int& len_out, int& nseg_out,
+                                         lds_u32* seg, int& len_out, int& nseg_out,
                                          int last_zyx[3], double out[4]) {
   // taken steps (increments 0 .. cnt-1 passed), last label, vacuum run
   int cnt = 0, prev = 0, va = 0;
   bool done = L.limit <= 0;
   bool overflow = false;
-  int wi = 0;                           // write slot * kBlock (32-bit LDS index)
-  constexpr int kTrash = CAP * kBlock;  // the scratch slot
+  lds_u32* wp = seg;                          // write slot (32-bit LDS address)
+  lds_u32* const trash = seg + CAP * kBlock;  // the scratch slot
   if (!done) {  // increment 0: its label opens no boundary
     const int lab = load_label(V, lane_offset<GUARD, WIDE>(L, 0));
     bool stop;
@@ -273,21 +282,21 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
       } else {
         stop = lb == 0;
       }
-      // once a lane stops it never takes another step, so prev/va need no select
+      // once a lane stops it never takes another step: prev/va need no select
       const bool take = !done && inc < L.limit && !stop;
       // int8 labels: only values 0..4 open a boundary (appending, :343-394)
       const bool chg = take && lb != prev && (unsigned)lb <= 4u;
-      seg[wi] = ((uint32_t)inc << 3) | (uint32_t)lb;
-      const bool room = wi != kTrash;
-      overflow = overflow || (chg && !room);
-      wi = (chg && room) ? wi + kBlock : wi;
+      *wp = ((uint32_t)inc << 3) | (uint32_t)lb;
+      const bool full = wp == trash;
+      overflow = overflow || (chg && full);
+      wp = (chg && !full) ? wp + kBlock : wp;
       cnt += take ? 1 : 0;
       prev = lb;
       done = done || !take;
     }
   }
 
-  int nseg = wi / kBlock;   // exact unless the list overflowed
+  int nseg = (int)(wp - seg) / kBlock;   // exact unless the list overflowed
   const int last = cnt - 1;
   const int len = cnt > 0 ? cnt : 1;
   len_out = len;
@@ -419,7 +428,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
   const int tile = lb - refl * a.tiles;
   const ReflRule* __restrict__ rule = a.rules + refl;
   const int tid = (int)threadIdx.x;
-  uint32_t* seg = seg_lds + tid;
+  lds_u32* seg = (lds_u32*)(seg_lds + tid);
   Volume V;
   V.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.vol, (short)0,
                                              a.Z * a.Y * a.X, 0x00020000);
@@ -635,7 +644,7 @@ static int choose_tile(const Context* c, int64_t R) {
 }
 
 constexpr int kCap = 16;     // boundaries kept per lane before the re-walk fallback
-constexpr int kUnroll = 4;   // label gathers in flight per lane
+constexpr int kUnroll = 4;   // label gathers in flight per lane (2, 6, 8 measured slower)
 
 template <bool GUARD, bool FULL, bool WIDE, bool F64, bool PATHS>
 static void launch_trace(const TraceArgs& a, int grid, hipStream_t s) {
