@@ -1,0 +1,227 @@
+"""Pin the oracle with what the mathematics fixes, independently of the
+reference build:
+
+* an axis-aligned beam through a single-material box, whose per-ray lengths
+  have a closed form (including the reference's len/(len-1) segment scaling,
+  ray_tracing_basic.c:1504-1542);
+* mu = 0 gives A = 1 exactly;
+* the Rodrigues matrix against scipy's rotation;
+* cube_face against a geometric march along the ray;
+* International Tables sphere and cylinder transmission factors (the values
+  the reference's verification compares with, verification/main_verification.py:92-106),
+  at percent level as voxelisation allows;
+* the rounding identity the GPU kernels use (round-half-away == trunc(w + 0.5 - 2^-54)
+  after the clamp), checked over adversarial doubles.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from anacor_public_amd import workload as W
+
+
+def box_workload(shape=(10, 12, 30), box=((2, 8), (3, 9), (5, 25)), mu=0.03):
+    vol = np.zeros(shape, np.uint8)
+    (z0, z1), (y0, y1), (x0, x1) = box
+    vol[z0:z1, y0:y1, x0:x1] = 3
+    coords = W.crystal_voxels(vol)
+    s1 = np.array([[0.0, -1.0, 0.0]])     # diffracted along the beam (2theta = 0)
+    om = np.array([0.0])
+    wl = W.Workload("box", vol, coords, s1, om)
+    wl.coefficients = np.array([0.0, 0.0, mu, 0.0])
+    return wl
+
+
+def test_axis_aligned_box_closed_form(oracle):
+    """Beam along -x of the volume: the incident ray (traced backwards) runs
+    to +x, the diffracted one to -x.  A ray of len steps has T = len-1 voxel
+    lengths and L = T * len/(len-1) = len (0 when len == 1), so a voxel at x
+    in a box [x0, x1) sees L1 = x1-x (0 at x = x1-1) and L2 = x-x0+1 (0 at x = x0)."""
+    mu = 0.03
+    wl = box_workload(mu=mu)
+    out = oracle.trace(wl, rays=True)
+    x = wl.coords[:, 2]
+    x0, x1 = 5, 25
+    l1 = np.where(x == x1 - 1, 0, x1 - x).astype(float)
+    l2 = np.where(x == x0, 0, x - x0 + 1).astype(float)
+    rays = out["rays"][0]
+    np.testing.assert_array_equal(rays[:, 0]["L"][:, 2], l1)
+    np.testing.assert_array_equal(rays[:, 1]["L"][:, 2], l2)
+    np.testing.assert_array_equal(rays[:, 0]["face"], 1)
+    np.testing.assert_array_equal(rays[:, 1]["face"], 6)
+    expect = np.mean(np.exp(-mu * (l1 + l2)))
+    assert abs(out["result"][0] - expect) <= 1e-13 * expect   # summation order only
+
+
+def test_mu_zero_is_one(oracle):
+    wl = W.small_config(R=8)
+    wl.coefficients = np.zeros(4)
+    assert np.all(oracle.trace(wl)["result"] == 1.0)
+
+
+def test_absorption_in_unit_interval_and_monotone_in_mu(oracle):
+    wl = W.small_config(R=6)
+    a1 = oracle.trace(wl)["result"]
+    wl2 = W.small_config(R=6)
+    wl2.coefficients = wl.coefficients * 2
+    a2 = oracle.trace(wl2)["result"]
+    assert np.all((a1 > 0) & (a1 <= 1)) and np.all(a2 < a1)
+
+
+def test_rodrigues_matches_scipy(oracle):
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        th = rng.uniform(-7, 7)
+        m = oracle.kp_rotation(axis, th).reshape(3, 3)
+        ref = Rotation.from_rotvec(axis * th).as_matrix()
+        np.testing.assert_allclose(m, ref, atol=2e-15)
+
+
+def test_cube_face_matches_geometric_march(oracle):
+    """The exit plane equals the first plane a fine march along the ray
+    crosses (random directions, interior origins, so no ties)."""
+    rng = np.random.default_rng(5)
+    shape = np.array([17, 23, 29])
+    names = {6: (2, 0), 1: (2, 1), 4: (1, 0), 5: (1, 1), 2: (0, 0), 3: (0, 1)}
+    for _ in range(400):
+        o = np.array([rng.integers(1, s - 1) for s in shape])
+        d = rng.normal(size=3)
+        L1 = int(rng.integers(0, 2))
+        face = oracle.cube_face(o, d, shape, L1)
+        dd = -d if L1 else d
+        t = np.inf
+        hit = None
+        for a in range(3):
+            for side, bound in ((0, 0.0), (1, float(shape[a]))):
+                if dd[a] != 0:
+                    ta = (bound - o[a]) / dd[a]
+                    if ta >= 0 and ta < t:
+                        t, hit = ta, (a, side)
+        # march: first coordinate that leaves [0, shape]
+        p = o + dd * (t * (1 - 1e-9))
+        q = o + dd * (t * (1 + 1e-9))
+        left = [(a, 0 if q[a] < 0 else 1) for a in range(3) if q[a] < 0 or q[a] > shape[a]]
+        assert np.all((p >= 0) & (p <= shape)) and left == [hit]
+        assert names[face] == hit
+
+
+# International Tables for Crystallography Vol. C, Tables 6.3.3.2 (sphere) and
+# 6.3.3.3 (cylinder): A* = 1/A at Bragg angle theta = 0, 5, ..., 90 degrees,
+# as listed in verification/main_verification.py:92-106.
+SPHERE_A_STAR = {
+    0.5: [2.0755, 2.0743, 2.0706, 2.0647, 2.0565, 2.0462, 2.0340, 2.0204, 2.0056, 1.9901,
+          1.9745, 1.9592, 1.9445, 1.9311, 1.9194, 1.9097, 1.9024, 1.8979, 1.8964],
+    1.0: [4.1237, 4.1131, 4.0815, 4.0304, 3.9625, 3.8816, 3.7917, 3.6966, 3.6001, 3.5048,
+          3.4135, 3.3280, 3.2499, 3.1807, 3.1216, 3.0738, 3.0383, 3.0163, 3.0090],
+}
+CYLINDER_A_STAR = {
+    0.5: [2.2996, 2.2979, 2.2926, 2.2840, 2.2721, 2.2572, 2.2398, 2.2204, 2.1996, 2.1781,
+          2.1564, 2.1352, 2.1152, 2.0969, 2.0809, 2.0677, 2.0579, 2.0518, 2.0497],
+    1.0: [5.0907, 5.0724, 5.0185, 4.9323, 4.8196, 4.6877, 4.5439, 4.3948, 4.2461, 4.1022,
+          3.9664, 3.8413, 3.7286, 3.6298, 3.5462, 3.4790, 3.4295, 3.3990, 3.3886],
+}
+
+
+def _phantom_A(oracle, vol, radius, mu_r, two_theta_deg):
+    """A for a crystal phantom: beam (0,-1,0), diffracted (0,-cos 2t,-sin 2t)
+    as main_verification.py:238-239 sets them, mu on the crystal only."""
+    coords = W.sample_even(W.crystal_voxels(vol), 6000)
+    mu = mu_r / radius
+    out = []
+    for tt in two_theta_deg:
+        a = math.radians(tt)
+        s1 = np.array([0.0, -math.cos(a), -math.sin(a)])
+        out.append(oracle.single(coords, s1, np.array([0.0, -1.0, 0.0]), np.ones(3),
+                                 np.array([0.0, 0.0, mu, 0.0]), vol))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("mu_r", [0.5, 1.0])
+def test_sphere_international_tables(oracle, mu_r):
+    R = 30
+    n = 3 * R
+    c = n // 2 - 1
+    z, y, x = np.ogrid[:n, :n, :n]
+    vol = (((z - c) ** 2 + (y - c) ** 2 + (x - c) ** 2) < R * R).astype(np.uint8) * 3
+    tt = np.arange(0, 180, 20.0)                     # 2theta; table index = theta / 5
+    A = _phantom_A(oracle, vol, R, mu_r, tt)
+    table = 1.0 / np.array(SPHERE_A_STAR[mu_r])[(tt / 2 / 5).astype(int)]
+    rel = np.abs(A - table) / table
+    assert rel.max() < 0.05, rel
+
+
+@pytest.mark.parametrize("mu_r", [0.5, 1.0])
+def test_cylinder_international_tables(oracle, mu_r):
+    """Infinite cylinder (axis along the volume's z, perpendicular to the
+    scattering plane): voxels in the middle slab, ends far away."""
+    R = 30
+    n = 3 * R
+    c = n // 2 - 1
+    y, x = np.ogrid[:n, :n]
+    disc = (((y - c) ** 2 + (x - c) ** 2) < R * R).astype(np.uint8) * 3
+    vol = np.repeat(disc[None], 24, axis=0)
+    vol[:4] = 0
+    vol[-4:] = 0
+    coords = W.crystal_voxels(vol)
+    coords = coords[(coords[:, 0] >= 10) & (coords[:, 0] < 14)]
+    mu = mu_r / R
+    tt = np.arange(0, 180, 20.0)
+    A = []
+    for t in tt:
+        a = math.radians(t)
+        s1 = np.array([0.0, -math.cos(a), -math.sin(a)])
+        A.append(oracle.single(W.sample_even(coords, 4000), s1, np.array([0.0, -1.0, 0.0]),
+                               np.ones(3), np.array([0.0, 0.0, mu, 0.0]), vol))
+    A = np.array(A)
+    table = 1.0 / np.array(CYLINDER_A_STAR[mu_r])[(tt / 2 / 5).astype(int)]
+    rel = np.abs(A - table) / table
+    assert rel.max() < 0.05, rel
+
+
+def _round_half_away(w):
+    f = np.floor(w)
+    return f + ((w - f) >= 0.5)
+
+
+def test_biased_trunc_rounding_identity():
+    """For every double w: clamp(round_half_away(w)) == clamp(trunc_sat(w + 0.5 - 2^-54)),
+    clamp to [0, n-1]; floor likewise equals trunc(w) after the clamp."""
+    rng = np.random.default_rng(11)
+    n = 1 << 20
+    parts = [rng.uniform(-50, 2100, 1 << 20)]
+    k = np.arange(-30, 1100, dtype=np.float64)
+    halves = np.concatenate([k + 0.5, k])
+    for steps in range(1, 9):
+        up = halves.copy()
+        dn = halves.copy()
+        for _ in range(steps):
+            up = np.nextafter(up, np.inf)
+            dn = np.nextafter(dn, -np.inf)
+        parts += [up, dn]
+    parts.append(np.array([0.5, np.nextafter(0.5, 0), np.nextafter(0.5, 1), -0.5, -0.0, 0.0,
+                           2.0 ** 52, 2.0 ** 52 + 1, 2.0 ** 53, 999.5, np.nan, -np.inf]))
+    w = np.concatenate(parts)
+    hi = 1000 - 1
+
+    def ref(v):
+        with np.errstate(invalid="ignore"):
+            r = np.where(np.isfinite(v), v, -1.0)
+            return np.clip(r, 0, hi).astype(np.int64)
+
+    def sat_trunc(t):
+        t = np.where(np.isnan(t), 0.0, t)
+        return np.clip(np.trunc(t), -2.0 ** 31, 2.0 ** 31 - 1).astype(np.int64)
+
+    bias = 0.5 - 2.0 ** -54
+    got_r = np.clip(sat_trunc(w + bias), 0, hi)
+    got_f = np.clip(sat_trunc(w), 0, hi)
+    with np.errstate(invalid="ignore"):
+        np.testing.assert_array_equal(got_r, ref(_round_half_away(w)))
+        np.testing.assert_array_equal(got_f, ref(np.floor(w)))
+    # the naive bias 0.5 fails on exactly one double
+    bad = np.nextafter(0.5, 0)
+    assert sat_trunc(np.array([bad + 0.5]))[0] == 1 and _round_half_away(bad) == 0
