@@ -24,6 +24,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# L2<->fabric bytes per voxel step of trace_kernel on the 256^3 workload:
+# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a 1 GiB copy) over
+# 1.017e11 voxel steps, profiles/trace_kernel_plain-256cube.md.
+TRAFFIC_BYTES_PER_STEP = {"plain-256^3-stride2": 0.885}
 
 
 def cpu_cores() -> int:
@@ -172,7 +176,10 @@ def main():
         "rays_per_s": 2.0 * N * value,
         "voxel_steps_per_s": vsteps / (ms_per_step * 1e-3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (TRAFFIC_BYTES_PER_STEP[wl.name] * vsteps
+                                 if wl.name in TRAFFIC_BYTES_PER_STEP and not args.reflections
+                                 else None),
                      "kernel": "trace_kernel", "kernel_ms": k_ms, "bytes_per_launch": vsteps},
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
