@@ -30,7 +30,8 @@ EXPORTED = (
     "ray_tracing_overall", "ray_tracing_gpu_overall", "ray_tracing_single",
     "ray_tracing_single_mp", "anacor_last_error", "anacor_device_count", "anacor_ctx_create",
     "anacor_ctx_destroy", "anacor_ctx_set_tile", "anacor_ctx_set_voxel_block", "anacor_ctx_run", "anacor_ctx_paths",
-    "anacor_ctx_stats", "anacor_reflection_geometry",
+    "anacor_ctx_stats", "anacor_reflection_geometry", "anacor_batch_create",
+    "anacor_batch_destroy", "anacor_batch_sample", "anacor_batch_run", "anacor_batch_stats",
 )
 
 PATH_DTYPE = np.dtype([("face", np.int32, 2), ("len", np.int32, 2), ("nseg", np.int32, 2),
@@ -42,6 +43,12 @@ assert PATH_DTYPE.itemsize == 120
 class Stats(ct.Structure):
     _fields_ = [("kernel_ms", ct.c_double), ("voxel_steps", ct.c_uint64),
                 ("workgroups", ct.c_int64), ("n_voxels", ct.c_int64)]
+
+
+class Sample(ct.Structure):
+    _fields_ = [("volume", ct.c_void_p), ("shape", ct.c_int64 * 3), ("coords", ct.c_void_p),
+                ("n_coords", ct.c_int64), ("voxel_size", ct.c_double * 3),
+                ("coefficients", ct.c_double * 4)]
 
 
 class AnacorError(RuntimeError):
@@ -74,6 +81,16 @@ def _prototypes(lib):
     lib.anacor_ctx_paths.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp]
     lib.anacor_ctx_stats.restype = ct.c_int
     lib.anacor_ctx_stats.argtypes = [vp, ct.POINTER(Stats)]
+    lib.anacor_batch_create.restype = vp
+    lib.anacor_batch_create.argtypes = [ct.POINTER(Sample), i32, i32, i32]
+    lib.anacor_batch_destroy.restype = None
+    lib.anacor_batch_destroy.argtypes = [vp]
+    lib.anacor_batch_sample.restype = vp
+    lib.anacor_batch_sample.argtypes = [vp, i32]
+    lib.anacor_batch_run.restype = ct.c_int
+    lib.anacor_batch_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, vp]
+    lib.anacor_batch_stats.restype = ct.c_int
+    lib.anacor_batch_stats.argtypes = [vp, ct.POINTER(Stats)]
     lib.anacor_reflection_geometry.restype = ct.c_int
     lib.anacor_reflection_geometry.argtypes = [vp, vp, i64, vp, vp, vp, i64, vp, vp]
     # drop-in entry points, with the argtypes the reference's binding uses

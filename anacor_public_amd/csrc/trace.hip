@@ -37,6 +37,23 @@
 #include "../../include/anacor_mi355x.h"
 
 namespace anacor {
+struct Context;
+}
+
+// The public handle types of include/anacor_mi355x.h.
+struct anacor_ctx {
+  anacor::Context* c;
+};
+
+namespace anacor {
+struct Batch;
+}
+
+struct anacor_batch {
+  anacor::Batch* b;
+};
+
+namespace anacor {
 
 // ------------------------------------------------------------ error state
 
@@ -415,33 +432,32 @@ __device__ __forceinline__ void merge_pair(float& s, float& c, float os, float o
   c = (c + oc) - e;
 }
 
-template <int CAP, int U, bool GUARD, bool FULL, bool WIDE, bool F64, bool PATHS>
-__global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
-  __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
-  __shared__ double red64[kBlock / kWave];
-  __shared__ float red32[2][kBlock / kWave];
-  __shared__ unsigned long long red_steps[kBlock / kWave];
-
-  const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
-  if (lb >= a.n_blocks) return;
-  const int refl = lb / a.tiles;
-  const int tile = lb - refl * a.tiles;
-  const ReflRule* __restrict__ rule = a.rules + refl;
-  const int tid = (int)threadIdx.x;
-  lds_u32* seg = (lds_u32*)(seg_lds + tid);
+// One sample as the kernels see it.
+struct SampleView {
   Volume V;
-  V.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.vol, (short)0,
-                                             a.Z * a.Y * a.X, 0x00020000);
-  V.Z = a.Z; V.Y = a.Y; V.X = a.X;
+  const int4* coords;          // (z, y, x, caller index), wave-grouped order
+  int N;
+  const ReflRule* rules;       // in processing order
+  double vs[3];
+  double mu[4];
+};
 
-  float ks = 0.f, kc = 0.f;   // Kahan sum (value ks - kc)
-  double acc64 = 0.0;
-  unsigned long long steps = 0;
-  unsigned err = 0;
+// Accumulators of one lane over its voxels of a tile.
+struct TileAcc {
+  float ks, kc;                // Kahan sum, value ks - kc
+  double acc64;
+  unsigned long long steps;
+  unsigned err;
+};
 
-  const int v_end = min(a.N, (tile + 1) * a.vox_per_tile);
-  for (int v = tile * a.vox_per_tile + tid; v < v_end; v += kBlock) {
-    const int4 co = a.coords[v];
+// Trace the voxels [v_begin, v_end) of reflection `rule` with this lane's
+// stride; accumulate exp(-sum mu*L) (or write path records).
+template <int CAP, int U, bool GUARD, bool FULL, bool WIDE, bool F64, bool PATHS>
+__device__ __forceinline__ void trace_tile(const SampleView& S, const ReflRule* __restrict__ rule,
+                                           int v_begin, int v_end, int is_exp, lds_u32* seg,
+                                           TileAcc& acc, PathRecord* __restrict__ paths) {
+  for (int v = v_begin + (int)threadIdx.x; v < v_end; v += kBlock) {
+    const int4 co = S.coords[v];
     double Lr[2][4];
     int lens[2], nsegs[2], faces[2], lasts[2][3];
     unsigned verr = 0;
@@ -449,8 +465,8 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       const RayRule* rr = &rule->ray[k];
-      faces[k] = exit_face(co.x, co.y, co.z, rr->dir[0], rr->dir[1], rr->dir[2], a.Z, a.Y, a.X,
-                           rr->neg != 0);
+      faces[k] = exit_face(co.x, co.y, co.z, rr->dir[0], rr->dir[1], rr->dir[2], S.V.Z, S.V.Y,
+                           S.V.X, rr->neg != 0);
     }
     if (faces[0] == 0 || faces[1] == 0) verr = ANACOR_RAY_ERR_FACE;
 #pragma unroll
@@ -460,16 +476,16 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
       lasts[k][0] = lasts[k][1] = lasts[k][2] = 0;
       if (verr) continue;
       RayLane L;
-      if (!make_lane(&rule->ray[k], faces[k], co.x, co.y, co.z, V, L)) {
+      if (!make_lane(&rule->ray[k], faces[k], co.x, co.y, co.z, S.V, L)) {
         verr = ANACOR_RAY_ERR_ASSERT;
         continue;
       }
-      walk_ray<CAP, U, GUARD, FULL, WIDE>(L, V, a.vs, seg, lens[k], nsegs[k], lasts[k], Lr[k]);
-      steps += (unsigned long long)lens[k];
+      walk_ray<CAP, U, GUARD, FULL, WIDE>(L, S.V, S.vs, seg, lens[k], nsegs[k], lasts[k], Lr[k]);
+      acc.steps += (unsigned long long)lens[k];
     }
-    err |= verr;
+    acc.err |= verr;
     if (PATHS) {
-      PathRecord& p = a.paths[(size_t)a.order[refl] * a.N + co.w];
+      PathRecord& p = paths[co.w];
       p.err = (int)verr;
       p.pad = 0;
 #pragma unroll
@@ -483,57 +499,104 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
       continue;
     }
     // cal_rate (ray_tracing_basic.c:1554-1583), same association order
-    const double r = a.mu[0] * (Lr[0][0] + Lr[1][0]) + a.mu[1] * (Lr[0][1] + Lr[1][1]) +
-                     a.mu[2] * (Lr[0][2] + Lr[1][2]) + a.mu[3] * (Lr[0][3] + Lr[1][3]);
+    const double r = S.mu[0] * (Lr[0][0] + Lr[1][0]) + S.mu[1] * (Lr[0][1] + Lr[1][1]) +
+                     S.mu[2] * (Lr[0][2] + Lr[1][2]) + S.mu[3] * (Lr[0][3] + Lr[1][3]);
     if (F64) {
-      acc64 += a.is_exp ? exp(-r) : r;
+      acc.acc64 += is_exp ? exp(-r) : r;
     } else {
-      const float val = a.is_exp ? exp_neg(r) : (float)r;
-      const float y = val - kc;
-      const float t = ks + y;
-      kc = (t - ks) - y;
-      ks = t;
+      const float val = is_exp ? exp_neg(r) : (float)r;
+      const float y = val - acc.kc;
+      const float t = acc.ks + y;
+      acc.kc = (t - acc.ks) - y;
+      acc.ks = t;
     }
   }
+}
 
-  // ---- workgroup reduction (wave shuffles, then one LDS round)
+struct ReduceLds {
+  double r64[kBlock / kWave];
+  float r32[2][kBlock / kWave];
+  unsigned long long steps[kBlock / kWave];
+};
+
+// Workgroup reduction of the lanes' accumulators (wave shuffles, one LDS
+// round, fixed order, so results are deterministic); thread 0 publishes.
+template <bool F64, bool PATHS>
+__device__ __forceinline__ void reduce_tile(TileAcc& acc, ReduceLds& red, unsigned* err_word,
+                                            unsigned long long* steps_total, float2* p32,
+                                            double* p64) {
+  const int tid = (int)threadIdx.x;
   const int lane = tid & (kWave - 1), wave = tid / kWave;
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
-    steps += __shfl_xor(steps, off);
-    err |= __shfl_xor(err, off);
+    acc.steps += __shfl_xor(acc.steps, off);
+    acc.err |= __shfl_xor(acc.err, off);
     if (!PATHS) {
       if (F64) {
-        acc64 += __shfl_xor(acc64, off);
+        acc.acc64 += __shfl_xor(acc.acc64, off);
       } else {
-        const float os = __shfl_xor(ks, off), oc = __shfl_xor(kc, off);
-        merge_pair(ks, kc, os, oc);
+        const float os = __shfl_xor(acc.ks, off), oc = __shfl_xor(acc.kc, off);
+        merge_pair(acc.ks, acc.kc, os, oc);
       }
     }
   }
   if (lane == 0) {
-    red_steps[wave] = steps;
-    if (F64) red64[wave] = acc64;
-    red32[0][wave] = ks;
-    red32[1][wave] = kc;
-    if (err) atomicOr(&a.err[refl], err);
+    red.steps[wave] = acc.steps;
+    red.r64[wave] = acc.acc64;
+    red.r32[0][wave] = acc.ks;
+    red.r32[1][wave] = acc.kc;
+    if (acc.err) atomicOr(err_word, acc.err);
   }
   __syncthreads();
   if (tid == 0) {
-    unsigned long long st = red_steps[0];
-    double s64 = red64[0];
-    float s = red32[0][0], c = red32[1][0];
+    unsigned long long st = red.steps[0];
+    double s64 = red.r64[0];
+    float s = red.r32[0][0], c = red.r32[1][0];
     for (int w = 1; w < kBlock / kWave; ++w) {
-      st += red_steps[w];
-      if (F64) s64 += red64[w];
-      merge_pair(s, c, red32[0][w], red32[1][w]);
+      st += red.steps[w];
+      s64 += red.r64[w];
+      merge_pair(s, c, red.r32[0][w], red.r32[1][w]);
     }
-    atomicAdd(a.steps, st);
+    atomicAdd(steps_total, st);
     if (!PATHS) {
-      if (F64) a.partial64[lb] = s64;
-      else a.partial32[lb] = make_float2(s, c);
+      if (F64) *p64 = s64;
+      else *p32 = make_float2(s, c);
     }
   }
+}
+
+__device__ __forceinline__ Volume make_volume(const uint8_t* vol, int Z, int Y, int X) {
+  Volume V;
+  V.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)vol, (short)0, Z * Y * X, 0x00020000);
+  V.Z = Z; V.Y = Y; V.X = X;
+  return V;
+}
+
+template <int CAP, int U, bool GUARD, bool FULL, bool WIDE, bool F64, bool PATHS>
+__global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
+  __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
+  __shared__ ReduceLds red;
+
+  const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
+  if (lb >= a.n_blocks) return;
+  const int refl = lb / a.tiles;
+  const int tile = lb - refl * a.tiles;
+  SampleView S;
+  S.V = make_volume(a.vol, a.Z, a.Y, a.X);
+  S.coords = a.coords;
+  S.N = a.N;
+  S.rules = a.rules;
+  for (int i = 0; i < 3; ++i) S.vs[i] = a.vs[i];
+  for (int i = 0; i < 4; ++i) S.mu[i] = a.mu[i];
+  lds_u32* seg = (lds_u32*)(seg_lds + threadIdx.x);
+
+  TileAcc acc = {0.f, 0.f, 0.0, 0ull, 0u};
+  const int v_begin = tile * a.vox_per_tile;
+  const int v_end = min(a.N, v_begin + a.vox_per_tile);
+  trace_tile<CAP, U, GUARD, FULL, WIDE, F64, PATHS>(
+      S, a.rules + refl, v_begin, v_end, a.is_exp, seg, acc,
+      PATHS ? a.paths + (size_t)a.order[refl] * a.N : nullptr);
+  reduce_tile<F64, PATHS>(acc, red, &a.err[refl], a.steps, a.partial32 + lb, a.partial64 + lb);
 }
 
 // Per reflection: combine the tile partials in tile order and divide by N.
@@ -554,6 +617,123 @@ __global__ void finish_kernel(const float2* __restrict__ p32, const double* __re
     sum = (double)s - (double)c;
   }
   out[order[r]] = err[r] ? __builtin_nan("") : sum / (double)N;
+}
+
+// ------------------------------------------------------------ batched path
+//
+// Several independent samples (own volume, voxel list, reflections) traced by
+// one persistent kernel.  Work items (sample, reflection, tile) are numbered
+// sample by sample, reflections in processing (direction) order, and split
+// into 8 contiguous chunks, one queue per XCD: workgroup b serves the queue of
+// the XCD it was dispatched to (b mod 8) and steals from the next queues once
+// its own is drained, so every workgroup ends once all queues are empty.
+
+struct BatchSample {
+  const uint8_t* vol;
+  int Z, Y, X, N;
+  const int4* coords;
+  const ReflRule* rules;
+  const int* order;
+  int R, tiles, vpt, pad;
+  long long item_base;
+  double vs[3];
+  double mu[4];
+  float2* p32;
+  double* p64;
+  unsigned* err;
+  double* out;
+};
+
+struct BatchArgs {
+  const BatchSample* samples;
+  int S;
+  int is_exp;
+  long long total;             // work items
+  long long chunk;             // items per XCD queue
+  unsigned long long* cursor;  // [8] per-queue cursors
+  unsigned long long* steps;
+};
+
+template <int CAP, int U, bool GUARD, bool FULL, bool WIDE, bool F64>
+__global__ __launch_bounds__(kBlock) void batch_kernel(BatchArgs a) {
+  __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
+  __shared__ ReduceLds red;
+  __shared__ long long item_sh;
+  lds_u32* seg = (lds_u32*)(seg_lds + threadIdx.x);
+  int q = (int)(blockIdx.x & 7u);
+  int drained = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      long long it = -1;
+      while (drained < 8) {
+        const unsigned long long k = atomicAdd(&a.cursor[q], 1ull);
+        const long long idx = (long long)q * a.chunk + (long long)k;
+        if ((long long)k < a.chunk && idx < a.total) { it = idx; break; }
+        q = (q + 1) & 7;      // this queue is empty: move on to the next one
+        ++drained;
+      }
+      item_sh = it;
+    }
+    __syncthreads();
+    const long long it = item_sh;
+    if (it < 0) break;        // uniform: every queue is drained
+    int s = 0;
+    while (s + 1 < a.S && a.samples[s + 1].item_base <= it) ++s;
+    const BatchSample& B = a.samples[s];
+    const int local = (int)(it - B.item_base);
+    const int refl = local / B.tiles;
+    const int tile = local - refl * B.tiles;
+    SampleView S;
+    S.V = make_volume(B.vol, B.Z, B.Y, B.X);
+    S.coords = B.coords;
+    S.N = B.N;
+    S.rules = B.rules;
+    for (int i = 0; i < 3; ++i) S.vs[i] = B.vs[i];
+    for (int i = 0; i < 4; ++i) S.mu[i] = B.mu[i];
+    TileAcc acc = {0.f, 0.f, 0.0, 0ull, 0u};
+    const int v_begin = tile * B.vpt;
+    const int v_end = min(B.N, v_begin + B.vpt);
+    trace_tile<CAP, U, GUARD, FULL, WIDE, F64, false>(S, B.rules + refl, v_begin, v_end, a.is_exp,
+                                                      seg, acc, nullptr);
+    reduce_tile<F64, false>(acc, red, &B.err[refl], a.steps, B.p32 + local, B.p64 + local);
+  }
+}
+
+// Clears the queue cursors, the step counter and every sample's error words
+// (a kernel rather than hipMemsetAsync, so the graph holds kernels only).
+__global__ void batch_clear_kernel(const BatchSample* __restrict__ samples, int S,
+                                   unsigned long long* cursor, unsigned long long* steps) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t < 8) cursor[t] = 0ull;
+  if (t == 8) *steps = 0ull;
+  for (int s = 0; s < S; ++s) {
+    const BatchSample& B = samples[s];
+    for (int r = t; r < B.R; r += (int)(gridDim.x * blockDim.x)) B.err[r] = 0u;
+  }
+}
+
+// Per (sample, reflection): merge the tile partials in order, divide by N.
+__global__ void batch_finish_kernel(const BatchSample* __restrict__ samples, int S, int f64,
+                                    long long total_refl) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total_refl) return;
+  long long base = 0;
+  int s = 0;
+  while (s < S && g >= base + samples[s].R) { base += samples[s].R; ++s; }
+  const BatchSample& B = samples[s];
+  const int r = (int)(g - base);
+  double sum = 0.0;
+  if (f64) {
+    for (int t = 0; t < B.tiles; ++t) sum += B.p64[(size_t)r * B.tiles + t];
+  } else {
+    float a = 0.f, c = 0.f;
+    for (int t = 0; t < B.tiles; ++t) {
+      const float2 q = B.p32[(size_t)r * B.tiles + t];
+      merge_pair(a, c, q.x, q.y);
+    }
+    sum = (double)a - (double)c;
+  }
+  B.out[B.order[r]] = B.err[r] ? __builtin_nan("") : sum / (double)B.N;
 }
 
 // ------------------------------------------------------------ host side
@@ -653,6 +833,9 @@ static void launch_trace(const TraceArgs& a, int grid, hipStream_t s) {
 }
 
 using LaunchFn = void (*)(const TraceArgs&, int, hipStream_t);
+using LaunchBatchFn = void (*)(const BatchArgs&, int, hipStream_t);
+
+
 
 template <int I>
 static constexpr LaunchFn launcher() {
@@ -990,15 +1173,248 @@ static const uint8_t* flatten_table(int8_t*** table, const int64_t shape[3],
   return storage.data();
 }
 
+
+// ------------------------------------------------------------ batch (host)
+
+template <bool GUARD, bool FULL, bool WIDE, bool F64>
+static void launch_batch(const BatchArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((batch_kernel<kCap, kUnroll, GUARD, FULL, WIDE, F64>), dim3(grid),
+                     dim3(kBlock), 0, s, a);
+}
+
+template <int I>
+static constexpr LaunchBatchFn batch_launcher() {
+  return &launch_batch<(I & 1) != 0, (I & 2) != 0, (I & 4) != 0, (I & 8) != 0>;
+}
+
+template <int... I>
+static constexpr std::array<LaunchBatchFn, sizeof...(I)> batch_table(
+    std::integer_sequence<int, I...>) {
+  return {batch_launcher<I>()...};
+}
+
+static const std::array<LaunchBatchFn, 16> kLaunchBatch =
+    batch_table(std::make_integer_sequence<int, 16>{});
+
+struct Batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Context*> samples;
+  std::vector<anacor_ctx> handles;
+  int full = 0;
+  BatchSample* d_desc = nullptr;
+  size_t desc_cap = 0;
+  unsigned long long* d_counters = nullptr;   // [8] cursors, [8] steps
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<long long> graph_key;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0.f;
+  unsigned long long last_steps = 0;
+  int grid = 0;
+  std::vector<BatchSample> h_desc;
+  std::mutex mu;
+};
+
+static void batch_destroy(Batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_counters) (void)hipFree(b->d_counters);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  for (Context* c : b->samples) ctx_destroy(c);
+  delete b;
+}
+
+static int batch_create(const anacor_sample_t* in, int n, int full, int device, Batch** out) {
+  *out = nullptr;
+  if (!in || n <= 0 || n > 4096) {
+    set_error(ANACOR_ERR_ARG, "bad sample list (n=%d)", n);
+    return ANACOR_ERR_ARG;
+  }
+  Batch* b = new Batch();
+  b->full = full;
+  for (int i = 0; i < n; ++i) {
+    Context* c = nullptr;
+    const int rc = ctx_create(in[i].volume, in[i].shape, in[i].coords, nullptr, in[i].n_coords,
+                              in[i].voxel_size, in[i].coefficients, full, device, &c);
+    if (rc != ANACOR_OK) {
+      batch_destroy(b);
+      return rc;
+    }
+    b->samples.push_back(c);
+  }
+  b->device = b->samples[0]->device;
+  b->handles.resize((size_t)n);
+  for (int i = 0; i < n; ++i) b->handles[(size_t)i].c = b->samples[(size_t)i];
+#define BATCH_TRY(expr)                                                            \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_error(ANACOR_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+      batch_destroy(b);                                                            \
+      return ANACOR_ERR_HIP;                                                       \
+    }                                                                              \
+  } while (0)
+  BATCH_TRY(hipSetDevice(b->device));
+  BATCH_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  BATCH_TRY(hipEventCreate(&b->ev0));
+  BATCH_TRY(hipEventCreate(&b->ev1));
+  BATCH_TRY(hipMalloc(&b->d_counters, 16 * sizeof(unsigned long long)));
+  // persistent grid: every workgroup the chip holds at once, a multiple of 8
+  int cus = 0, per_cu = 0;
+  BATCH_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+  BATCH_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, (const void*)batch_kernel<kCap, kUnroll, false, false, false, false>, kBlock, 0));
+#undef BATCH_TRY
+  b->grid = std::max(8, (cus * std::max(per_cu, 1)) / 8 * 8);
+  *out = b;
+  return ANACOR_OK;
+}
+
+static int batch_run(Batch* b, const double* const* s1, const double* const* omega,
+                     const int64_t* R, const double* raw_xray, const double* omega_axis,
+                     const double* F, int flags, double* const* out) {
+  std::lock_guard<std::mutex> lk(b->mu);
+  HIP_TRY(hipSetDevice(b->device));
+  const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
+  const int S = (int)b->samples.size();
+  bool guard = false, wide = false;
+  long long items = 0, total_refl = 0;
+  b->h_desc.resize((size_t)S);
+  for (int i = 0; i < S; ++i) {
+    Context* c = b->samples[(size_t)i];
+    const int64_t Ri = R[i];
+    if (Ri < 0 || (Ri > 0 && (!s1[i] || !omega[i] || !out[i]))) {
+      set_error(ANACOR_ERR_ARG, "sample %d: bad reflection arguments", i);
+      return ANACOR_ERR_ARG;
+    }
+    c->h_rules.resize((size_t)std::max<int64_t>(Ri, 1));
+    host_setup(s1[i], omega[i], Ri, raw_xray, omega_axis, F, max_dim(c), c->h_rules.data(),
+               nullptr, host_threads(0));
+    order_reflections(c, c->h_rules.data(), Ri);
+    c->h_sorted.resize((size_t)std::max<int64_t>(Ri, 1));
+    for (int64_t r = 0; r < Ri; ++r) {
+      c->h_sorted[(size_t)r] = c->h_rules[(size_t)c->h_order[(size_t)r]];
+      guard = guard || c->h_sorted[(size_t)r].ray[0].guard || c->h_sorted[(size_t)r].ray[1].guard;
+    }
+    wide = wide || (int64_t)c->Y * c->X >= (1 << 23) || c->Z >= (1 << 23);
+    const int vpt = c->N > 0 ? std::min(c->tile_voxels > 0 ? c->tile_voxels : kBlock * 64,
+                                        std::max(c->N, 1))
+                             : 1;
+    const int tiles = c->N > 0 ? (c->N + vpt - 1) / vpt : 0;
+    const int rc = ensure_run_buffers(c, std::max<int64_t>(Ri, 1),
+                                      std::max<int64_t>(Ri * tiles, 1), f64);
+    if (rc) return rc;
+    if (Ri > 0) {
+      HIP_TRY(hipMemcpyAsync(c->d_rules, c->h_sorted.data(), (size_t)Ri * sizeof(ReflRule),
+                             hipMemcpyHostToDevice, b->stream));
+      HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order.data(), (size_t)Ri * sizeof(int),
+                             hipMemcpyHostToDevice, b->stream));
+    }
+    BatchSample& d = b->h_desc[(size_t)i];
+    d.vol = c->d_vol;
+    d.Z = c->Z; d.Y = c->Y; d.X = c->X; d.N = c->N;
+    d.coords = c->d_coords;
+    d.rules = c->d_rules;
+    d.order = c->d_order;
+    d.R = (int)Ri;
+    d.tiles = std::max(tiles, 1);
+    d.vpt = vpt;
+    d.pad = 0;
+    d.item_base = items;
+    for (int k = 0; k < 3; ++k) d.vs[k] = c->vs[k];
+    for (int k = 0; k < 4; ++k) d.mu[k] = c->mu[k];
+    d.p32 = (float2*)c->d_partial;
+    d.p64 = (double*)c->d_partial;
+    d.err = c->d_err;
+    d.out = c->d_out;
+    items += (c->N > 0) ? Ri * tiles : 0;
+    total_refl += Ri;
+  }
+  if ((size_t)S > b->desc_cap) {
+    if (b->d_desc) HIP_TRY(hipFree(b->d_desc));
+    b->d_desc = nullptr;
+    HIP_TRY(hipMalloc(&b->d_desc, (size_t)S * sizeof(BatchSample)));
+    b->desc_cap = (size_t)S;
+  }
+  HIP_TRY(hipMemcpyAsync(b->d_desc, b->h_desc.data(), (size_t)S * sizeof(BatchSample),
+                         hipMemcpyHostToDevice, b->stream));
+
+  // the captured graph depends on the launch shape and every buffer address
+  std::vector<long long> key = {S, items, total_refl, f64, guard, wide, b->full,
+                                (long long)(uintptr_t)b->d_desc};
+  for (int i = 0; i < S; ++i) {
+    const BatchSample& d = b->h_desc[(size_t)i];
+    key.push_back(d.R);
+    key.push_back(d.tiles);
+  }
+  if (key != b->graph_key || !b->exec) {
+    if (b->exec) HIP_TRY(hipGraphExecDestroy(b->exec));
+    if (b->graph) HIP_TRY(hipGraphDestroy(b->graph));
+    b->exec = nullptr;
+    b->graph = nullptr;
+    BatchArgs a;
+    a.samples = b->d_desc;
+    a.S = S;
+    a.is_exp = (flags & ANACOR_FLAG_NO_EXP) ? 0 : 1;
+    a.total = items;
+    a.chunk = (items + 7) / 8;
+    a.cursor = b->d_counters;
+    a.steps = b->d_counters + 8;
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(batch_clear_kernel, dim3(64), dim3(256), 0, b->stream, b->d_desc, S,
+                       a.cursor, a.steps);
+    if (items > 0) {
+      const int i = (guard ? 1 : 0) | (b->full ? 2 : 0) | (wide ? 4 : 0) | (f64 ? 8 : 0);
+      kLaunchBatch[i](a, b->grid, b->stream);
+    }
+    if (total_refl > 0)
+      hipLaunchKernelGGL(batch_finish_kernel, dim3((unsigned)((total_refl + 255) / 256)),
+                         dim3(256), 0, b->stream, b->d_desc, S, f64 ? 1 : 0, total_refl);
+    HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
+    HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_key = key;
+  }
+  HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  for (int i = 0; i < S; ++i) {
+    Context* c = b->samples[(size_t)i];
+    if (R[i] > 0)
+      HIP_TRY(hipMemcpyAsync(out[i], c->d_out, (size_t)R[i] * sizeof(double),
+                             hipMemcpyDeviceToHost, b->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(&b->last_steps, b->d_counters + 8, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  HIP_TRY(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+  for (int i = 0; i < S; ++i) {
+    Context* c = b->samples[(size_t)i];
+    if (c->N == 0)
+      for (int64_t r = 0; r < R[i]; ++r) out[i][r] = __builtin_nan("");
+  }
+  // report the first reflection the reference would have aborted on
+  for (int i = 0; i < S; ++i)
+    for (int64_t r = 0; r < R[i]; ++r)
+      if (out[i][r] != out[i][r] && b->samples[(size_t)i]->N > 0) {
+        set_error(ANACOR_ERR_GEOMETRY, "sample %d reflection %lld: the reference aborts here; "
+                  "result set to NaN", i, (long long)r);
+        return ANACOR_ERR_GEOMETRY;
+      }
+  return ANACOR_OK;
+}
+
 }  // namespace anacor
 
 // ===================================================================== C-ABI
 
 using namespace anacor;
-
-struct anacor_ctx {
-  Context* c;
-};
 
 extern "C" {
 
@@ -1078,6 +1494,47 @@ int anacor_ctx_stats(const anacor_ctx* ctx, anacor_stats_t* st) {
   st->voxel_steps = ctx->c->last_steps;
   st->workgroups = ctx->c->last_blocks;
   st->n_voxels = ctx->c->N;
+  return ANACOR_OK;
+}
+
+anacor_batch* anacor_batch_create(const anacor_sample_t* samples, int n_samples,
+                                  int full_iteration, int device) {
+  clear_error();
+  Batch* b = nullptr;
+  if (batch_create(samples, n_samples, full_iteration, device, &b) != ANACOR_OK) return nullptr;
+  return new anacor_batch{b};
+}
+
+void anacor_batch_destroy(anacor_batch* batch) {
+  if (!batch) return;
+  batch_destroy(batch->b);
+  delete batch;
+}
+
+anacor_ctx* anacor_batch_sample(anacor_batch* batch, int i) {
+  if (!batch || i < 0 || i >= (int)batch->b->handles.size()) return nullptr;
+  return &batch->b->handles[(size_t)i];
+}
+
+int anacor_batch_run(anacor_batch* batch, const double* const* s1, const double* const* omega,
+                     const int64_t* R, const double raw_xray[3], const double omega_axis[3],
+                     const double F[9], int flags, double* const* out) {
+  clear_error();
+  if (!batch || !s1 || !omega || !R || !out || !raw_xray || !omega_axis || !F) {
+    set_error(ANACOR_ERR_ARG, "null argument");
+    return ANACOR_ERR_ARG;
+  }
+  return batch_run(batch->b, s1, omega, R, raw_xray, omega_axis, F, flags, out);
+}
+
+int anacor_batch_stats(const anacor_batch* batch, anacor_stats_t* st) {
+  if (!batch || !st) return ANACOR_ERR_ARG;
+  st->kernel_ms = batch->b->last_ms;
+  st->voxel_steps = batch->b->last_steps;
+  st->workgroups = batch->b->grid;
+  int64_t n = 0;
+  for (Context* c : batch->b->samples) n += c->N;
+  st->n_voxels = n;
   return ANACOR_OK;
 }
 

@@ -197,12 +197,12 @@ class Tracer:
         self.close()
 
     def set_tile(self, voxels_per_tile: int):
-        self._lib.anacor_ctx_set_tile(self._h, int(voxels_per_tile))
+        self._lib.anacor_ctx_set_tile(self.handle, int(voxels_per_tile))
 
     def set_voxel_block(self, bz: int, by: int, bx: int):
         """Group crystal voxels into waves by (bz, by, bx) voxel blocks
         ((0, 0, 0): Morton order)."""
-        _lib.check(self._lib.anacor_ctx_set_voxel_block(self._h, int(bz), int(by), int(bx)),
+        _lib.check(self._lib.anacor_ctx_set_voxel_block(self.handle, int(bz), int(by), int(bx)),
                    self._lib)
 
     def run(self, s1, omega, raw_xray, omega_axis, F, f64_mean: bool = False, is_exp: bool = True,
@@ -212,7 +212,7 @@ class Tracer:
         R = len(om)
         out = np.empty(R, dtype=np.float64)
         flags = (_lib.FLAG_F64_MEAN if f64_mean else 0) | (0 if is_exp else _lib.FLAG_NO_EXP)
-        rc = self._lib.anacor_ctx_run(self._h, _p(s1), _p(om), R, _p(_f64(raw_xray)),
+        rc = self._lib.anacor_ctx_run(self.handle, _p(s1), _p(om), R, _p(_f64(raw_xray)),
                                       _p(_f64(omega_axis)), _p(_f64(F)), flags, _p(out))
         _lib.check(rc, self._lib, allow=(_lib.ANACOR_OK,) if strict else
                    (_lib.ANACOR_OK, _lib.ANACOR_ERR_GEOMETRY))
@@ -224,16 +224,31 @@ class Tracer:
         om = _f64(omega)
         R = len(om)
         out = np.zeros(R * self.n_voxels, dtype=_lib.PATH_DTYPE)
-        rc = self._lib.anacor_ctx_paths(self._h, _p(s1), _p(om), R, _p(_f64(raw_xray)),
+        rc = self._lib.anacor_ctx_paths(self.handle, _p(s1), _p(om), R, _p(_f64(raw_xray)),
                                         _p(_f64(omega_axis)), _p(_f64(F)), _p(out))
         _lib.check(rc, self._lib)
         return out.reshape(R, self.n_voxels)
 
     def stats(self) -> dict:
         st = _lib.Stats()
-        self._lib.anacor_ctx_stats(self._h, ct.byref(st))
+        self._lib.anacor_ctx_stats(self.handle, ct.byref(st))
         return dict(kernel_ms=st.kernel_ms, voxel_steps=int(st.voxel_steps),
                     workgroups=int(st.workgroups), n_voxels=int(st.n_voxels))
+
+    @classmethod
+    def _borrow(cls, handle: int, n_voxels: int, shape) -> "Tracer":
+        """A non-owning Tracer over a handle owned by a Batch."""
+        t = cls.__new__(cls)
+        t._lib = _lib.load()
+        t._h = None
+        t._borrowed = ct.c_void_p(handle)
+        t.n_voxels = n_voxels
+        t.shape = np.array(shape, dtype=np.int64)
+        return t
+
+    @property
+    def handle(self):
+        return self._h if self._h is not None else getattr(self, "_borrowed", None)
 
     @classmethod
     def for_workload(cls, wl, device: int = 0) -> "Tracer":
@@ -256,3 +271,80 @@ def reflection_geometry(s1, omega, raw_xray, omega_axis, F, max_dim: int):
                                               _p(_f64(omega_axis)), _p(_f64(F)), int(max_dim),
                                               _p(tp), _p(dt)), lib)
     return tp, dt
+
+
+class Batch:
+    """Several independent samples traced by one persistent, graph-launched
+    kernel (``anacor_batch_*``).  ``samples`` are Workload-like objects
+    (volume, coords, voxel_size, coefficients); all share full_iteration."""
+
+    def __init__(self, samples, full_iteration: int = 0, device: int = 0):
+        self._lib = _lib.load()
+        self._keep = []
+        arr = (_lib.Sample * len(samples))()
+        for i, wl in enumerate(samples):
+            vol = np.ascontiguousarray(wl.volume).view(np.uint8)
+            co = np.ascontiguousarray(wl.coords, dtype=np.int64).reshape(-1, 3)
+            self._keep += [vol, co]
+            arr[i].volume = vol.ctypes.data
+            arr[i].shape[:] = list(vol.shape)
+            arr[i].coords = co.ctypes.data
+            arr[i].n_coords = len(co)
+            arr[i].voxel_size[:] = [float(v) for v in wl.voxel_size]
+            arr[i].coefficients[:] = [float(v) for v in wl.coefficients]
+        h = self._lib.anacor_batch_create(arr, len(samples), int(full_iteration), int(device))
+        if not h:
+            code, msg = _lib.last_error(self._lib)
+            raise _lib.AnacorError(code, msg)
+        self._h = ct.c_void_p(h)
+        self.n = len(samples)
+        self._sizes = [(len(np.asarray(w.coords).reshape(-1, 3)), np.asarray(w.volume).shape)
+                       for w in samples]
+        self._keep = []   # the library copied volumes and voxel lists to the GPU
+
+    def sample(self, i: int) -> Tracer:
+        h = self._lib.anacor_batch_sample(self._h, int(i))
+        if not h:
+            raise IndexError(i)
+        return Tracer._borrow(h, self._sizes[i][0], self._sizes[i][1])
+
+    def run(self, s1_list, omega_list, raw_xray, omega_axis, F, f64_mean: bool = False,
+            is_exp: bool = True, strict: bool = False):
+        n = self.n
+        s1s = [_f64(a).reshape(-1, 3) for a in s1_list]
+        oms = [_f64(a) for a in omega_list]
+        outs = [np.empty(len(o), dtype=np.float64) for o in oms]
+        p_s1 = (ct.c_void_p * n)(*[a.ctypes.data for a in s1s])
+        p_om = (ct.c_void_p * n)(*[a.ctypes.data for a in oms])
+        p_out = (ct.c_void_p * n)(*[a.ctypes.data for a in outs])
+        Rs = np.array([len(o) for o in oms], dtype=np.int64)
+        flags = (_lib.FLAG_F64_MEAN if f64_mean else 0) | (0 if is_exp else _lib.FLAG_NO_EXP)
+        rc = self._lib.anacor_batch_run(self._h, ct.cast(p_s1, _vp), ct.cast(p_om, _vp), _p(Rs),
+                                        _p(_f64(raw_xray)), _p(_f64(omega_axis)), _p(_f64(F)),
+                                        flags, ct.cast(p_out, _vp))
+        _lib.check(rc, self._lib, allow=(_lib.ANACOR_OK,) if strict else
+                   (_lib.ANACOR_OK, _lib.ANACOR_ERR_GEOMETRY))
+        return outs
+
+    def stats(self) -> dict:
+        st = _lib.Stats()
+        self._lib.anacor_batch_stats(self._h, ct.byref(st))
+        return dict(kernel_ms=st.kernel_ms, voxel_steps=int(st.voxel_steps),
+                    workgroups=int(st.workgroups), n_voxels=int(st.n_voxels))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.anacor_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -145,6 +145,43 @@ typedef struct {
 } anacor_stats_t;
 int anacor_ctx_stats(const anacor_ctx *ctx, anacor_stats_t *stats);
 
+/* ----------------------------------------------------------- batched path */
+
+/* One independent sample of a batch: its label volume, crystal voxels,
+ * voxel size and coefficients, as for anacor_ctx_create. */
+typedef struct {
+    const uint8_t *volume;
+    int64_t        shape[3];
+    const int64_t *coords;
+    int64_t        n_coords;
+    double         voxel_size[3];
+    double         coefficients[4];
+} anacor_sample_t;
+
+/* Several samples traced by one persistent kernel (work items = sample x
+ * reflection x voxel tile, per-XCD queues with stealing) launched through a
+ * captured HIP graph that is replayed while the launch shape is unchanged.
+ * No counterpart in the reference, whose driver runs samples one after the
+ * other (main_lite.py:276-341).  All samples share full_iteration. */
+typedef struct anacor_batch anacor_batch;
+anacor_batch *anacor_batch_create(const anacor_sample_t *samples, int n_samples,
+                                  int full_iteration, int device);
+void anacor_batch_destroy(anacor_batch *batch);
+
+/* Borrowed handle of sample i (for anacor_ctx_set_voxel_block /
+ * anacor_ctx_set_tile / anacor_ctx_run, e.g. by an autotuner); never pass it
+ * to anacor_ctx_destroy. */
+anacor_ctx *anacor_batch_sample(anacor_batch *batch, int i);
+
+/* Trace R[i] reflections of every sample i (s1[i] [R[i]][3], omega[i] [R[i]])
+ * with one goniometer (raw_xray, omega_axis, F); out[i] receives R[i] means. */
+int anacor_batch_run(anacor_batch *batch, const double *const *s1, const double *const *omega,
+                     const int64_t *R, const double raw_xray[3], const double omega_axis[3],
+                     const double F[9], int flags, double *const *out);
+
+/* Statistics of the most recent batch run (kernel_ms covers the whole graph). */
+int anacor_batch_stats(const anacor_batch *batch, anacor_stats_t *stats);
+
 /* Host-side geometry of R reflections (diagnostics): theta_phi[R][4] =
  * (theta, phi) of the incident then the diffracted ray; d_table[R][2][6][3] =
  * per-face per-increment deltas (z, y, x).  Either output may be NULL. */

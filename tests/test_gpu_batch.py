@@ -1,0 +1,101 @@
+"""GPU tests of the batched path (several samples, one persistent kernel in a
+replayed HIP graph) and of the autotuner.
+
+Each sample's result must equal the unbatched path on the same sample
+bit for bit (same voxel order and tiles, so the same partial sums), across
+graph replays with other GPU work in between, and match the FP64 oracle to
+1e-6."""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+@pytest.fixture(scope="module")
+def samples():
+    from anacor_public_amd import workload as W
+    return W.batched_config(R_scale=0.004)
+
+
+def test_batch_matches_unbatched_and_oracle(gpu, oracle, samples):
+    from anacor_public_amd.ray_tracing import Batch, Tracer
+    say(f"case: batch of {len(samples)} samples, R={[len(w.omega) for w in samples]}")
+    args = ([w.s1 for w in samples], [w.omega for w in samples], samples[0].raw_xray,
+            samples[0].omega_axis, samples[0].F)
+    with Batch(samples) as b:
+        outs = b.run(*args)                      # each path picks its own tiles
+        st = b.stats()
+        for i in range(len(samples)):
+            b.sample(i).set_tile(512)
+        pinned = b.run(*args)                    # the same tiles as below
+    total_steps = 0
+    for i, wl in enumerate(samples):
+        with Tracer.for_workload(wl) as t:
+            auto = t.run_workload(wl)
+            total_steps += t.stats()["voxel_steps"]
+            t.set_tile(512)
+            single = t.run_workload(wl)
+        # same voxel order and tiles: the same partial sums, bit for bit
+        np.testing.assert_array_equal(bits(pinned[i]), bits(single), err_msg=f"sample {i}")
+        # different tiles only regroup the compensated FP32 sum
+        np.testing.assert_allclose(outs[i], auto, rtol=1e-7, atol=0, err_msg=f"sample {i}")
+        ref = oracle.trace(wl.subset(3))["result"]
+        np.testing.assert_allclose(outs[i][:3], ref, rtol=1e-6, atol=0, err_msg=f"sample {i}")
+    assert st["voxel_steps"] == total_steps
+
+
+def test_graph_replays_with_eager_work_between(gpu, samples):
+    from anacor_public_amd.ray_tracing import Batch, Tracer
+    say("case: graph replays")
+    wl0 = samples[0]
+    args = ([w.s1 for w in samples], [w.omega for w in samples], wl0.raw_xray, wl0.omega_axis,
+            wl0.F)
+    with Batch(samples) as b, Tracer.for_workload(samples[2]) as eager:
+        first = b.run(*args)
+        for rep in range(4):
+            eager.run_workload(samples[2])                  # unrelated eager launches
+            again = b.run(*args)
+            for i in range(len(samples)):
+                np.testing.assert_array_equal(bits(again[i]), bits(first[i]),
+                                              err_msg=f"replay {rep} sample {i}")
+        # a new launch shape rebuilds the graph
+        short = b.run([w.s1[:5] for w in samples], [w.omega[:5] for w in samples],
+                      *args[2:])
+        for i in range(len(samples)):
+            np.testing.assert_array_equal(bits(short[i]), bits(first[i][:5]))
+        # FP64 mean through the same graph machinery
+        f64 = b.run(*args, f64_mean=True)
+        for i in range(len(samples)):
+            np.testing.assert_allclose(f64[i], first[i], rtol=1e-6, atol=0)
+
+
+def test_autotuner_keeps_results(gpu, oracle, tmp_path, monkeypatch, samples):
+    from anacor_public_amd import autotune
+    from anacor_public_amd.ray_tracing import Batch
+    say("case: autotuner")
+    monkeypatch.setenv("ANACOR_TUNE_CACHE", str(tmp_path / "tune.json"))
+    wls = samples[:3]
+    with Batch(wls) as b:
+        before = b.run([w.s1 for w in wls], [w.omega for w in wls], wls[0].raw_xray,
+                       wls[0].omega_axis, wls[0].F)
+        picks = autotune.tune_batch(b, wls, trial=64,
+                                    blocks=((1, 2, 64), (0, 0, 0), (2, 4, 16)), tiles=(0, 1024))
+        after = b.run([w.s1 for w in wls], [w.omega for w in wls], wls[0].raw_xray,
+                      wls[0].omega_axis, wls[0].F)
+        again = autotune.tune_batch(b, wls, trial=64)         # served from the cache
+    assert all(not p["cached"] for p in picks) and all(p["cached"] for p in again)
+    assert all(len(p["table"]) == 6 for p in picks)
+    for i in range(len(wls)):
+        np.testing.assert_allclose(after[i], before[i], rtol=1e-6, atol=0)
