@@ -42,7 +42,7 @@ assert PATH_DTYPE.itemsize == 120
 
 class Stats(ct.Structure):
     _fields_ = [("kernel_ms", ct.c_double), ("voxel_steps", ct.c_uint64),
-                ("workgroups", ct.c_int64), ("n_voxels", ct.c_int64)]
+                ("workgroups", ct.c_int64), ("n_voxels", ct.c_int64), ("host_ms", ct.c_double)]
 
 
 class Sample(ct.Structure):

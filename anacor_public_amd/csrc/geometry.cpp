@@ -10,6 +10,7 @@
 #include "geometry.h"
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -87,32 +88,52 @@ void set_d(RayRule& r, int face, double dz, double dy, double dx) {
   r.d[face - 1][3] = 0.0;
 }
 
+static bool same_bits(double a, double b) {
+  uint64_t x, y;
+  std::memcpy(&x, &a, 8);
+  std::memcpy(&y, &b, 8);
+  return x == y;
+}
+
 // The six exit-face rules of cal_coord for one ray with angles (t, p).
 // Axis deltas carry the sign with which the reference combines them
-// ("c - inc*r" == "c + inc*(-r)" in IEEE arithmetic).
+// ("c - inc*r" == "c + inc*(-r)" in IEEE arithmetic).  Each libm value is
+// computed once per distinct argument (bit-identical arguments give
+// bit-identical results), e.g. cos(|p|) is cos(p) when p >= +0.
 void face_rules(double t, double p, RayRule& r) {
   const double half_pi = kPi / 2;
   const bool narrow = std::fabs(t) < half_pi;
   const bool pos = t > 0;  // floor when theta > 0, else round (faces 1,2,3,6)
+  const double ap = std::fabs(p), at = std::fabs(t);
+  const bool p_is_ap = same_bits(p, ap), t_is_at = same_bits(t, at);
+
+  const double tan_p = std::tan(p);
+  const double cos_p = std::cos(p);
+  const double tan_t = std::tan(t);
+  const double tan_pi_t = std::tan(kPi - t);
+  const double cos_ap = p_is_ap ? cos_p : std::cos(ap);
+  const double tan_ap = p_is_ap ? tan_p : std::tan(ap);
+  const double sin_ap = std::sin(ap);
+  const double sin_p = p_is_ap ? sin_ap : std::sin(p);
 
   // face 6: x -> 0 (:418-511)
-  set_d(r, 6, std::tan(p), -(std::tan(t) / std::cos(p)), -1.0);
-  r.meta[5] = pack_meta(std::fabs(t) <= half_pi, !pos, true, false, 2, 50);
+  set_d(r, 6, tan_p, -(tan_t / cos_p), -1.0);
+  r.meta[5] = pack_meta(at <= half_pi, !pos, true, false, 2, 50);
 
   // face 1: x -> x_max, exclusive, vacuum run 10 (:1280-1377)
-  set_d(r, 1, -std::tan(p), -(std::tan(kPi - t) / std::cos(std::fabs(p))), 1.0);
-  r.meta[0] = pack_meta(std::fabs(t) >= half_pi, !pos, false, true, 2, 10);
+  set_d(r, 1, -tan_p, -(tan_pi_t / cos_ap), 1.0);
+  r.meta[0] = pack_meta(at >= half_pi, !pos, false, true, 2, 10);
 
   // faces 2 (z -> 0, inclusive) and 3 (z -> z_max) (:512-897)
-  const double rx = 1 / std::tan(std::fabs(p));
+  const double rx = 1 / tan_ap;
   if (narrow) {
-    const double ry = std::tan(t) / std::sin(std::fabs(p));
+    const double ry = tan_t / sin_ap;
     set_d(r, 2, -1.0, -ry, -rx);
     r.meta[1] = pack_meta(p < 0, !pos, true, false, 0, 50);
     set_d(r, 3, 1.0, -ry, -rx);
     r.meta[2] = pack_meta(p > 0, !pos, false, true, 0, 50);
   } else {
-    const double ry = std::tan(kPi - t) / std::sin(std::fabs(p));
+    const double ry = tan_pi_t / sin_ap;
     set_d(r, 2, -1.0, -ry, rx);
     r.meta[1] = pack_meta(p > 0, !pos, true, false, 0, 50);
     set_d(r, 3, 1.0, -ry, rx);
@@ -122,16 +143,18 @@ void face_rules(double t, double p, RayRule& r) {
   // face 4: y -> 0, always floor, inclusive (:899-1102)
   // face 5: y -> y_max, always round (:1104-1278)
   if (narrow) {
-    const double tt = std::tan(std::fabs(t));
-    const double rx4 = std::cos(std::fabs(p)) / tt;
-    set_d(r, 4, std::sin(p) / tt, -1.0, -rx4);
-    set_d(r, 5, std::sin(p) / tt, 1.0, -rx4);
+    const double tt = t_is_at ? tan_t : std::tan(at);
+    const double rx4 = cos_ap / tt;
+    set_d(r, 4, sin_p / tt, -1.0, -rx4);
+    set_d(r, 5, sin_p / tt, 1.0, -rx4);
     r.meta[4] = pack_meta(t < 0, true, false, true, 1, 50);
   } else {
-    const double tt = std::tan(kPi - std::fabs(t));
-    const double rx4 = std::cos(std::fabs(p)) / tt;
-    set_d(r, 4, std::sin(-p) / tt, -1.0, rx4);
-    set_d(r, 5, -(std::sin(p) / tt), 1.0, rx4);
+    const double tt = t_is_at ? tan_pi_t : std::tan(kPi - at);
+    const double rx4 = cos_ap / tt;
+    const double mp = -p;
+    const double sin_mp = same_bits(mp, ap) ? sin_ap : std::sin(mp);
+    set_d(r, 4, sin_mp / tt, -1.0, rx4);
+    set_d(r, 5, -(sin_p / tt), 1.0, rx4);
     r.meta[4] = pack_meta(t < 0, true, true, true, 1, 50);
   }
   r.meta[3] = pack_meta(t > 0, false, true, false, 1, 50);

@@ -25,6 +25,8 @@
 #include <array>
 #include <utility>
 #include <atomic>
+#include <chrono>
+#include <thread>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -117,7 +119,7 @@ struct TraceArgs {
   unsigned* err;               // [R]
   unsigned long long* steps;   // sum of len_path_2
   PathRecord* paths;           // [R * N] in original voxel order (paths kernel)
-  const int* order;            // [R] caller's index of each (sorted) reflection
+  const int* order;            // [R] caller's index of the reflection at each processing position
 };
 
 // Map hardware workgroup id to a logical one so that consecutive logical
@@ -579,8 +581,9 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
 
   const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
   if (lb >= a.n_blocks) return;
-  const int refl = lb / a.tiles;
-  const int tile = lb - refl * a.tiles;
+  const int pos = lb / a.tiles;          // processing position
+  const int tile = lb - pos * a.tiles;
+  const int refl = a.order[pos];         // the caller's reflection index
   SampleView S;
   S.V = make_volume(a.vol, a.Z, a.Y, a.X);
   S.coords = a.coords;
@@ -595,7 +598,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
   const int v_end = min(a.N, v_begin + a.vox_per_tile);
   trace_tile<CAP, U, GUARD, FULL, WIDE, F64, PATHS>(
       S, a.rules + refl, v_begin, v_end, a.is_exp, seg, acc,
-      PATHS ? a.paths + (size_t)a.order[refl] * a.N : nullptr);
+      PATHS ? a.paths + (size_t)refl * a.N : nullptr);
   reduce_tile<F64, PATHS>(acc, red, &a.err[refl], a.steps, a.partial32 + lb, a.partial64 + lb);
 }
 
@@ -616,7 +619,8 @@ __global__ void finish_kernel(const float2* __restrict__ p32, const double* __re
     }
     sum = (double)s - (double)c;
   }
-  out[order[r]] = err[r] ? __builtin_nan("") : sum / (double)N;
+  const int refl = order[r];
+  out[refl] = err[refl] ? __builtin_nan("") : sum / (double)N;
 }
 
 // ------------------------------------------------------------ batched path
@@ -681,8 +685,9 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(BatchArgs a) {
     while (s + 1 < a.S && a.samples[s + 1].item_base <= it) ++s;
     const BatchSample& B = a.samples[s];
     const int local = (int)(it - B.item_base);
-    const int refl = local / B.tiles;
-    const int tile = local - refl * B.tiles;
+    const int pos = local / B.tiles;
+    const int tile = local - pos * B.tiles;
+    const int refl = B.order[pos];
     SampleView S;
     S.V = make_volume(B.vol, B.Z, B.Y, B.X);
     S.coords = B.coords;
@@ -733,10 +738,33 @@ __global__ void batch_finish_kernel(const BatchSample* __restrict__ samples, int
     }
     sum = (double)a - (double)c;
   }
-  B.out[B.order[r]] = B.err[r] ? __builtin_nan("") : sum / (double)B.N;
+  const int refl = B.order[r];
+  B.out[refl] = B.err[refl] ? __builtin_nan("") : sum / (double)B.N;
 }
 
 // ------------------------------------------------------------ host side
+
+// Grow-only page-locked host buffer (hipHostMalloc) for uploads that should
+// run as asynchronous DMA.  Callers synchronise their stream before it grows.
+template <class T>
+struct Pinned {
+  T* p = nullptr;
+  size_t cap = 0;
+  hipError_t reserve(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const hipError_t e = hipHostMalloc((void**)&p, n * sizeof(T), hipHostMallocDefault);
+    if (e == hipSuccess) cap = n;
+    return e;
+  }
+  ~Pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+  T& operator[](size_t i) { return p[i]; }
+  T* data() { return p; }
+};
 
 struct Context {
   int device = 0;
@@ -760,14 +788,15 @@ struct Context {
   double* d_out = nullptr;
   size_t refl_cap = 0;
   unsigned long long* d_steps = nullptr;
-  std::vector<ReflRule> h_rules;
-  std::vector<ReflRule> h_sorted;   // rules in processing order
+  Pinned<ReflRule> h_rules;         // rules in the caller's order (upload staging)
   std::vector<int> h_order;         // processing position -> caller's index
+  Pinned<int> h_order_pin;          // its upload staging
   int* d_order = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_kernel_ms = 0.f;
   unsigned long long last_steps = 0;
   int last_blocks = 0;
+  double last_host_ms = 0.0;
   std::mutex mu_run;
 };
 
@@ -903,9 +932,9 @@ static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flag
   int rc = ensure_run_buffers(c, R, nblk, f64);
   if (rc) return rc;
   order_reflections(c, h_rules_in, R);
-  c->h_sorted.resize((size_t)R);
-  for (int64_t i = 0; i < R; ++i) c->h_sorted[(size_t)i] = h_rules_in[c->h_order[(size_t)i]];
-  const ReflRule* h_rules = c->h_sorted.data();
+  HIP_TRY(c->h_order_pin.reserve((size_t)R));
+  for (int64_t i = 0; i < R; ++i) c->h_order_pin[(size_t)i] = c->h_order[(size_t)i];
+  const ReflRule* h_rules = h_rules_in;
   bool guard = false;
   for (int64_t i = 0; i < R; ++i)
     guard = guard || h_rules[i].ray[0].guard || h_rules[i].ray[1].guard;
@@ -915,7 +944,7 @@ static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flag
 
   HIP_TRY(hipMemcpyAsync(c->d_rules, h_rules, (size_t)R * sizeof(ReflRule),
                          hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order.data(), (size_t)R * sizeof(int),
+  HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order_pin.data(), (size_t)R * sizeof(int),
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemsetAsync(c->d_err, 0, (size_t)R * sizeof(unsigned), c->stream));
   HIP_TRY(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
@@ -964,21 +993,27 @@ static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flag
   HIP_TRY(hipEventElapsedTime(&c->last_kernel_ms, c->ev0, c->ev1));
   c->last_blocks = (int)nblk;
   if (d_paths) HIP_TRY(hipFree(d_paths));
-  std::vector<unsigned> herr;
   if (!paths) {
-    // report the first failing reflection (results are already NaN there)
-    herr.resize((size_t)R);
-    HIP_TRY(hipMemcpy(herr.data(), c->d_err, (size_t)R * sizeof(unsigned),
-                      hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < R; ++i)
-      if (herr[i]) {
-        set_error(ANACOR_ERR_GEOMETRY,
-                  "reflection %lld: %s (the reference aborts here); result set to NaN",
-                  (long long)c->h_order[(size_t)i],
-                  (herr[i] & ANACOR_RAY_ERR_FACE) ? "no exit face (cube_face)"
-                                                  : "exit face inconsistent with beam angles");
-        return ANACOR_ERR_GEOMETRY;
-      }
+    // report the first failing reflection (its result is already NaN); the
+    // error words are read back only when some result is NaN
+    int64_t bad = -1;
+    for (int64_t i = 0; i < R && bad < 0; ++i)
+      if (out[i] != out[i]) bad = i;
+    if (bad >= 0) {
+      std::vector<unsigned> herr((size_t)R);
+      HIP_TRY(hipMemcpy(herr.data(), c->d_err, (size_t)R * sizeof(unsigned),
+                        hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < R; ++i)
+        if (herr[(size_t)i]) {
+          set_error(ANACOR_ERR_GEOMETRY,
+                    "reflection %lld: %s (the reference aborts here); result set to NaN",
+                    (long long)i,
+                    (herr[(size_t)i] & ANACOR_RAY_ERR_FACE)
+                        ? "no exit face (cube_face)"
+                        : "exit face inconsistent with beam angles");
+          return ANACOR_ERR_GEOMETRY;
+        }
+    }
   }
   return ANACOR_OK;
 }
@@ -1128,7 +1163,12 @@ static int host_threads(int requested) {
   if (requested > 0) return std::min(requested, 64);
   const char* e = std::getenv("ANACOR_HOST_THREADS");
   if (e && std::atoi(e) > 0) return std::min(std::atoi(e), 64);
-  return 8;
+  const unsigned hw = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hw, 32u));
+}
+
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 static int max_dim(const Context* c) { return std::max(c->Z, std::max(c->Y, c->X)); }
@@ -1146,9 +1186,11 @@ static int ctx_run(Context* c, const double* s1, const double* omega, int64_t R,
   }
   std::lock_guard<std::mutex> lk(c->mu_run);
   HIP_TRY(hipSetDevice(c->device));
-  c->h_rules.resize((size_t)std::max<int64_t>(R, 1));
+  const auto t0 = std::chrono::steady_clock::now();
+  HIP_TRY(c->h_rules.reserve((size_t)std::max<int64_t>(R, 1)));
   host_setup(s1, omega, R, raw_xray, omega_axis, F, max_dim(c), c->h_rules.data(), nullptr,
              host_threads(n_threads));
+  c->last_host_ms = ms_since(t0);
   return run_rules(c, c->h_rules.data(), R, flags, out, paths);
 }
 
@@ -1211,6 +1253,7 @@ struct Batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
   unsigned long long last_steps = 0;
+  double last_host_ms = 0.0;
   int grid = 0;
   std::vector<BatchSample> h_desc;
   std::mutex mu;
@@ -1282,6 +1325,7 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
                      const double* F, int flags, double* const* out) {
   std::lock_guard<std::mutex> lk(b->mu);
   HIP_TRY(hipSetDevice(b->device));
+  const auto t_host = std::chrono::steady_clock::now();
   const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
   const int S = (int)b->samples.size();
   bool guard = false, wide = false;
@@ -1294,14 +1338,14 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
       set_error(ANACOR_ERR_ARG, "sample %d: bad reflection arguments", i);
       return ANACOR_ERR_ARG;
     }
-    c->h_rules.resize((size_t)std::max<int64_t>(Ri, 1));
+    HIP_TRY(c->h_rules.reserve((size_t)std::max<int64_t>(Ri, 1)));
     host_setup(s1[i], omega[i], Ri, raw_xray, omega_axis, F, max_dim(c), c->h_rules.data(),
                nullptr, host_threads(0));
     order_reflections(c, c->h_rules.data(), Ri);
-    c->h_sorted.resize((size_t)std::max<int64_t>(Ri, 1));
+    HIP_TRY(c->h_order_pin.reserve((size_t)std::max<int64_t>(Ri, 1)));
     for (int64_t r = 0; r < Ri; ++r) {
-      c->h_sorted[(size_t)r] = c->h_rules[(size_t)c->h_order[(size_t)r]];
-      guard = guard || c->h_sorted[(size_t)r].ray[0].guard || c->h_sorted[(size_t)r].ray[1].guard;
+      c->h_order_pin[(size_t)r] = c->h_order[(size_t)r];
+      guard = guard || c->h_rules[(size_t)r].ray[0].guard || c->h_rules[(size_t)r].ray[1].guard;
     }
     wide = wide || (int64_t)c->Y * c->X >= (1 << 23) || c->Z >= (1 << 23);
     const int vpt = c->N > 0 ? std::min(c->tile_voxels > 0 ? c->tile_voxels : kBlock * 64,
@@ -1312,9 +1356,9 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
                                       std::max<int64_t>(Ri * tiles, 1), f64);
     if (rc) return rc;
     if (Ri > 0) {
-      HIP_TRY(hipMemcpyAsync(c->d_rules, c->h_sorted.data(), (size_t)Ri * sizeof(ReflRule),
+      HIP_TRY(hipMemcpyAsync(c->d_rules, c->h_rules.data(), (size_t)Ri * sizeof(ReflRule),
                              hipMemcpyHostToDevice, b->stream));
-      HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order.data(), (size_t)Ri * sizeof(int),
+      HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order_pin.data(), (size_t)Ri * sizeof(int),
                              hipMemcpyHostToDevice, b->stream));
     }
     BatchSample& d = b->h_desc[(size_t)i];
@@ -1381,6 +1425,7 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
     HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->graph_key = key;
   }
+  b->last_host_ms = ms_since(t_host);
   HIP_TRY(hipEventRecord(b->ev0, b->stream));
   HIP_TRY(hipGraphLaunch(b->exec, b->stream));
   HIP_TRY(hipEventRecord(b->ev1, b->stream));
@@ -1494,6 +1539,7 @@ int anacor_ctx_stats(const anacor_ctx* ctx, anacor_stats_t* st) {
   st->voxel_steps = ctx->c->last_steps;
   st->workgroups = ctx->c->last_blocks;
   st->n_voxels = ctx->c->N;
+  st->host_ms = ctx->c->last_host_ms;
   return ANACOR_OK;
 }
 
@@ -1535,6 +1581,7 @@ int anacor_batch_stats(const anacor_batch* batch, anacor_stats_t* st) {
   int64_t n = 0;
   for (Context* c : batch->b->samples) n += c->N;
   st->n_voxels = n;
+  st->host_ms = batch->b->last_host_ms;
   return ANACOR_OK;
 }
 

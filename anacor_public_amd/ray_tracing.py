@@ -233,7 +233,8 @@ class Tracer:
         st = _lib.Stats()
         self._lib.anacor_ctx_stats(self.handle, ct.byref(st))
         return dict(kernel_ms=st.kernel_ms, voxel_steps=int(st.voxel_steps),
-                    workgroups=int(st.workgroups), n_voxels=int(st.n_voxels))
+                    workgroups=int(st.workgroups), n_voxels=int(st.n_voxels),
+                    host_ms=st.host_ms)
 
     @classmethod
     def _borrow(cls, handle: int, n_voxels: int, shape) -> "Tracer":
@@ -330,7 +331,8 @@ class Batch:
         st = _lib.Stats()
         self._lib.anacor_batch_stats(self._h, ct.byref(st))
         return dict(kernel_ms=st.kernel_ms, voxel_steps=int(st.voxel_steps),
-                    workgroups=int(st.workgroups), n_voxels=int(st.n_voxels))
+                    workgroups=int(st.workgroups), n_voxels=int(st.n_voxels),
+                    host_ms=st.host_ms)
 
     def close(self):
         if getattr(self, "_h", None):

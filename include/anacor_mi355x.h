@@ -142,6 +142,7 @@ typedef struct {
     uint64_t voxel_steps;    /* sum of len_path_2 over all rays */
     int64_t  workgroups;     /* workgroups of the trace launch */
     int64_t  n_voxels;
+    double   host_ms;        /* host setup of the reflections (rotation, angles, face rules, order) */
 } anacor_stats_t;
 int anacor_ctx_stats(const anacor_ctx *ctx, anacor_stats_t *stats);
 
