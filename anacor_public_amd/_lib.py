@@ -32,7 +32,17 @@ EXPORTED = (
     "anacor_ctx_destroy", "anacor_ctx_set_tile", "anacor_ctx_set_voxel_block", "anacor_ctx_run", "anacor_ctx_paths",
     "anacor_ctx_stats", "anacor_reflection_geometry", "anacor_batch_create",
     "anacor_batch_destroy", "anacor_batch_sample", "anacor_batch_run", "anacor_batch_stats",
+    "anacor_mlp_create", "anacor_mlp_destroy", "anacor_mlp_num_params", "anacor_mlp_get_params",
+    "anacor_mlp_set_params", "anacor_mlp_set_scale", "anacor_mlp_set_features",
+    "anacor_mlp_set_envelope", "anacor_mlp_features", "anacor_mlp_forward",
+    "anacor_mlp_gradients", "anacor_mlp_train", "anacor_mlp_predict", "anacor_mlp_stats",
 )
+
+MLP_INPUTS = 12
+MLP_WIDTH = 256
+MLP_PARAMS = 200961
+MLP_FEATURES_HARMONIC = 0
+MLP_FEATURES_CHAMBER = 1
 
 PATH_DTYPE = np.dtype([("face", np.int32, 2), ("len", np.int32, 2), ("nseg", np.int32, 2),
                        ("err", np.int32), ("pad", np.int32), ("last", np.int32, (2, 3)),
@@ -49,6 +59,17 @@ class Sample(ct.Structure):
     _fields_ = [("volume", ct.c_void_p), ("shape", ct.c_int64 * 3), ("coords", ct.c_void_p),
                 ("n_coords", ct.c_int64), ("voxel_size", ct.c_double * 3),
                 ("coefficients", ct.c_double * 4)]
+
+
+class Envelope(ct.Structure):
+    _fields_ = [("axis", ct.c_double * 3), ("e1", ct.c_double * 3), ("e2", ct.c_double * 3),
+                ("axial", ct.c_double * 2), ("two_theta", ct.c_double * 2),
+                ("cells", ct.c_int32 * 3), ("pad", ct.c_int32)]
+
+
+class MlpStats(ct.Structure):
+    _fields_ = [("mlp_ms", ct.c_double), ("total_ms", ct.c_double), ("rows", ct.c_int64),
+                ("in_envelope", ct.c_int64)]
 
 
 class AnacorError(RuntimeError):
@@ -93,6 +114,35 @@ def _prototypes(lib):
     lib.anacor_batch_stats.argtypes = [vp, ct.POINTER(Stats)]
     lib.anacor_reflection_geometry.restype = ct.c_int
     lib.anacor_reflection_geometry.argtypes = [vp, vp, i64, vp, vp, vp, i64, vp, vp]
+    u64, f32, sz = ct.c_uint64, ct.c_float, ct.c_size_t
+    lib.anacor_mlp_create.restype = vp
+    lib.anacor_mlp_create.argtypes = [u64, i32]
+    lib.anacor_mlp_destroy.restype = None
+    lib.anacor_mlp_destroy.argtypes = [vp]
+    lib.anacor_mlp_num_params.restype = sz
+    lib.anacor_mlp_num_params.argtypes = []
+    lib.anacor_mlp_get_params.restype = ct.c_int
+    lib.anacor_mlp_get_params.argtypes = [vp, vp, sz]
+    lib.anacor_mlp_set_params.restype = ct.c_int
+    lib.anacor_mlp_set_params.argtypes = [vp, vp, sz]
+    lib.anacor_mlp_set_scale.restype = ct.c_int
+    lib.anacor_mlp_set_scale.argtypes = [vp, dbl, dbl]
+    lib.anacor_mlp_set_features.restype = ct.c_int
+    lib.anacor_mlp_set_features.argtypes = [vp, i32]
+    lib.anacor_mlp_set_envelope.restype = ct.c_int
+    lib.anacor_mlp_set_envelope.argtypes = [vp, ct.POINTER(Envelope), vp]
+    lib.anacor_mlp_features.restype = ct.c_int
+    lib.anacor_mlp_features.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, vp]
+    lib.anacor_mlp_forward.restype = ct.c_int
+    lib.anacor_mlp_forward.argtypes = [vp, vp, i64, vp]
+    lib.anacor_mlp_gradients.restype = ct.c_int
+    lib.anacor_mlp_gradients.argtypes = [vp, vp, vp, i64, vp, vp]
+    lib.anacor_mlp_train.restype = ct.c_int
+    lib.anacor_mlp_train.argtypes = [vp, vp, vp, i64, i32, i32, f32, f32, u64, vp]
+    lib.anacor_mlp_predict.restype = ct.c_int
+    lib.anacor_mlp_predict.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, vp]
+    lib.anacor_mlp_stats.restype = ct.c_int
+    lib.anacor_mlp_stats.argtypes = [vp, ct.POINTER(MlpStats)]
     # drop-in entry points, with the argtypes the reference's binding uses
     # (utils/utils_mp.py:59-79, :82-94, :128-148)
     lib.ray_tracing_overall.restype = ct.POINTER(dbl)

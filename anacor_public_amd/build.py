@@ -4,7 +4,7 @@
 
 Host geometry (csrc/geometry.cpp) is compiled with g++ against glibc libm and
 without FMA contraction, so its FP64 results carry the reference's bits; the
-HIP kernels and the C-ABI (csrc/trace.hip) are compiled with hipcc for gfx950,
+HIP kernels and the C-ABI (csrc/trace.hip, csrc/mlp.hip) are compiled with hipcc for gfx950,
 also with -ffp-contract=off (bit-exact stepping) and packed-FP32 code
 generation disabled.
 """
@@ -36,6 +36,9 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contr
              "-Wall", "-Wno-unused-result"]
 
 
+HIP_SOURCES = ("trace.hip", "mlp.hip")   # tracer + C-ABI; learned surrogate
+
+
 def _run(cmd):
     print("+", " ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
@@ -45,13 +48,15 @@ def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     geo_o = os.path.join(BUILD, "geometry.o")
-    trace_o = os.path.join(BUILD, "trace.o")
     _run([CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, "geometry.cpp"), "-o", geo_o])
-    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c",
-          os.path.join(CSRC, "trace.hip"), "-o", trace_o])
+    objs = [geo_o]
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c",
+              os.path.join(CSRC, src), "-o", obj])
+        objs.insert(0, obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, trace_o, geo_o,
-          "-lpthread"])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)
     for alias in ALIASES:
         dst = os.path.join(LIBDIR, alias)

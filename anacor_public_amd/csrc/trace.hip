@@ -35,6 +35,7 @@
 #include <mutex>
 #include <vector>
 
+#include "errors.h"
 #include "geometry.h"
 #include "../../include/anacor_mi355x.h"
 
@@ -62,7 +63,7 @@ namespace anacor {
 static thread_local char g_err[512];
 static std::atomic<int> g_err_code{0};
 
-static void set_error(int code, const char* fmt, ...) {
+void set_error(int code, const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof(g_err), fmt, ap);
@@ -71,7 +72,7 @@ static void set_error(int code, const char* fmt, ...) {
   if (std::getenv("ANACOR_VERBOSE")) fprintf(stderr, "anacor: %s\n", g_err);
 }
 
-static void clear_error() {
+void clear_error() {
   g_err[0] = 0;
   g_err_code.store(0);
 }

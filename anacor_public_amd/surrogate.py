@@ -1,0 +1,327 @@
+"""Learned surrogate of the absorption factor for one sample.
+
+An MLP (12 -> 256 x 4 -> 1, SiLU; BF16-MFMA kernels with FP32 accumulation,
+csrc/mlp.hip) maps the incident and diffracted directions of a reflection in
+the crystal frame to log A_h.  It is trained on the GPU on absorption factors
+the ray tracer computes for the same sample, answers only inside the
+envelope it was trained on (the training 2-theta range and the training
+goniometer's rotation circle), and leaves every other reflection to the
+tracer, which also judges its error.
+
+    with Tracer(volume, coords, voxel_size, coefficients) as t:
+        sur = Surrogate(t, raw_xray, omega_axis, F)
+        report = sur.fit(s1_train, omega_train)          # traces, then trains
+        A, used = sur.predict(s1, omega)                 # float64[R], bool[R]
+
+The reference has no surrogate; its nearest analogue is the gridding
+approximation (src/gridding_interpolation.c:98-161, utils/utils_gridding.py).
+There is no CPU path: the network and the fallback both run on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import time
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+
+_vp = ct.c_void_p
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_vp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class MLP:
+    """The GPU network (anacor_mlp_* of include/anacor_mi355x.h)."""
+
+    n_params = _lib.MLP_PARAMS
+    n_inputs = _lib.MLP_INPUTS
+
+    def __init__(self, seed: int = 0, device: int = 0):
+        self._lib = _lib.load()
+        h = self._lib.anacor_mlp_create(int(seed), int(device))
+        if not h:
+            code, msg = _lib.last_error(self._lib)
+            raise _lib.AnacorError(code, msg)
+        self._h = ct.c_void_p(h)
+
+    @property
+    def handle(self):
+        if not getattr(self, "_h", None):
+            raise ValueError("MLP is closed")
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.anacor_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        _lib.check(rc, self._lib)
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        self._check(self._lib.anacor_mlp_get_params(self.handle, _p(out), out.size))
+        return out
+
+    def set_params(self, params) -> None:
+        p = _f32(params).reshape(-1)
+        if p.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters, got {p.size}")
+        self._check(self._lib.anacor_mlp_set_params(self.handle, _p(p), p.size))
+
+    def set_scale(self, mean: float, std: float) -> None:
+        self._check(self._lib.anacor_mlp_set_scale(self.handle, float(mean), float(std)))
+
+    def set_features(self, feature_set: int) -> None:
+        self._check(self._lib.anacor_mlp_set_features(self.handle, int(feature_set)))
+
+    def set_envelope(self, envelope, trust=None) -> None:
+        """envelope: _lib.Envelope or None (removes it); trust: uint8 per cell or None."""
+        if envelope is None:
+            self._check(self._lib.anacor_mlp_set_envelope(self.handle, None, None))
+            return
+        n = int(np.prod(envelope.cells[:]))
+        t = None
+        if trust is not None:
+            t = np.ascontiguousarray(trust, dtype=np.uint8).reshape(-1)
+            if t.size != n:
+                raise ValueError(f"trust table has {t.size} cells, envelope {n}")
+        self._check(self._lib.anacor_mlp_set_envelope(self.handle, ct.byref(envelope),
+                                                      _p(t) if t is not None else None))
+        self._trust_keep = t
+
+    def features(self, s1, omega, raw_xray, omega_axis, F, cells: bool = False,
+                 flags: bool = False):
+        """X float32[R][12], invariants float64[R][4] (axial, two_theta, alpha,
+        beta) and, with an envelope set, cell indices and in-envelope flags."""
+        s1 = _f64(s1).reshape(-1, 3)
+        om = _f64(omega).reshape(-1)
+        R = len(om)
+        if len(s1) != R:
+            raise ValueError("s1 and omega lengths differ")
+        X = np.empty((R, 12), dtype=np.float32)
+        inv = np.empty((R, 4), dtype=np.float64)
+        cl = np.empty(R, dtype=np.int32) if cells else None
+        fl = np.empty(R, dtype=np.uint8) if flags else None
+        self._check(self._lib.anacor_mlp_features(
+            self.handle, _p(s1), _p(om), R, _p(_f64(raw_xray)), _p(_f64(omega_axis)),
+            _p(_f64(F)), _p(X), _p(inv), _p(cl) if cl is not None else None,
+            _p(fl) if fl is not None else None))
+        return X, inv, cl, (fl.astype(bool) if fl is not None else None)
+
+    def forward(self, X) -> np.ndarray:
+        X = _f32(X).reshape(-1, 12)
+        y = np.empty(len(X), dtype=np.float32)
+        self._check(self._lib.anacor_mlp_forward(self.handle, _p(X), len(X), _p(y)))
+        return y
+
+    def gradients(self, X, t):
+        """(gradient float32[n_params], loss) of the mean squared error."""
+        X = _f32(X).reshape(-1, 12)
+        t = _f32(t).reshape(-1)
+        g = np.empty(self.n_params, dtype=np.float32)
+        loss = ct.c_float(0.0)
+        self._check(self._lib.anacor_mlp_gradients(self.handle, _p(X), _p(t), len(X), _p(g),
+                                                   ct.byref(loss)))
+        return g, float(loss.value)
+
+    def train(self, X, t, epochs: int, batch: int = 1024, lr: float = 1e-3,
+              lr_min: float = 1e-5, seed: int = 0) -> np.ndarray:
+        X = _f32(X).reshape(-1, 12)
+        t = _f32(t).reshape(-1)
+        losses = np.zeros(int(epochs), dtype=np.float32)
+        self._check(self._lib.anacor_mlp_train(self.handle, _p(X), _p(t), len(X), int(epochs),
+                                               int(batch), float(lr), float(lr_min), int(seed),
+                                               _p(losses)))
+        return losses
+
+    def predict(self, s1, omega, raw_xray, omega_axis, F):
+        s1 = _f64(s1).reshape(-1, 3)
+        om = _f64(omega).reshape(-1)
+        R = len(om)
+        A = np.empty(R, dtype=np.float64)
+        flag = np.empty(R, dtype=np.uint8)
+        self._check(self._lib.anacor_mlp_predict(
+            self.handle, _p(s1), _p(om), R, _p(_f64(raw_xray)), _p(_f64(omega_axis)),
+            _p(_f64(F)), _p(A), _p(flag)))
+        return A, flag.astype(bool)
+
+    def stats(self) -> dict:
+        st = _lib.MlpStats()
+        self._check(self._lib.anacor_mlp_stats(self.handle, ct.byref(st)))
+        return {"mlp_ms": st.mlp_ms, "total_ms": st.total_ms, "rows": st.rows,
+                "in_envelope": st.in_envelope}
+
+
+def crystal_axis(omega_axis, F) -> np.ndarray:
+    """The goniometer axis in the crystal frame, unit F^T omega_axis (the
+    rotation about omega_axis leaves it fixed)."""
+    u = np.asarray(F, dtype=np.float64).reshape(3, 3).T @ np.asarray(omega_axis, dtype=np.float64)
+    return u / np.linalg.norm(u)
+
+
+def make_envelope(axis, axial_range, two_theta_range, cells=(64, 8, 32)) -> "_lib.Envelope":
+    """Envelope over a training set's axial and two-theta ranges with a trust
+    grid of `cells` (alpha, two_theta, beta)."""
+    u = np.asarray(axis, dtype=np.float64)
+    u = u / np.linalg.norm(u)
+    r = np.zeros(3)
+    r[int(np.argmin(np.abs(u)))] = 1.0
+    e1 = np.cross(u, r)
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(u, e1)
+    env = _lib.Envelope()
+    for i in range(3):
+        env.axis[i], env.e1[i], env.e2[i] = float(u[i]), float(e1[i]), float(e2[i])
+        env.cells[i] = int(cells[i])
+    env.axial[0], env.axial[1] = float(axial_range[0]), float(axial_range[1])
+    env.two_theta[0], env.two_theta[1] = float(two_theta_range[0]), float(two_theta_range[1])
+    return env
+
+
+class Surrogate:
+    """MLP surrogate of one sample's A_h with the tracer as fallback and judge."""
+
+    def __init__(self, tracer, raw_xray, omega_axis, F, seed: int = 0, device: int = 0):
+        self.tracer = tracer
+        self.raw_xray = _f64(raw_xray).reshape(3)
+        self.omega_axis = _f64(omega_axis).reshape(3)
+        self.F = _f64(F).reshape(9)
+        self.mlp = MLP(seed=seed, device=device)
+        self.feature_set = _lib.MLP_FEATURES_HARMONIC
+        self.mlp.set_features(self.feature_set)
+        self.envelope: Optional[_lib.Envelope] = None
+        self.trust: Optional[np.ndarray] = None
+        self.y_mean = 0.0
+        self.y_std = 1.0
+
+    def close(self):
+        self.mlp.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_feature_set(self, feature_set: int) -> None:
+        self.mlp.set_features(feature_set)
+        self.feature_set = int(feature_set)
+
+    def _geom(self):
+        return self.raw_xray, self.omega_axis, self.F
+
+    def trace(self, s1, omega) -> np.ndarray:
+        return self.tracer.run(s1, omega, *self._geom())
+
+    def fit(self, s1, omega, epochs: int = 200, batch: int = 1024, lr: float = 1e-3,
+            lr_min: float = 1e-6, seed: int = 0, targets: Optional[np.ndarray] = None,
+            cells=(64, 8, 32), tau: float = 5e-4, min_count: int = 4,
+            axial_tol: float = 1e-9) -> dict:
+        """Trace the training reflections (unless targets are given), train
+        with Adam, and set the envelope: the training axial and two-theta
+        ranges, and trust only in the (alpha, two_theta, beta) cells holding at
+        least min_count training reflections whose relative error is at
+        most tau."""
+        t0 = time.perf_counter()
+        A = self.trace(s1, omega) if targets is None else _f64(targets)
+        t_trace = time.perf_counter() - t0
+        ok = np.isfinite(A) & (A > 0)
+        if not np.all(ok):
+            raise ValueError(f"{int((~ok).sum())} training reflections have no finite A_h > 0")
+        y = np.log(A)
+        self.y_mean = float(y.mean())
+        self.y_std = float(max(y.std(), 1e-12))
+        self.mlp.set_scale(self.y_mean, self.y_std)
+        self.mlp.set_envelope(None)
+        _, inv, _, _ = self.mlp.features(s1, omega, *self._geom())
+        env = make_envelope(crystal_axis(self.omega_axis, self.F),
+                            (inv[:, 0].min() - axial_tol, inv[:, 0].max() + axial_tol),
+                            (inv[:, 1].min(), inv[:, 1].max()), cells)
+        self.mlp.set_envelope(env)
+        X, inv, cell, _ = self.mlp.features(s1, omega, *self._geom(), cells=True)
+        t1 = time.perf_counter()
+        losses = self.mlp.train(X, (y - self.y_mean) / self.y_std, epochs=epochs, batch=batch,
+                                lr=lr, lr_min=lr_min, seed=seed)
+        t_train = time.perf_counter() - t1
+        pred = np.exp(self.mlp.forward(X).astype(np.float64) * self.y_std + self.y_mean)
+        res = np.abs(pred - A) / A
+        n_cells = int(np.prod(cells))
+        count = np.bincount(cell, minlength=n_cells)
+        worst = np.zeros(n_cells)
+        np.maximum.at(worst, cell, res)
+        trust = ((count >= min_count) & (worst <= tau)).astype(np.uint8)
+        self.mlp.set_envelope(env, trust)
+        self.envelope, self.trust = env, trust
+        served = trust[cell].astype(bool)
+        return {"n_train": int(len(y)), "trace_s": t_trace, "train_s": t_train,
+                "epochs": int(epochs), "batch": int(batch), "lr": lr, "lr_min": lr_min,
+                "final_loss": float(losses[-1]), "losses": losses,
+                "y_mean": self.y_mean, "y_std": self.y_std,
+                "two_theta_deg": (float(np.degrees(env.two_theta[0])),
+                                  float(np.degrees(env.two_theta[1]))),
+                "train_rel_max": float(res.max()), "train_rel_mean": float(res.mean()),
+                "cells": list(cells), "tau": tau, "min_count": min_count,
+                "trusted_cells": float(trust.mean()), "train_served": float(served.mean())}
+
+    def predict(self, s1, omega, fallback: bool = True):
+        """(A float64[R], used bool[R]): the network's A_h where it may answer
+        (used), the tracer's elsewhere (NaN there if fallback is False)."""
+        if self.envelope is None:
+            raise ValueError("the surrogate has not been fitted")
+        A, used = self.mlp.predict(s1, omega, *self._geom())
+        if fallback and not np.all(used):
+            out = np.flatnonzero(~used)
+            s1 = _f64(s1).reshape(-1, 3)
+            A[out] = self.trace(s1[out], _f64(omega).reshape(-1)[out])
+        return A, used
+
+    def state(self) -> dict:
+        """Everything needed to rebuild the fitted surrogate (numpy arrays)."""
+        e = self.envelope
+        return {"params": self.mlp.get_params(), "y_mean": self.y_mean, "y_std": self.y_std,
+                "axis": np.array(e.axis[:]), "e1": np.array(e.e1[:]), "e2": np.array(e.e2[:]),
+                "axial": np.array(e.axial[:]), "two_theta": np.array(e.two_theta[:]),
+                "cells": np.array(e.cells[:]), "trust": self.trust,
+                "feature_set": self.feature_set}
+
+    def load_state(self, st: dict) -> None:
+        self.set_feature_set(int(st["feature_set"]))
+        self.mlp.set_params(st["params"])
+        self.y_mean, self.y_std = float(st["y_mean"]), float(st["y_std"])
+        self.mlp.set_scale(self.y_mean, self.y_std)
+        env = _lib.Envelope()
+        for i in range(3):
+            env.axis[i], env.e1[i], env.e2[i] = (float(st["axis"][i]), float(st["e1"][i]),
+                                                 float(st["e2"][i]))
+            env.cells[i] = int(st["cells"][i])
+        for i in range(2):
+            env.axial[i] = float(st["axial"][i])
+            env.two_theta[i] = float(st["two_theta"][i])
+        self.envelope = env
+        self.trust = np.ascontiguousarray(st["trust"], dtype=np.uint8)
+        self.mlp.set_envelope(env, self.trust)

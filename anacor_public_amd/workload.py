@@ -258,3 +258,47 @@ def batched_config(seed: int = 3, n_samples: int = 8, R_scale: float = 1.0):
         wls.append(Workload(f"batched-{sp['shape']}", vol,
                             sample_stride(crystal_voxels(vol), sp["stride"]), s1, om))
     return wls
+
+
+@dataclasses.dataclass
+class SurrogateWorkload:
+    """One sample of the batched set with its training, held-out and
+    inference reflection sets (disjoint seeds)."""
+    sample: Workload               # volume, voxels; s1/omega = the training set
+    heldout_s1: np.ndarray
+    heldout_omega: np.ndarray
+    infer_s1: np.ndarray
+    infer_omega: np.ndarray
+    infer_outside: np.ndarray      # bool: drawn outside the training 2-theta range
+
+
+SURROGATE_SAMPLE = 2   # of batched_specs(): 100^3, crystal 23.2/19.0/10.9, all 20,089 voxels
+
+
+def surrogate_config(seed: int = 5, n_train: int = 200_000, n_heldout: int = 10_000,
+                     n_infer: int = 1_000_000, outside_fraction: float = 0.01,
+                     sample: int = SURROGATE_SAMPLE) -> SurrogateWorkload:
+    """Batched sample `sample` (100^3, every crystal voxel traced) with
+    n_train training reflections (2theta 5-60 deg), n_heldout held-out ones
+    from the same distribution and n_infer inference reflections of which
+    outside_fraction lie at 2theta 60-75 deg, outside the training range."""
+    sp = batched_specs()[sample]
+    vol = make_volume(VolumeSpec(sp["shape"], sp["cyl_radius"], sp["slab"], sp["semi_axes"],
+                                 sp["seed"]))
+    coords = sample_stride(crystal_voxels(vol), sp["stride"])
+    s1, om = make_reflections(n_train, seed * 1000 + 1)
+    hs1, hom = make_reflections(n_heldout, seed * 1000 + 2)
+    n_out = int(round(n_infer * outside_fraction))
+    is1, iom = make_reflections(n_infer - n_out, seed * 1000 + 3)
+    os1, oom = make_reflections(n_out, seed * 1000 + 4, two_theta_deg=(60.0, 75.0))
+    outside = np.zeros(n_infer, dtype=bool)        # n_out seeded positions
+    if n_out:
+        outside[np.argpartition(splitmix64(seed, 505, n_infer), n_out - 1)[:n_out]] = True
+    all_s1 = np.empty((n_infer, 3))
+    all_om = np.empty(n_infer)
+    all_s1[outside], all_om[outside] = os1, oom
+    all_s1[~outside], all_om[~outside] = is1, iom
+    wl = Workload(f"surrogate-batched{sample}-{'x'.join(str(s) for s in sp['shape'])}", vol,
+                  coords, s1, om)
+    return SurrogateWorkload(wl, hs1, hom, np.ascontiguousarray(all_s1),
+                             np.ascontiguousarray(all_om), outside)

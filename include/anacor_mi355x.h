@@ -191,6 +191,95 @@ int anacor_reflection_geometry(const double *s1, const double *omega, int64_t R,
                                const double F[9], int64_t max_dim, double *theta_phi,
                                double *d_table);
 
+/* -------------------------------------------------------- learned surrogate */
+
+/* An MLP 12 -> 256 -> 256 -> 256 -> 256 -> 1 (SiLU) from the incident and
+ * diffracted unit vectors in the crystal frame (plus their second harmonics)
+ * to log A_h for one sample, trained on the GPU on absorption factors the
+ * tracer computes.  No counterpart in the reference; its nearest analogue is
+ * the gridding approximation (src/gridding_interpolation.c:98-161).  The
+ * 256x256 layers use BF16 weights on MFMA with FP32 accumulation. */
+#define ANACOR_MLP_INPUTS 12
+#define ANACOR_MLP_WIDTH  256
+#define ANACOR_MLP_PARAMS 200961   /* FP32 master parameters, layout in mlp.hip */
+
+/* Network input sets (anacor_mlp_set_features): six more inputs after the two
+ * unit vectors x (incident) and s (diffracted), crystal frame. */
+#define ANACOR_MLP_FEATURES_HARMONIC 0  /* per vector: 2 v0 v1, v0^2 - v1^2, 2 v2^2 - 1 */
+#define ANACOR_MLP_FEATURES_CHAMBER  1  /* +-1 sides of the planes where the tracer's
+                                           discrete rules switch (sign of theta, diagonals) */
+
+/* Where the network may answer.  Invariants of a reflection: axial = x . axis,
+ * two_theta = angle(x, s), alpha = atan2(x . e2, x . e1) (azimuth of the
+ * incident direction about the axis), beta = azimuth of s about x (basis
+ * axis x x, x (axis x x)).  A reflection is inside when axial and two_theta
+ * are in range and its (alpha, two_theta, beta) cell is trusted. */
+typedef struct {
+    double  axis[3];
+    double  e1[3], e2[3];
+    double  axial[2];
+    double  two_theta[2];     /* radians */
+    int32_t cells[3];         /* grid over alpha [-pi, pi], two_theta range, beta [-pi, pi] */
+    int32_t pad;
+} anacor_envelope_t;
+
+typedef struct anacor_mlp anacor_mlp;
+
+/* Seeded uniform fan-in initialisation on `device`; NULL on failure. */
+anacor_mlp *anacor_mlp_create(uint64_t seed, int device);
+void anacor_mlp_destroy(anacor_mlp *mlp);
+size_t anacor_mlp_num_params(void);
+int anacor_mlp_get_params(anacor_mlp *mlp, float *out, size_t n);
+/* Also resets the Adam state. */
+int anacor_mlp_set_params(anacor_mlp *mlp, const float *in, size_t n);
+/* log A_h = y_mean + y_std * network output. */
+int anacor_mlp_set_scale(anacor_mlp *mlp, double y_mean, double y_std);
+
+int anacor_mlp_set_features(anacor_mlp *mlp, int feature_set);
+
+/* Envelope and trust table (uint8[cells[0] * cells[1] * cells[2]], cell
+ * index (ia * cells[1] + it) * cells[2] + ib; NULL = every cell trusted).
+ * env NULL removes the envelope. */
+int anacor_mlp_set_envelope(anacor_mlp *mlp, const anacor_envelope_t *env, const uint8_t *trust);
+
+/* Network inputs of R reflections (features[R][12]) and invariants[R][4] =
+ * (axial, two_theta, alpha, beta); with an envelope set also the cell index
+ * and the in-envelope flag.  Every output is nullable; without an envelope
+ * the axis is F^T omega_axis. */
+int anacor_mlp_features(anacor_mlp *mlp, const double *s1, const double *omega, int64_t R,
+                        const double raw_xray[3], const double omega_axis[3],
+                        const double F[9], float *features, double *invariants,
+                        int32_t *cells, uint8_t *in_envelope);
+
+/* Network outputs (normalised) for n rows of features. */
+int anacor_mlp_forward(anacor_mlp *mlp, const float *x, int64_t n, float *y);
+
+/* Gradient of the mean squared error of n <= 65536 rows against targets t
+ * (normalised) into grad[ANACOR_MLP_PARAMS]; no update. */
+int anacor_mlp_gradients(anacor_mlp *mlp, const float *x, const float *t, int64_t n,
+                         float *grad, float *loss);
+
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over `epochs` shuffled passes of
+ * mini-batches, learning rate cosine-decayed from lr to lr_min;
+ * epoch_loss[epochs] (nullable) receives the mean batch loss per epoch. */
+int anacor_mlp_train(anacor_mlp *mlp, const float *x, const float *t, int64_t n, int epochs,
+                     int batch, float lr, float lr_min, uint64_t seed, float *epoch_loss);
+
+/* A_h of R reflections: features, envelope test and forward pass on the GPU.
+ * Reflections outside the envelope get NaN and in_envelope 0; the caller
+ * traces them.  Needs an envelope (anacor_mlp_set_envelope). */
+int anacor_mlp_predict(anacor_mlp *mlp, const double *s1, const double *omega, int64_t R,
+                       const double raw_xray[3], const double omega_axis[3], const double F[9],
+                       double *A, uint8_t *in_envelope);
+
+typedef struct {
+    double  mlp_ms;       /* forward pass (HIP events on the model's stream) */
+    double  total_ms;     /* the whole anacor_mlp_predict call */
+    int64_t rows;
+    int64_t in_envelope;
+} anacor_mlp_stats_t;
+int anacor_mlp_stats(const anacor_mlp *mlp, anacor_mlp_stats_t *stats);
+
 #ifdef __cplusplus
 }
 #endif

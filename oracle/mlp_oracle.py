@@ -1,0 +1,208 @@
+"""FP64 CPU restatement of the surrogate network (TEST INFRASTRUCTURE ONLY).
+
+The learned surrogate has no counterpart in the reference (its nearest
+analogue is the gridding approximation, src/gridding_interpolation.c:98-161);
+this module is the checker for the GPU kernels of anacor_public_amd/csrc/mlp.hip:
+
+* ``features`` -- the network inputs of a reflection: the crystal-frame
+  incident and diffracted unit vectors, rotated as the tracer's setup does
+  (Rodrigues matrix of utils/utils_os.py:26-43, ``(R(omega) F)^T v`` as
+  ray_tracing.c:397-405 applies it), plus their second harmonics;
+* ``forward`` / ``backward`` -- MLP 12 -> 256 x 4 -> 1 with SiLU and the mean
+  squared error, in FP64; with ``bf16=True`` the three 256x256 weight matrices
+  are first rounded to BF16 (round to nearest even) exactly as the GPU uses
+  them, so the GPU's remaining differences are FP32 accumulation only.
+
+Pinned by tests/test_oracle_mlp.py: central finite differences of the loss
+(FP64, ``bf16=False``) for every parameter block, BF16 rounding against
+torch's conversion and hand-worked tie cases, SiLU closed forms, and the
+feature identities (unit norms, harmonic identities, rotation invariance of
+x . s).  Never imported by the product path.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+N_IN = 12
+WIDTH = 256
+# parameter layout: W1[256][12] b1 W2[256][256] b2 W3 b3 W4 b4 w5[256] b5[1]
+_SHAPES = (("W1", (WIDTH, N_IN)), ("b1", (WIDTH,)), ("W2", (WIDTH, WIDTH)), ("b2", (WIDTH,)),
+           ("W3", (WIDTH, WIDTH)), ("b3", (WIDTH,)), ("W4", (WIDTH, WIDTH)), ("b4", (WIDTH,)),
+           ("w5", (WIDTH,)), ("b5", (1,)))
+N_PARAMS = sum(int(np.prod(s)) for _, s in _SHAPES)
+assert N_PARAMS == 200961
+
+
+def offsets():
+    out, o = {}, 0
+    for name, shape in _SHAPES:
+        n = int(np.prod(shape))
+        out[name] = (o, o + n, shape)
+        o += n
+    return out
+
+
+def unpack(flat: np.ndarray) -> dict:
+    flat = np.asarray(flat, dtype=np.float64).reshape(-1)
+    assert flat.size == N_PARAMS
+    return {k: flat[a:b].reshape(s) for k, (a, b, s) in offsets().items()}
+
+
+def pack(parts: dict) -> np.ndarray:
+    out = np.empty(N_PARAMS, dtype=np.float64)
+    for k, (a, b, s) in offsets().items():
+        out[a:b] = np.asarray(parts[k], dtype=np.float64).reshape(-1)
+    return out
+
+
+def bf16_round(x) -> np.ndarray:
+    """float32 -> nearest BF16 (ties to even), returned as float64 values."""
+    f = np.asarray(x, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    lsb = (u >> np.uint64(16)) & np.uint64(1)
+    r = ((u + np.uint64(0x7FFF) + lsb) >> np.uint64(16)) << np.uint64(16)
+    out = r.astype(np.uint32).view(np.float32).astype(np.float64)
+    return np.where(np.isfinite(f), out, f.astype(np.float64))
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def silu_grad(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
+def forward(params, X, bf16: bool = True):
+    """y[n] and the cache for ``backward``.  X: [n][12]."""
+    p = unpack(params)
+    W = [p["W2"], p["W3"], p["W4"]]
+    if bf16:
+        W = [bf16_round(w) for w in W]
+    X = np.asarray(X, dtype=np.float64)
+    zs, hs = [], [X]
+    z = X @ p["W1"].T + p["b1"]
+    zs.append(z)
+    hs.append(silu(z))
+    for w, b in zip(W, (p["b2"], p["b3"], p["b4"])):
+        z = hs[-1] @ w.T + b
+        zs.append(z)
+        hs.append(silu(z))
+    y = hs[-1] @ p["w5"] + p["b5"][0]
+    return y, (p, W, zs, hs)
+
+
+def backward(cache, y, t) -> tuple:
+    """Gradient of mean((y - t)^2) with respect to every parameter (flat,
+    the GPU layout) and the loss."""
+    p, W, zs, hs = cache
+    n = len(y)
+    d = y - np.asarray(t, dtype=np.float64)
+    loss = float(np.mean(d * d))
+    dy = 2.0 * d / n
+    g = {"w5": hs[4].T @ dy, "b5": np.array([dy.sum()])}
+    dz = np.outer(dy, p["w5"]) * silu_grad(zs[3])
+    for li in (2, 1, 0):                      # W4, W3, W2
+        name = ("W2", "W3", "W4")[li]
+        bname = ("b2", "b3", "b4")[li]
+        g[name] = dz.T @ hs[li + 1]
+        g[bname] = dz.sum(axis=0)
+        dz = (dz @ W[li]) * silu_grad(zs[li])
+    g["W1"] = dz.T @ hs[0]
+    g["b1"] = dz.sum(axis=0)
+    return pack(g), loss
+
+
+def loss_only(params, X, t, bf16: bool = False) -> float:
+    y, _ = forward(params, X, bf16)
+    d = y - np.asarray(t, dtype=np.float64)
+    return float(np.mean(d * d))
+
+
+def kp_rotation(axis, angle):
+    """Rodrigues matrix as utils/utils_os.py:26-43 writes it."""
+    x, y, z = axis
+    c, s = np.cos(angle), np.sin(angle)
+    k = 1.0 - c
+    return np.array([[c + x * x * k, x * y * k - z * s, y * s + x * z * k],
+                     [z * s + x * y * k, c + y * y * k, -x * s + y * z * k],
+                     [-y * s + x * z * k, x * s + y * z * k, c + z * z * k]])
+
+
+def crystal_frame(s1, omega, raw_xray, omega_axis, F):
+    """Incident and diffracted vectors of each reflection in the crystal
+    frame, (R(omega) F)^T v (ray_tracing.c:397-405), as unit vectors."""
+    s1 = np.asarray(s1, dtype=np.float64).reshape(-1, 3)
+    Fm = np.asarray(F, dtype=np.float64).reshape(3, 3)
+    xs, ss = [], []
+    for v, om in zip(s1, np.asarray(omega, dtype=np.float64).reshape(-1)):
+        M = kp_rotation(omega_axis, om) @ Fm
+        xs.append(M.T @ np.asarray(raw_xray, dtype=np.float64))
+        ss.append(M.T @ v)
+    x = np.array(xs).reshape(-1, 3)
+    s = np.array(ss).reshape(-1, 3)
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    s /= np.linalg.norm(s, axis=1, keepdims=True)
+    return x, s
+
+
+def harmonics(v):
+    return np.stack([2.0 * v[:, 0] * v[:, 1], v[:, 0] ** 2 - v[:, 1] ** 2,
+                     2.0 * v[:, 2] ** 2 - 1.0], axis=1)
+
+
+def chamber(x, s):
+    """Sides (+-1, 0 on the plane) of the planes where the tracer's discrete
+    rules switch: sign x2, sign(|x1| - |x2|), sign(|x0| - |x1|), sign s2,
+    sign(|s0| - |s1|), sign(|s1| - |s2|).  theta > 0 of dials_2_thetaphi_22
+    (ray_tracing_basic.c:13-50) is the sign of the z component, which picks
+    floor or round in cal_coord (ray_tracing_basic.c:428-438)."""
+    a = np.abs
+    return np.stack([np.sign(x[:, 2]), np.sign(a(x[:, 1]) - a(x[:, 2])),
+                     np.sign(a(x[:, 0]) - a(x[:, 1])), np.sign(s[:, 2]),
+                     np.sign(a(s[:, 0]) - a(s[:, 1])), np.sign(a(s[:, 1]) - a(s[:, 2]))], axis=1)
+
+
+def features(s1, omega, raw_xray, omega_axis, F, axis_c=None, feature_set: int = 0):
+    """Network inputs [R][12] and the envelope invariants [R][2] (axial, cos2t);
+    axis_c defaults to the unit F^T omega_axis.  feature_set 0: second
+    harmonics, 1: chamber signs."""
+    x, s = crystal_frame(s1, omega, raw_xray, omega_axis, F)
+    if axis_c is None:
+        u = np.asarray(F, dtype=np.float64).reshape(3, 3).T @ np.asarray(omega_axis, float)
+        axis_c = u / np.linalg.norm(u)
+    extra = (np.concatenate([harmonics(x), harmonics(s)], axis=1) if feature_set == 0
+             else chamber(x, s))
+    X = np.concatenate([x, s, extra], axis=1)
+    inv = np.stack([x @ np.asarray(axis_c, float), np.sum(x * s, axis=1)], axis=1)
+    return X, inv
+
+
+def invariants(s1, omega, raw_xray, omega_axis, F, axis, e1, e2):
+    """(axial, two_theta, alpha, beta) per reflection: alpha = azimuth of x
+    about the axis in the (e1, e2) basis, beta = azimuth of s about x in the
+    basis (axis x x, x (axis x x))."""
+    x, s = crystal_frame(s1, omega, raw_xray, omega_axis, F)
+    u = np.asarray(axis, dtype=np.float64)
+    c = np.sum(x * s, axis=1)
+    tth = np.arccos(np.clip(c, -1.0, 1.0))
+    alpha = np.arctan2(x @ np.asarray(e2, float), x @ np.asarray(e1, float))
+    f1 = np.cross(u[None, :], x)
+    f1 /= np.linalg.norm(f1, axis=1, keepdims=True)
+    f2 = np.cross(x, f1)
+    p = s - c[:, None] * x
+    beta = np.arctan2(np.sum(p * f2, axis=1), np.sum(p * f1, axis=1))
+    return np.stack([x @ u, tth, alpha, beta], axis=1)
+
+
+def cell_index(inv, two_theta_range, cells):
+    """Trust-grid cell of each reflection: alpha and beta over [-pi, pi],
+    two_theta over its range, bins clamped to the grid."""
+    def bin_(v, lo, hi, n):
+        k = np.floor((v - lo) / (hi - lo) * n) if hi > lo else np.zeros_like(v)
+        return np.clip(k, 0, n - 1).astype(np.int64)
+    ia = bin_(inv[:, 2], -np.pi, np.pi, cells[0])
+    it = bin_(inv[:, 1], two_theta_range[0], two_theta_range[1], cells[1])
+    ib = bin_(inv[:, 3], -np.pi, np.pi, cells[2])
+    return (ia * cells[1] + it) * cells[2] + ib

@@ -1,0 +1,132 @@
+"""Pins the FP64 CPU restatement of the surrogate network (oracle/mlp_oracle.py)
+before it is used to judge the GPU kernels: finite differences for the
+backward pass, a library conversion and tie cases for BF16 rounding, closed
+forms for SiLU, and geometric identities for the features."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as MO
+
+
+def _params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    parts = {}
+    for name, (a, b, shape) in MO.offsets().items():
+        fan_in = shape[1] if len(shape) == 2 else MO.WIDTH
+        lim = scale * np.sqrt(6.0 / fan_in)
+        parts[name] = rng.uniform(-lim, lim, size=shape)
+    return MO.pack(parts)
+
+
+def test_backward_matches_central_differences():
+    rng = np.random.default_rng(1)
+    P = _params(2)
+    X = rng.normal(size=(6, 12))
+    t = rng.normal(size=6)
+    y, cache = MO.forward(P, X, bf16=False)
+    g, loss = MO.backward(cache, y, t)
+    assert loss == pytest.approx(MO.loss_only(P, X, t), rel=1e-14)
+    h = 1e-6
+    for name, (a, b, _) in MO.offsets().items():
+        for j in rng.choice(np.arange(a, b), size=min(6, b - a), replace=False):
+            Pp, Pm = P.copy(), P.copy()
+            Pp[j] += h
+            Pm[j] -= h
+            fd = (MO.loss_only(Pp, X, t) - MO.loss_only(Pm, X, t)) / (2 * h)
+            assert g[j] == pytest.approx(fd, rel=1e-5, abs=1e-9), (name, j)
+
+
+def test_bf16_rounding_ties_and_library():
+    one = np.float32(1.0)
+    ulp = 2.0 ** -7
+    # exact halfway cases go to the even mantissa
+    assert MO.bf16_round(np.float32(1 + ulp / 2))[()] == 1.0
+    assert MO.bf16_round(np.float32(1 + 3 * ulp / 2))[()] == 1 + 2 * ulp
+    assert MO.bf16_round(np.float32(1 + ulp / 2 + 2.0 ** -20))[()] == 1 + ulp
+    assert MO.bf16_round(-one)[()] == -1.0
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.normal(size=20000).astype(np.float32),
+                        (rng.integers(1, 1 << 16, size=2000) * 2.0 ** -16 + 1).astype(np.float32)])
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float64).numpy()
+    np.testing.assert_array_equal(MO.bf16_round(x), ref)
+
+
+def test_silu_closed_forms():
+    assert MO.silu(0.0) == 0.0
+    assert MO.silu_grad(0.0) == 0.5
+    assert MO.silu(40.0) == pytest.approx(40.0, rel=1e-15)
+    assert MO.silu_grad(40.0) == pytest.approx(1.0, rel=1e-15)
+    z = np.linspace(-8, 8, 33)
+    h = 1e-6
+    np.testing.assert_allclose(MO.silu_grad(z), (MO.silu(z + h) - MO.silu(z - h)) / (2 * h),
+                               rtol=1e-8, atol=1e-10)
+
+
+def test_features_identities():
+    from anacor_public_amd import workload as W
+    s1, om = W.make_reflections(500, 9)
+    F = W.random_rotation(7).reshape(-1)
+    X, inv = MO.features(s1, om, W.RAW_XRAY, W.OMEGA_AXIS, F)
+    x, s = X[:, 0:3], X[:, 3:6]
+    np.testing.assert_allclose(np.linalg.norm(x, axis=1), 1.0, rtol=1e-14)
+    np.testing.assert_allclose(np.linalg.norm(s, axis=1), 1.0, rtol=1e-14)
+    for v, h in ((x, X[:, 6:9]), (s, X[:, 9:12])):
+        np.testing.assert_allclose(h[:, 0] ** 2 + h[:, 1] ** 2, (v[:, 0] ** 2 + v[:, 1] ** 2) ** 2,
+                                   atol=1e-14)
+        np.testing.assert_allclose(h[:, 2], 2 * v[:, 2] ** 2 - 1, atol=1e-15)
+    # a rotation keeps the angle between the beam and s1 (2 theta) ...
+    b = W.RAW_XRAY / np.linalg.norm(W.RAW_XRAY)
+    np.testing.assert_allclose(inv[:, 1], s1 @ b / np.linalg.norm(s1, axis=1), atol=1e-14)
+    # ... and the beam's component along the goniometer axis (0 here)
+    np.testing.assert_allclose(inv[:, 0], b @ W.OMEGA_AXIS, atol=1e-14)
+
+
+def test_rotation_agrees_with_c_oracle(oracle):
+    rng = np.random.default_rng(4)
+    for _ in range(5):
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        ang = rng.uniform(0, 2 * np.pi)
+        np.testing.assert_allclose(MO.kp_rotation(axis, ang).reshape(-1), oracle.kp_rotation(axis, ang),
+                                   rtol=0, atol=1e-15)
+
+
+def test_invariants_recover_constructed_angles():
+    """Build directions from chosen (alpha, two_theta, beta) and read them back."""
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    e1 = np.cross(u, [1.0, 0, 0])
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(u, e1)
+    n = 200
+    alpha = rng.uniform(-np.pi, np.pi, n)
+    tth = rng.uniform(0.05, 1.2, n)
+    beta = rng.uniform(-np.pi, np.pi, n)
+    x = np.cos(alpha)[:, None] * e1 + np.sin(alpha)[:, None] * e2
+    f1 = np.cross(u[None, :], x)
+    f1 /= np.linalg.norm(f1, axis=1, keepdims=True)
+    f2 = np.cross(x, f1)
+    s = (np.cos(tth)[:, None] * x + np.sin(tth)[:, None] *
+         (np.cos(beta)[:, None] * f1 + np.sin(beta)[:, None] * f2))
+    # omega = 0, F = I, raw_xray = x: the crystal frame is the lab frame
+    inv = np.concatenate([MO.invariants(s[i:i + 1], [0.0], x[i], u, np.eye(3).ravel(), u, e1, e2)
+                          for i in range(n)])
+    np.testing.assert_allclose(inv[:, 0], 0.0, atol=1e-14)
+    np.testing.assert_allclose(inv[:, 1], tth, atol=1e-12)
+    np.testing.assert_allclose(inv[:, 2], alpha, atol=1e-12)
+    np.testing.assert_allclose(inv[:, 3], beta, atol=1e-12)
+    cells = (8, 4, 6)
+    c = MO.cell_index(inv, (0.05, 1.2), cells)
+    ia = np.floor((alpha + np.pi) / (2 * np.pi) * 8)
+    it = np.minimum(np.floor((tth - 0.05) / 1.15 * 4), 3)
+    ib = np.floor((beta + np.pi) / (2 * np.pi) * 6)
+    np.testing.assert_array_equal(c, (ia * 4 + it) * 6 + ib)
+
+
+def test_chamber_signs():
+    x = np.array([[0.0, 0.6, 0.8], [0.0, -0.8, -0.6]])
+    s = np.array([[0.6, -0.48, 0.64], [-0.36, 0.48, -0.8]])
+    np.testing.assert_array_equal(MO.chamber(x, s),
+                                  [[1, -1, -1, 1, 1, -1], [-1, 1, -1, -1, -1, -1]])
