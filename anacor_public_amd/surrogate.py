@@ -206,13 +206,14 @@ def make_envelope(axis, axial_range, two_theta_range, cells=(64, 8, 32)) -> "_li
 class Surrogate:
     """MLP surrogate of one sample's A_h with the tracer as fallback and judge."""
 
-    def __init__(self, tracer, raw_xray, omega_axis, F, seed: int = 0, device: int = 0):
+    def __init__(self, tracer, raw_xray, omega_axis, F, seed: int = 0, device: int = 0,
+                 feature_set: int = _lib.MLP_FEATURES_CHAMBER):
         self.tracer = tracer
         self.raw_xray = _f64(raw_xray).reshape(3)
         self.omega_axis = _f64(omega_axis).reshape(3)
         self.F = _f64(F).reshape(9)
         self.mlp = MLP(seed=seed, device=device)
-        self.feature_set = _lib.MLP_FEATURES_HARMONIC
+        self.feature_set = int(feature_set)
         self.mlp.set_features(self.feature_set)
         self.envelope: Optional[_lib.Envelope] = None
         self.trust: Optional[np.ndarray] = None
@@ -238,15 +239,15 @@ class Surrogate:
     def trace(self, s1, omega) -> np.ndarray:
         return self.tracer.run(s1, omega, *self._geom())
 
-    def fit(self, s1, omega, epochs: int = 200, batch: int = 1024, lr: float = 1e-3,
+    def fit(self, s1, omega, epochs: int = 600, batch: int = 1024, lr: float = 1e-3,
             lr_min: float = 1e-6, seed: int = 0, targets: Optional[np.ndarray] = None,
-            cells=(64, 8, 32), tau: float = 5e-4, min_count: int = 4,
+            cells=(64, 8, 32), tau: Optional[float] = None, min_count: int = 4,
             axial_tol: float = 1e-9) -> dict:
         """Trace the training reflections (unless targets are given), train
         with Adam, and set the envelope: the training axial and two-theta
         ranges, and trust only in the (alpha, two_theta, beta) cells holding at
-        least min_count training reflections whose relative error is at
-        most tau."""
+        least min_count training reflections (and, with tau, whose training
+        relative errors are all at most tau)."""
         t0 = time.perf_counter()
         A = self.trace(s1, omega) if targets is None else _f64(targets)
         t_trace = time.perf_counter() - t0
@@ -274,7 +275,10 @@ class Surrogate:
         count = np.bincount(cell, minlength=n_cells)
         worst = np.zeros(n_cells)
         np.maximum.at(worst, cell, res)
-        trust = ((count >= min_count) & (worst <= tau)).astype(np.uint8)
+        ok = count >= min_count
+        if tau is not None:
+            ok &= worst <= tau
+        trust = ok.astype(np.uint8)
         self.mlp.set_envelope(env, trust)
         self.envelope, self.trust = env, trust
         served = trust[cell].astype(bool)

@@ -10,12 +10,19 @@ starts.  Prints one JSON line (absorption factors per second) with the trace
 kernel's roofline position and the reference C/OpenMP path timed on a bounded
 sample of the same workload on this host's cores.
 
+The learned surrogate (config "surrogate", and a "surrogate" record in the
+default run) is trained on 200k reflections the tracer computes for batched
+sample 2 (100^3, 20,089 crystal voxels), judged by the tracer on 10k held-out
+reflections, and timed on 1M reflections of which 1% lie outside its training
+2-theta range and go to the tracer.
+
 Workloads (anacor_public_amd/workload.py):
   single   256^3 volume, 25,323 crystal voxels (stride 2), 20,000 reflections (default)
   small    64^3 volume, all 3,169 crystal voxels, 256 reflections
   large    1000x1000x800 volume, 100,000 voxels, 100,000 reflections
   batched  8 uneven samples (100^3 .. 383x383x135), 194,281 reflections, one
            persistent graph-launched kernel, autotuned work shapes
+  surrogate  MLP surrogate of batched sample 2, 1M inferences with tracer fallback
 """
 from __future__ import annotations
 
@@ -31,6 +38,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+MLP_FLOPS_PER_ROW = 2 * (12 * 256 + 3 * 256 * 256 + 256)   # 2 x 199,936 MACs
 # L2<->fabric bytes per voxel step of trace_kernel on the 256^3 workload:
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a 1 GiB copy) over
 # 1.017e11 voxel steps, profiles/trace_kernel_plain-256cube.md.
@@ -108,12 +117,89 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def rel_stats(a, ref):
+    r = np.abs(np.asarray(a) - ref) / ref
+    return {"max": float(r.max()), "p99_9": float(np.quantile(r, 0.999)),
+            "p99": float(np.quantile(r, 0.99)), "mean": float(r.mean())}
+
+
+def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int = 600,
+                  cpu_budget: float = 0.0):
+    """Fit the surrogate, judge it on the held-out set, time 1M inferences
+    (features, envelope test, MLP on MFMA, tracer fallback) and the plain GPU
+    path on the same sample.  Returns the surrogate record."""
+    from anacor_public_amd import workload as W
+    from anacor_public_amd.ray_tracing import Tracer
+    from anacor_public_amd.surrogate import Surrogate
+    t0 = time.perf_counter()
+    sw = W.surrogate_config(n_infer=n_infer)
+    wl = sw.sample
+    geo = (wl.raw_xray, wl.omega_axis, wl.F)
+    log(f"# surrogate workload {wl.name}: N={len(wl.coords)}, train {len(wl.omega)}, "
+        f"held-out {len(sw.heldout_omega)}, inference {n_infer} "
+        f"({int(sw.infer_outside.sum())} outside), built in {time.perf_counter() - t0:.1f}s")
+    rec = {}
+    with Tracer.for_workload(wl) as tr, Surrogate(tr, *geo) as sur:
+        # the plain GPU path on the same sample (20k of the inference reflections)
+        ps1, pom = sw.infer_s1[:20000], sw.infer_omega[:20000]
+        tr.run(ps1, pom, *geo)
+        t1 = time.perf_counter()
+        for _ in range(3):
+            tr.run(ps1, pom, *geo)
+        plain_rate = 3 * len(pom) / (time.perf_counter() - t1)
+        rep = sur.fit(wl.s1, wl.omega, epochs=epochs)
+        log(f"# surrogate fit: trace {rep['trace_s']:.2f}s, train {rep['train_s']:.1f}s "
+            f"({epochs} epochs), final loss {rep['final_loss']:.3e}")
+        A_true = tr.run(sw.heldout_s1, sw.heldout_omega, *geo)
+        A_sur, used = sur.predict(sw.heldout_s1, sw.heldout_omega)
+        err = rel_stats(A_sur[used], A_true[used])
+        log(f"# held-out ({int(used.sum())}/{len(used)} answered by the network): {err}")
+        for _ in range(warmup):
+            sur.predict(sw.infer_s1, sw.infer_omega)
+        mlp_ms, fb = [], 0
+        t2 = time.perf_counter()
+        for _ in range(steps):
+            _, u = sur.predict(sw.infer_s1, sw.infer_omega)
+            mlp_ms.append(sur.mlp.stats()["mlp_ms"])
+            fb = int((~u).sum())
+        elapsed = time.perf_counter() - t2
+        value = n_infer * steps / elapsed
+        mms = float(np.mean(mlp_ms))
+        achieved = n_infer * MLP_FLOPS_PER_ROW / (mms * 1e-3) / 1e12
+        rec = {
+            "value": value, "unit": "absorption factors/s", "ms_per_step": 1e3 * elapsed / steps,
+            "dtype": "bf16", "workload": wl.name, "n_voxels": int(len(wl.coords)),
+            "inferences": n_infer, "fallback_reflections": fb,
+            "training_pairs": int(len(wl.omega)), "epochs": epochs,
+            "train_s": rep["train_s"], "train_trace_s": rep["trace_s"],
+            "heldout": int(len(sw.heldout_omega)), "heldout_answered": int(used.sum()),
+            "heldout_rel_error": err,
+            "plain_gpu_same_sample": plain_rate, "speedup_vs_plain_gpu": value / plain_rate,
+            "mlp_only_rate": n_infer / (mms * 1e-3),
+            "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS,
+                         "traffic": None, "kernel": "MLP forward (HIP events)",
+                         "kernel_ms": mms, "flops_per_launch": n_infer * MLP_FLOPS_PER_ROW},
+        }
+        log(f"# surrogate: {value:.3e} A_h/s ({rec['speedup_vs_plain_gpu']:.1f}x the plain GPU "
+            f"path's {plain_rate:.3e} on this sample), MLP {mms:.2f} ms per 1M, fallback {fb}")
+        if cpu_budget > 0:
+            cpu, _ = cpu_baseline([W.Workload(wl.name, wl.volume, wl.coords, sw.infer_s1[:4096],
+                                              sw.infer_omega[:4096])], cpu_budget)
+            rec["cpu_baseline"] = cpu
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="single", choices=["single", "small", "large", "batched"])
+    ap.add_argument("--config", default="single",
+                    choices=["single", "small", "large", "batched", "surrogate"])
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the surrogate record of the default run")
+    ap.add_argument("--epochs", type=int, default=600, help="surrogate training epochs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--tile", type=int, default=0)
@@ -132,6 +218,24 @@ def main():
         sys.exit(2)
 
     from anacor_public_amd import workload as W
+
+    if args.config == "surrogate":
+        rec = run_surrogate(args.steps, args.warmup, epochs=args.epochs,
+                            cpu_budget=0.0 if args.no_cpu_baseline else args.cpu_budget)
+        cpu = rec.pop("cpu_baseline", None)
+        out = {"metric": METRIC, "value": rec["value"], "unit": rec["unit"], "n_gpus": 1,
+               "steps": args.steps, "warmup": args.warmup, "ms_per_step": rec["ms_per_step"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "bf16", "data": "synthetic (seeded four-material volume and reflections)",
+               "config": {"workload": rec["workload"], "path": "surrogate",
+                          "network": "MLP 12-256x4-1 SiLU, BF16 weights, FP32 accumulation",
+                          "inferences": rec["inferences"],
+                          "training_pairs": rec["training_pairs"]},
+               "roofline": rec["roofline"], "cpu_baseline": cpu,
+               "speedup_vs_cpu": (rec["value"] / cpu["value"]) if cpu else None,
+               "surrogate": rec}
+        print(json.dumps(out), flush=True)
+        return
 
     t0 = time.perf_counter()
     if args.config == "batched":
@@ -237,8 +341,10 @@ def main():
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
     }
     rec.update(extra)
-    print(json.dumps(rec), flush=True)
     runner.close()
+    if args.config == "single" and not args.no_surrogate and not args.reflections:
+        rec["surrogate"] = run_surrogate(args.steps, args.warmup, epochs=args.epochs)
+    print(json.dumps(rec), flush=True)
 
 
 if __name__ == "__main__":
