@@ -50,12 +50,16 @@ constexpr int kParams = kOffB5 + 1;
 static_assert(kParams == ANACOR_MLP_PARAMS, "parameter count");
 static_assert(kIn == 12 && kH == 256, "kernels assume 12 inputs and width 256");
 
-constexpr int kChunk = 1 << 16;   // rows per forward pass in inference
+constexpr int kChunk = 1 << 16;   // largest training batch / gradient call
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// sigmoid with the hardware exp and reciprocal (v_exp_f32, v_rcp_f32: about
+// 1 ulp each), the same function in training and inference.
+__device__ __forceinline__ float sigmoidf_(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
 __device__ __forceinline__ float silu(float z) { return z * sigmoidf_(z); }
 __device__ __forceinline__ float silu_grad(float z) {
   const float s = sigmoidf_(z);
@@ -290,6 +294,147 @@ __global__ void split_reduce(const float* __restrict__ ws, int slices, int M, in
   }
 }
 
+// ------------------------------------------------- fused inference kernel
+//
+// The whole network for a tile of 64 rows in one workgroup of 4 waves, on
+// MFMA with the units as the M dimension (D[unit][row] = W . H^T): wave w
+// owns units 64 (w % 4) .. + 63 of every layer for all 64 rows.  Weights come
+// from L2 as 16-byte A fragments (BF16 copies; the 12 -> 256 layer as hi/lo
+// pairs padded to K = 32); activations stay in LDS as BF16 hi/lo [row][unit],
+// so a lane's 4 consecutive units of one row leave as one 8-byte store.  The
+// output layer is reduced in registers.  With `flag`/`A` the kernel also
+// writes A = exp(y std + mean) inside the envelope and NaN outside.
+
+constexpr int kHP = kH + 8;   // padded LDS row of activations (bf16)
+constexpr int kXP = 32 + 8;   // padded LDS row of the inputs (bf16, K = 12 padded to 32)
+// BF16 weight block: W2, W3, W4 [256][256], then W1 hi and lo [256][32].
+constexpr size_t kWbW1h = (size_t)3 * kH * kH;
+constexpr size_t kWbW1l = kWbW1h + (size_t)kH * 32;
+constexpr size_t kWbSize = kWbW1l + (size_t)kH * 32;
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void mlp_infer_kernel(
+    const float* __restrict__ X, int64_t n, const float* __restrict__ P,
+    const __bf16* __restrict__ Wb, const uint8_t* __restrict__ flag, double mean, double stdv,
+    float* __restrict__ y, double* __restrict__ A) {
+  constexpr int ROWS = 64;
+  __shared__ __attribute__((aligned(16))) __bf16 Hh[ROWS][kHP];
+  __shared__ __attribute__((aligned(16))) __bf16 Hl[ROWS][kHP];
+  __shared__ __attribute__((aligned(16))) __bf16 Xh[ROWS][kXP];
+  __shared__ __attribute__((aligned(16))) __bf16 Xl[ROWS][kXP];
+  __shared__ float part[4][ROWS];
+  const int t = (int)threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * ROWS;
+  for (int i = t; i < ROWS * 32; i += 256) {   // inputs as hi/lo, k >= 12 zero
+    const int r = i >> 5, k = i & 31;
+    const float v = (k < kIn && r0 + r < n) ? X[(r0 + r) * kIn + k] : 0.f;
+    const __bf16 h = (__bf16)v;
+    Xh[r][k] = h;
+    Xl[r][k] = (__bf16)(v - (float)h);
+  }
+  __syncthreads();
+  const int lane = t & 63, wave = t >> 6;
+  const int ub = wave * 64;                     // this wave's units
+  const int r16 = lane & 15, kg = (lane >> 4) * 8, q4 = (lane >> 4) * 4;
+  floatx4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  {  // layer 1: W1 (hi, lo) x X (hi, lo), three products
+    bf16x8 wh[4], wl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      wh[i] = *(const bf16x8*)(Wb + kWbW1h + (size_t)(ub + i * 16 + r16) * 32 + kg);
+      wl[i] = *(const bf16x8*)(Wb + kWbW1l + (size_t)(ub + i * 16 + r16) * 32 + kg);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const bf16x8 xh = *(const bf16x8*)&Xh[j * 16 + r16][kg];
+      const bf16x8 xl = *(const bf16x8*)&Xl[j * 16 + r16][kg];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[i], xh, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[i], xl, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[i], xh, acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+  const int offb[4] = {kOffB1, kOffB2, kOffB3, kOffB4};
+  for (int l = 0; l < 4; ++l) {
+    if (l > 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+      const __bf16* W = Wb + (size_t)(l - 1) * kH * kH;
+      for (int k0 = 0; k0 < kH; k0 += 32) {
+        bf16x8 w[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          w[i] = *(const bf16x8*)(W + (size_t)(ub + i * 16 + r16) * kH + k0 + kg);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const bf16x8 hh = *(const bf16x8*)&Hh[j * 16 + r16][k0 + kg];
+          const bf16x8 hl = *(const bf16x8*)&Hl[j * 16 + r16][k0 + kg];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[i], hl, acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[i], hh, acc[i][j], 0, 0, 0);
+          }
+        }
+      }
+      __syncthreads();   // every wave has read this layer's input
+    }
+    if (l == 3) break;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int u = ub + i * 16 + q4;
+      const float4 bias = *(const float4*)(P + offb[l] + u);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = j * 16 + r16;
+        bf16x4 hh, hl;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float h = silu(acc[i][j][r] + (r == 0 ? bias.x : r == 1 ? bias.y
+                                                : r == 2 ? bias.z : bias.w));
+          hh[r] = (__bf16)h;
+          hl[r] = (__bf16)(h - (float)hh[r]);
+        }
+        *(bf16x4*)&Hh[row][u] = hh;
+        *(bf16x4*)&Hl[row][u] = hl;
+      }
+    }
+    __syncthreads();
+  }
+  // output layer: per row, this wave's 64 units of SiLU(z4) . w5
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    float v = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int u = ub + i * 16 + q4;
+      const float4 bias = *(const float4*)(P + kOffB4 + u);
+      const float4 w5 = *(const float4*)(P + kOffW5 + u);
+      v = fmaf(silu(acc[i][j][0] + bias.x), w5.x, v);
+      v = fmaf(silu(acc[i][j][1] + bias.y), w5.y, v);
+      v = fmaf(silu(acc[i][j][2] + bias.z), w5.z, v);
+      v = fmaf(silu(acc[i][j][3] + bias.w), w5.w, v);
+    }
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    if (lane < 16) part[wave][j * 16 + lane] = v;
+  }
+  __syncthreads();
+  if (t < ROWS && r0 + t < n) {
+    const float yv = ((part[0][t] + part[1][t]) + (part[2][t] + part[3][t])) + P[kOffB5];
+    y[r0 + t] = yv;
+    if (A) A[r0 + t] = flag[r0 + t] ? exp((double)yv * stdv + mean) : __builtin_nan("");
+  }
+}
+
 // ------------------------------------------------------- small kernels
 
 // H1 = SiLU(X W1^T + b1), FP32; one thread per (row, unit).  With perm
@@ -392,7 +537,24 @@ __global__ void head_finish(const float* __restrict__ part, int nb, float* __res
   }
 }
 
-// Adam on every parameter; W2..W4 also refresh their BF16 copies.
+// BF16 copies of W2, W3, W4 (round to nearest even) and W1 as hi/lo pairs
+// padded to K = 32 (the inference kernel's layout).
+__device__ __forceinline__ void store_bf16_copy(int i, float p, __bf16* __restrict__ Wb) {
+  int l = -1, j = 0;
+  if (i >= kOffW2 && i < kOffB2) { l = 0; j = i - kOffW2; }
+  else if (i >= kOffW3 && i < kOffB3) { l = 1; j = i - kOffW3; }
+  else if (i >= kOffW4 && i < kOffB4) { l = 2; j = i - kOffW4; }
+  if (l >= 0) {
+    Wb[(size_t)l * kH * kH + j] = (__bf16)p;
+  } else if (i >= kOffW1 && i < kOffB1) {
+    const int u = (i - kOffW1) / kIn, k = (i - kOffW1) % kIn;
+    const __bf16 h = (__bf16)p;
+    Wb[kWbW1h + (size_t)u * 32 + k] = h;
+    Wb[kWbW1l + (size_t)u * 32 + k] = (__bf16)(p - (float)h);
+  }
+}
+
+// Adam on every parameter, refreshing the BF16 copies of the weights.
 __global__ void adam_step(float* __restrict__ P, const float* __restrict__ G,
                           float* __restrict__ m, float* __restrict__ v, __bf16* __restrict__ Wb,
                           float lr, float b1, float b2, float eps, float c1, float c2) {
@@ -405,20 +567,12 @@ __global__ void adam_step(float* __restrict__ P, const float* __restrict__ G,
   v[i] = vi;
   const float p = P[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
   P[i] = p;
-  int l = -1, j = 0;
-  if (i >= kOffW2 && i < kOffB2) { l = 0; j = i - kOffW2; }
-  else if (i >= kOffW3 && i < kOffB3) { l = 1; j = i - kOffW3; }
-  else if (i >= kOffW4 && i < kOffB4) { l = 2; j = i - kOffW4; }
-  if (l >= 0) Wb[(size_t)l * kH * kH + j] = (__bf16)p;
+  store_bf16_copy(i, p, Wb);
 }
 
-// BF16 copies of W2, W3, W4 (round to nearest even).
 __global__ void weights_to_bf16(const float* __restrict__ P, __bf16* __restrict__ Wb) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (i >= 3 * kH * kH) return;
-  const int l = i / (kH * kH), j = i % (kH * kH);
-  const int off = l == 0 ? kOffW2 : l == 1 ? kOffW3 : kOffW4;
-  Wb[i] = (__bf16)P[off + j];
+  if (i < kParams) store_bf16_copy(i, P[i], Wb);
 }
 
 __global__ void zero_floats(float* __restrict__ p, int n) {
@@ -558,14 +712,6 @@ __global__ void features_kernel(const double* __restrict__ s1, const double* __r
   }
 }
 
-// A = exp(y std + mean) inside the envelope, NaN outside.
-__global__ void finish_predict(const float* __restrict__ y, const uint8_t* __restrict__ flag,
-                               int64_t R, double mean, double stdv, double* __restrict__ A) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= R) return;
-  A[i] = flag[i] ? exp((double)y[i] * stdv + mean) : __builtin_nan("");
-}
-
 // ---------------------------------------------------------------- model
 
 template <typename T>
@@ -651,27 +797,17 @@ static int ensure_rows(Model* M, int n) {
 }
 
 static void refresh_bf16(Model* M) {
-  hipLaunchKernelGGL(weights_to_bf16, dim3(nblocks(3 * kH * kH, 256)), dim3(256), 0, M->stream,
+  hipLaunchKernelGGL(weights_to_bf16, dim3(nblocks(kParams, 256)), dim3(256), 0, M->stream,
                      M->P.p, M->Wb.p);
 }
 
-// Inference forward pass over n rows of device features X into y[n].
-static void forward(Model* M, const float* X, int n, float* y) {
-  hipStream_t s = M->stream;
-  hipLaunchKernelGGL(layer1_fwd, dim3(nblocks((long long)n * kH, 256)), dim3(256), 0, s, X,
-                     (const int*)nullptr, (const float*)nullptr, n, M->P.p, (float*)nullptr,
-                     (float*)nullptr, (float*)nullptr, M->H[0].p);
-  const int offb[3] = {kOffB2, kOffB3, kOffB4};
-  for (int l = 0; l < 3; ++l) {
-    GemmArgs g{};
-    g.M = n; g.N = kH; g.K = kH;
-    g.A = M->H[l].p; g.lda = kH;                       // [n][256]
-    g.B = M->Wb.p + (size_t)l * kH * kH; g.ldb = kH;   // W[out][in] = [N][K]
-    g.C = M->H[l + 1].p; g.ldc = kH;
-    g.bias = M->P.p + offb[l];
-    gemm<true, true, true, false, kEpiBiasSilu>(g, s);
-  }
-  hipLaunchKernelGGL(out_fwd, dim3(nblocks(n, 4)), dim3(256), 0, s, M->H[3].p, n, M->P.p, y);
+// Inference over n rows of device features X: y[n] and, with flag/A, the
+// absorption factors (fused inference kernel, one launch).
+static void infer(Model* M, const float* X, int64_t n, const uint8_t* flag, float* y,
+                  double* A) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3(nblocks(n, 64)), dim3(256), 0, M->stream, X, n,
+                     M->P.p, M->Wb.p, flag, M->y_mean, M->y_std, y, A);
 }
 
 // K slices of a weight-gradient GEMM with K = n rows.
@@ -874,7 +1010,7 @@ anacor_mlp* anacor_mlp_create(uint64_t seed, int device) {
   if (hipEventCreate(&M->ev[0]) != hipSuccess || hipEventCreate(&M->ev[1]) != hipSuccess)
     return fail("hipEventCreate");
   if (M->P.grow(kParams) || M->G.grow(kParams) || M->m.grow(kParams) || M->v.grow(kParams) ||
-      M->Wb.grow((size_t)3 * kH * kH) || M->loss.grow(1))
+      M->Wb.grow(kWbSize) || M->loss.grow(1))
     return fail("hipMalloc");
   // uniform He-style initialisation, zero biases
   std::vector<float> w(kParams, 0.f);
@@ -895,6 +1031,9 @@ anacor_mlp* anacor_mlp_create(uint64_t seed, int device) {
                      kParams);
   hipLaunchKernelGGL(zero_floats, dim3(nblocks(kParams, 256)), dim3(256), 0, M->stream, M->v.p,
                      kParams);
+  // the K padding of the W1 hi/lo block stays zero for the model's lifetime
+  if (hipMemsetAsync(M->Wb.p, 0, kWbSize * sizeof(__bf16), M->stream) != hipSuccess)
+    return fail("hipMemsetAsync");
   refresh_bf16(M);
   if (hipStreamSynchronize(M->stream) != hipSuccess) return fail("initialisation");
   return h;
@@ -1063,13 +1202,9 @@ int anacor_mlp_forward(anacor_mlp* mlp, const float* x, int64_t n, float* y) {
   MLP_TRY(hipSetDevice(M->device));
   MLP_CHECK(M->X.grow((size_t)n * kIn));
   MLP_CHECK(M->y.grow((size_t)n));
-  MLP_CHECK(ensure_rows(M, (int)std::min<int64_t>(n, kChunk)));
   MLP_TRY(hipMemcpyAsync(M->X.p, x, (size_t)n * kIn * sizeof(float), hipMemcpyHostToDevice,
                          M->stream));
-  for (int64_t s = 0; s < n; s += kChunk) {
-    const int c = (int)std::min<int64_t>(kChunk, n - s);
-    forward(M, M->X.p + s * kIn, c, M->y.p + s);
-  }
+  infer(M, M->X.p, n, nullptr, M->y.p, nullptr);
   MLP_TRY(hipGetLastError());
   MLP_TRY(hipMemcpyAsync(y, M->y.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
                          M->stream));
@@ -1185,15 +1320,9 @@ int anacor_mlp_predict(anacor_mlp* mlp, const double* s1, const double* omega, i
   MLP_CHECK(run_features(M, s1, omega, R, g, false, false));
   MLP_CHECK(M->y.grow((size_t)R));
   MLP_CHECK(M->A.grow((size_t)R));
-  MLP_CHECK(ensure_rows(M, (int)std::min<int64_t>(R, kChunk)));
   MLP_TRY(hipEventRecord(M->ev[0], M->stream));
-  for (int64_t s = 0; s < R; s += kChunk) {
-    const int c = (int)std::min<int64_t>(kChunk, R - s);
-    forward(M, M->X.p + s * kIn, c, M->y.p + s);
-  }
+  infer(M, M->X.p, R, M->flag.p, M->y.p, M->A.p);
   MLP_TRY(hipEventRecord(M->ev[1], M->stream));
-  hipLaunchKernelGGL(finish_predict, dim3(nblocks(R, 256)), dim3(256), 0, M->stream, M->y.p,
-                     M->flag.p, R, M->y_mean, M->y_std, M->A.p);
   MLP_TRY(hipGetLastError());
   MLP_TRY(hipMemcpyAsync(A, M->A.p, (size_t)R * sizeof(double), hipMemcpyDeviceToHost,
                          M->stream));
