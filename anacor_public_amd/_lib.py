@@ -22,6 +22,11 @@ ANACOR_ERR_HIP = 3
 ANACOR_ERR_GEOMETRY = 4
 FLAG_F64_MEAN = 1
 FLAG_NO_EXP = 2
+LAYOUT_AUTO = -1
+LAYOUT_RAW = 0
+LAYOUT_BRICK = 1
+LAYOUT_BRICK2 = 2
+LAYOUT_BRICK4 = 4
 RAY_ERR_FACE = 1
 RAY_ERR_ASSERT = 2
 
@@ -29,7 +34,8 @@ RAY_ERR_ASSERT = 2
 EXPORTED = (
     "ray_tracing_overall", "ray_tracing_gpu_overall", "ray_tracing_single",
     "ray_tracing_single_mp", "anacor_last_error", "anacor_device_count", "anacor_ctx_create",
-    "anacor_ctx_destroy", "anacor_ctx_set_tile", "anacor_ctx_set_voxel_block", "anacor_ctx_run", "anacor_ctx_paths",
+    "anacor_ctx_destroy", "anacor_ctx_set_tile", "anacor_ctx_set_voxel_block",
+    "anacor_ctx_set_layout", "anacor_ctx_get_layout", "anacor_ctx_run", "anacor_ctx_paths",
     "anacor_ctx_stats", "anacor_reflection_geometry", "anacor_batch_create",
     "anacor_batch_destroy", "anacor_batch_sample", "anacor_batch_run", "anacor_batch_stats",
     "anacor_mlp_create", "anacor_mlp_destroy", "anacor_mlp_num_params", "anacor_mlp_get_params",
@@ -94,6 +100,10 @@ def _prototypes(lib):
     lib.anacor_ctx_destroy.argtypes = [vp]
     lib.anacor_ctx_set_tile.restype = ct.c_int
     lib.anacor_ctx_set_tile.argtypes = [vp, i32]
+    lib.anacor_ctx_set_layout.restype = ct.c_int
+    lib.anacor_ctx_set_layout.argtypes = [vp, i32]
+    lib.anacor_ctx_get_layout.restype = ct.c_int
+    lib.anacor_ctx_get_layout.argtypes = [vp]
     lib.anacor_ctx_set_voxel_block.restype = ct.c_int
     lib.anacor_ctx_set_voxel_block.argtypes = [vp, i32, i32, i32]
     lib.anacor_ctx_run.restype = ct.c_int

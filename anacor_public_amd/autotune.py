@@ -1,12 +1,14 @@
 """Autotuner for the trace kernels' work shapes.
 
-Two knobs change speed but never a path length:
+Three knobs change speed but never a path length:
 
 * the wave grouping of crystal voxels (``(bz, by, bx)`` blocks, lanes along
   x; ``(0, 0, 0)`` = Morton order), which decides how many volume rows a
   wave's 64 parallel rays touch per step;
 * the tile size (voxels per workgroup), which trades partial sums and
-  scheduling granularity against load balance.
+  scheduling granularity against load balance;
+* the volume layout (raw uint8 or 8x8x8 bricks of 2-/4-bit labels) of a
+  single-sample tracer (the batched kernel reads the raw layout).
 
 ``tune(tracer, ...)`` times every candidate on a slice of the sample's own
 reflections with the trace kernel's HIP-event duration and keeps the fastest.
@@ -25,6 +27,7 @@ import numpy as np
 BLOCKS: Tuple[Tuple[int, int, int], ...] = ((1, 2, 64), (1, 1, 128), (2, 2, 32), (1, 4, 32),
                                              (2, 4, 16), (0, 0, 0))
 TILES: Tuple[int, ...] = (0, 4096, 8192)      # 0 = the library's automatic choice
+LAYOUTS: Tuple[int, ...] = (0, 1)             # raw, bricks (_lib.LAYOUT_RAW, _lib.LAYOUT_BRICK)
 
 _lock = threading.Lock()
 
@@ -76,14 +79,18 @@ def _store_cache(key: str, entry: dict) -> None:
 
 def tune(tracer, s1, omega, raw_xray, omega_axis, F, coords, trial: int = 512,
          blocks: Sequence[Tuple[int, int, int]] = BLOCKS, tiles: Iterable[int] = TILES,
-         use_cache: bool = True, repeats: int = 2) -> dict:
-    """Pick the fastest (block, tile) for ``tracer`` (a Tracer or a borrowed
-    batch-sample handle) on its first ``trial`` reflections, apply it, and
-    return {"block": ..., "tile": ..., "ms": ..., "table": [...]}."""
-    key = sample_key(tracer.shape, coords)
+         layouts: Optional[Sequence[int]] = None, use_cache: bool = True,
+         repeats: int = 2) -> dict:
+    """Pick the fastest (layout, block, tile) for ``tracer`` (a Tracer, or a
+    borrowed batch-sample handle with ``layouts=None``) on its first ``trial``
+    reflections, apply it, and return {"layout", "block", "tile", "ms",
+    "table"}.  layouts None leaves the layout alone."""
+    key = sample_key(tracer.shape, coords) + ("" if layouts is None else "|layouts")
     if use_cache:
         hit = _load_cache().get(key)
         if hit:
+            if layouts is not None:
+                tracer.set_layout(int(hit["layout"]))
             tracer.set_voxel_block(*hit["block"])
             tracer.set_tile(int(hit["tile"]))
             return dict(hit, cached=True)
@@ -91,15 +98,22 @@ def tune(tracer, s1, omega, raw_xray, omega_axis, F, coords, trial: int = 512,
     omega = np.ascontiguousarray(omega)[:trial]
     table = []
     best = None
-    for blk in blocks:
-        tracer.set_voxel_block(*blk)
-        for tile in tiles:
-            tracer.set_tile(int(tile))
-            tracer.run(s1, omega, raw_xray, omega_axis, F)          # warm
-            ms = min(_timed(tracer, s1, omega, raw_xray, omega_axis, F) for _ in range(repeats))
-            table.append({"block": list(blk), "tile": int(tile), "ms": ms})
-            if best is None or ms < best["ms"]:
-                best = {"block": list(blk), "tile": int(tile), "ms": ms}
+    for lay in (layouts if layouts is not None else (None,)):
+        if lay is not None:
+            tracer.set_layout(int(lay))
+        for blk in blocks:
+            tracer.set_voxel_block(*blk)
+            for tile in tiles:
+                tracer.set_tile(int(tile))
+                tracer.run(s1, omega, raw_xray, omega_axis, F)          # warm
+                ms = min(_timed(tracer, s1, omega, raw_xray, omega_axis, F)
+                         for _ in range(repeats))
+                row = {"layout": lay, "block": list(blk), "tile": int(tile), "ms": ms}
+                table.append(row)
+                if best is None or ms < best["ms"]:
+                    best = dict(row)
+    if best["layout"] is not None:
+        tracer.set_layout(int(best["layout"]))
     tracer.set_voxel_block(*best["block"])
     tracer.set_tile(best["tile"])
     entry = dict(best, table=table, trial=int(len(omega)))

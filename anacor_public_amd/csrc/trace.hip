@@ -103,7 +103,9 @@ struct PathRecord {           // per (reflection, voxel): both rays
 static_assert(sizeof(PathRecord) == 120, "PathRecord layout");
 
 struct TraceArgs {
-  const uint8_t* vol;
+  const uint8_t* vol;          // raw volume or, for the bricked layouts, the packed one
+  int vol_bytes;               // bytes behind vol (range check of the loads)
+  int cz, cy;                  // brick constants (bricked layouts)
   int Z, Y, X;
   const int4* coords;          // (z, y, x, original index)
   int N;
@@ -224,6 +226,7 @@ struct RayLane {
   int p0, pdir, paxis;   // face axis origin, direction (+1/-1), index (0 z, 1 y, 2 x)
   int limit;             // increments tried: inc in [0, limit)
   int vac;               // full_iteration vacuum-run limit
+  int ca, cb, cp, sp;    // bricked layouts: brick constants of a, b, p and stride of p
 };
 
 template <bool GUARD, bool WIDE>
@@ -248,13 +251,54 @@ __device__ __forceinline__ void lane_zyx(const RayLane& L, int inc, int& z, int&
   else { z = ia; y = ib; x = ip; }
 }
 
+// Volume layouts (the LAYOUT template parameter of the walk):
+//   kLayoutRaw    uint8 [z][y][x], 24-bit address arithmetic
+//   kLayoutWide   uint8 [z][y][x], 32-bit address arithmetic
+//   kLayoutBrick2 8x8x8 bricks of 2-bit labels (one 128-byte line per brick)
+//   kLayoutBrick4 8x8x8 bricks of 4-bit codes: labels 0..4 as they are, any
+//                 other value as 5 (the walk only asks whether a label is 0,
+//                 whether it differs from the previous one and whether it is
+//                 in 0..4, so the boundaries are unchanged)
+// Brick b = (bz * NBY + by) * NBX + bx holds voxel (z&7, y&7, x&7) at index
+// 512 b + 64 (z&7) + 8 (y&7) + (x&7); per axis that index is
+// i * s + (i >> 3) * c with (s, c) = (64, 512 NBX NBY - 512),
+// (8, 512 NBX - 64), (1, 504) for z, y, x.
+enum { kLayoutRaw = 0, kLayoutWide = 1, kLayoutBrick2 = 2, kLayoutBrick4 = 3 };
+
 struct Volume {
   __amdgpu_buffer_rsrc_t rsrc;  // range-checked: offsets past the end read 0
   int Z, Y, X;
+  int cz, cy;                   // brick constants c of z and y (bricked layouts)
 };
 
 __device__ __forceinline__ int load_label(const Volume& V, int off) {
   return __builtin_amdgcn_raw_buffer_load_b8(V.rsrc, off, 0, 0);
+}
+
+// Label of brick index f in a packed layout: one dword load and a bit-field
+// extract (16 2-bit or 8 4-bit labels per dword).
+template <int LAYOUT>
+__device__ __forceinline__ int load_packed(const Volume& V, int f) {
+  constexpr int BITS = LAYOUT == kLayoutBrick2 ? 2 : 4;
+  constexpr int SH = LAYOUT == kLayoutBrick2 ? 4 : 3;
+  const int off = (f >> SH) << 2;
+  const unsigned dw = __builtin_amdgcn_raw_buffer_load_b32(V.rsrc, off, 0, 0);
+  return (int)__builtin_amdgcn_ubfe(dw, (unsigned)(f * BITS), BITS);
+}
+
+// Label at increment inc of the lane's ray in the given layout.
+template <bool GUARD, int LAYOUT>
+__device__ __forceinline__ int label_at(const RayLane& L, const Volume& V, int inc) {
+  if (LAYOUT == kLayoutRaw || LAYOUT == kLayoutWide)
+    return load_label(V, lane_offset<GUARD, LAYOUT == kLayoutWide>(L, inc));
+  const double incd = (double)inc;
+  const int ia = axis_index<GUARD>(L.ba, incd, L.da, L.bias, L.ha);
+  const int ib = axis_index<GUARD>(L.bb, incd, L.db, L.bias, L.hb);
+  const int ip = mad24(L.pdir, min(inc, L.incmax), L.p0);
+  // every factor fits 24 bits and every partial sum 31 (checked on the host)
+  const int f = mad24(ia, L.sa, mad24(ia >> 3, L.ca, mad24(ib, L.sb, mad24(ib >> 3, L.cb,
+                mad24(ip, L.sp, mad24(ip >> 3, L.cp, 0))))));
+  return load_packed<LAYOUT>(V, f);
 }
 
 // Walk one ray (cal_coord + cal_path2_plus).  Boundaries beyond the implicit
@@ -264,7 +308,7 @@ __device__ __forceinline__ int load_label(const Volume& V, int off) {
 // len_path_2; out receives [li, lo, cr, bu].
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
-template <int CAP, int U, bool GUARD, bool FULL, bool WIDE>
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT>
 __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, const double vs[3],
                                          lds_u32* seg, int& len_out, int& nseg_out,
                                          int last_zyx[3], double out[4]) {
@@ -275,7 +319,7 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
   lds_u32* wp = seg;                          // write slot (32-bit LDS address)
   lds_u32* const trash = seg + CAP * kBlock;  // the scratch slot
   if (!done) {  // increment 0: its label opens no boundary
-    const int lab = load_label(V, lane_offset<GUARD, WIDE>(L, 0));
+    const int lab = label_at<GUARD, LAYOUT>(L, V, 0);
     bool stop;
     if (FULL) {
       va = (lab == 0) ? 1 : 0;
@@ -290,7 +334,7 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
     int lab[U];
 #pragma unroll
     for (int u = 0; u < U; ++u)  // offsets stay inside the volume (clamped)
-      lab[u] = load_label(V, lane_offset<GUARD, WIDE>(L, inc0 + u));
+      lab[u] = label_at<GUARD, LAYOUT>(L, V, inc0 + u);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int inc = inc0 + u;
@@ -354,10 +398,10 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
     }
   } else {
     // the boundary list overflowed: walk the (now known) path again in order
-    int pl = load_label(V, lane_offset<GUARD, WIDE>(L, 0));
+    int pl = label_at<GUARD, LAYOUT>(L, V, 0);
     nseg = 0;
     for (int inc = 1; inc <= last; ++inc) {
-      const int lb = load_label(V, lane_offset<GUARD, WIDE>(L, inc));
+      const int lb = label_at<GUARD, LAYOUT>(L, V, inc);
       if (lb != pl && (unsigned)lb <= 4u) {
         add(cls, inc - start);
         cls = lb;
@@ -373,30 +417,36 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
 }
 
 // Set up the lane's ray for exit face `face` (1..6) of rule `rr`; false when
-// the reference's precondition for that face fails.
+// the reference's precondition for that face fails.  Strides are those of the
+// layout: element strides of [z][y][x], or in-brick strides plus brick
+// constants for the bricked layouts.
+template <int LAYOUT>
 __device__ __forceinline__ bool make_lane(const RayRule* __restrict__ rr, int face, int z, int y,
                                           int x, const Volume& V, RayLane& L) {
+  constexpr bool BRICK = LAYOUT == kLayoutBrick2 || LAYOUT == kLayoutBrick4;
   const uint32_t meta = rr->meta[face - 1];
   const double* d = rr->d[face - 1];
   const int axis = (int)((meta >> kMetaAxisShift) & 3u);
-  const int sz = V.Y * V.X, sy = V.X;
+  const int sz = BRICK ? 64 : V.Y * V.X, sy = BRICK ? 8 : V.X;
+  const int kz = V.cz, ky = V.cy, kx = 504;
   L.bias = (meta & kMetaRound) ? kRoundBias : 0.0;
   L.vac = (int)((meta >> kMetaVacShift) & 0xffu);
   L.paxis = axis;
   int c, ext, sp;
   if (axis == 0) {          // z: a = y, b = x
-    L.ba = (double)y; L.da = d[1]; L.sa = sy; L.ha = V.Y - 1;
-    L.bb = (double)x; L.db = d[2]; L.sb = 1;  L.hb = V.X - 1;
-    sp = sz; c = z; ext = V.Z;
+    L.ba = (double)y; L.da = d[1]; L.sa = sy; L.ha = V.Y - 1; L.ca = ky;
+    L.bb = (double)x; L.db = d[2]; L.sb = 1;  L.hb = V.X - 1; L.cb = kx;
+    sp = sz; c = z; ext = V.Z; L.cp = kz;
   } else if (axis == 1) {   // y: a = z, b = x
-    L.ba = (double)z; L.da = d[0]; L.sa = sz; L.ha = V.Z - 1;
-    L.bb = (double)x; L.db = d[2]; L.sb = 1;  L.hb = V.X - 1;
-    sp = sy; c = y; ext = V.Y;
+    L.ba = (double)z; L.da = d[0]; L.sa = sz; L.ha = V.Z - 1; L.ca = kz;
+    L.bb = (double)x; L.db = d[2]; L.sb = 1;  L.hb = V.X - 1; L.cb = kx;
+    sp = sy; c = y; ext = V.Y; L.cp = ky;
   } else {                  // x: a = z, b = y
-    L.ba = (double)z; L.da = d[0]; L.sa = sz; L.ha = V.Z - 1;
-    L.bb = (double)y; L.db = d[1]; L.sb = sy; L.hb = V.Y - 1;
-    sp = 1; c = x; ext = V.X;
+    L.ba = (double)z; L.da = d[0]; L.sa = sz; L.ha = V.Z - 1; L.ca = kz;
+    L.bb = (double)y; L.db = d[1]; L.sb = sy; L.hb = V.Y - 1; L.cb = ky;
+    sp = 1; c = x; ext = V.X; L.cp = kx;
   }
+  L.sp = sp;
   L.p0 = c;
   L.pbase = c * sp;
   if (meta & kMetaToMax) {
@@ -455,7 +505,7 @@ struct TileAcc {
 
 // Trace the voxels [v_begin, v_end) of reflection `rule` with this lane's
 // stride; accumulate exp(-sum mu*L) (or write path records).
-template <int CAP, int U, bool GUARD, bool FULL, bool WIDE, bool F64, bool PATHS>
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS>
 __device__ __forceinline__ void trace_tile(const SampleView& S, const ReflRule* __restrict__ rule,
                                            int v_begin, int v_end, int is_exp, lds_u32* seg,
                                            TileAcc& acc, PathRecord* __restrict__ paths) {
@@ -479,11 +529,12 @@ __device__ __forceinline__ void trace_tile(const SampleView& S, const ReflRule* 
       lasts[k][0] = lasts[k][1] = lasts[k][2] = 0;
       if (verr) continue;
       RayLane L;
-      if (!make_lane(&rule->ray[k], faces[k], co.x, co.y, co.z, S.V, L)) {
+      if (!make_lane<LAYOUT>(&rule->ray[k], faces[k], co.x, co.y, co.z, S.V, L)) {
         verr = ANACOR_RAY_ERR_ASSERT;
         continue;
       }
-      walk_ray<CAP, U, GUARD, FULL, WIDE>(L, S.V, S.vs, seg, lens[k], nsegs[k], lasts[k], Lr[k]);
+      walk_ray<CAP, U, GUARD, FULL, LAYOUT>(L, S.V, S.vs, seg, lens[k], nsegs[k], lasts[k],
+                                            Lr[k]);
       acc.steps += (unsigned long long)lens[k];
     }
     acc.err |= verr;
@@ -568,14 +619,17 @@ __device__ __forceinline__ void reduce_tile(TileAcc& acc, ReduceLds& red, unsign
   }
 }
 
-__device__ __forceinline__ Volume make_volume(const uint8_t* vol, int Z, int Y, int X) {
+// nbytes: size of the buffer behind vol (the raw volume, or the packed one).
+__device__ __forceinline__ Volume make_volume(const uint8_t* vol, int Z, int Y, int X, int nbytes,
+                                              int cz, int cy) {
   Volume V;
-  V.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)vol, (short)0, Z * Y * X, 0x00020000);
+  V.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)vol, (short)0, nbytes, 0x00020000);
   V.Z = Z; V.Y = Y; V.X = X;
+  V.cz = cz; V.cy = cy;
   return V;
 }
 
-template <int CAP, int U, bool GUARD, bool FULL, bool WIDE, bool F64, bool PATHS>
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS>
 __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
   __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
   __shared__ ReduceLds red;
@@ -586,7 +640,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
   const int tile = lb - pos * a.tiles;
   const int refl = a.order[pos];         // the caller's reflection index
   SampleView S;
-  S.V = make_volume(a.vol, a.Z, a.Y, a.X);
+  S.V = make_volume(a.vol, a.Z, a.Y, a.X, a.vol_bytes, a.cz, a.cy);
   S.coords = a.coords;
   S.N = a.N;
   S.rules = a.rules;
@@ -597,7 +651,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
   TileAcc acc = {0.f, 0.f, 0.0, 0ull, 0u};
   const int v_begin = tile * a.vox_per_tile;
   const int v_end = min(a.N, v_begin + a.vox_per_tile);
-  trace_tile<CAP, U, GUARD, FULL, WIDE, F64, PATHS>(
+  trace_tile<CAP, U, GUARD, FULL, LAYOUT, F64, PATHS>(
       S, a.rules + refl, v_begin, v_end, a.is_exp, seg, acc,
       PATHS ? a.paths + (size_t)refl * a.N : nullptr);
   reduce_tile<F64, PATHS>(acc, red, &a.err[refl], a.steps, a.partial32 + lb, a.partial64 + lb);
@@ -690,7 +744,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(BatchArgs a) {
     const int tile = local - pos * B.tiles;
     const int refl = B.order[pos];
     SampleView S;
-    S.V = make_volume(B.vol, B.Z, B.Y, B.X);
+    S.V = make_volume(B.vol, B.Z, B.Y, B.X, B.Z * B.Y * B.X, 0, 0);
     S.coords = B.coords;
     S.N = B.N;
     S.rules = B.rules;
@@ -699,7 +753,8 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(BatchArgs a) {
     TileAcc acc = {0.f, 0.f, 0.0, 0ull, 0u};
     const int v_begin = tile * B.vpt;
     const int v_end = min(B.N, v_begin + B.vpt);
-    trace_tile<CAP, U, GUARD, FULL, WIDE, F64, false>(S, B.rules + refl, v_begin, v_end, a.is_exp,
+    trace_tile<CAP, U, GUARD, FULL, WIDE ? kLayoutWide : kLayoutRaw, F64, false>(
+        S, B.rules + refl, v_begin, v_end, a.is_exp,
                                                       seg, acc, nullptr);
     reduce_tile<F64, false>(acc, red, &B.err[refl], a.steps, B.p32 + local, B.p64 + local);
   }
@@ -743,6 +798,42 @@ __global__ void batch_finish_kernel(const BatchSample* __restrict__ samples, int
   B.out[refl] = B.err[refl] ? __builtin_nan("") : sum / (double)B.N;
 }
 
+// ------------------------------------------------------- volume packing
+
+// flags |= 1 when some label is outside 0..3 (2-bit bricks impossible).
+__global__ void label_scan_kernel(const uint8_t* __restrict__ vol, long long n,
+                                  unsigned* __restrict__ flags) {
+  bool big = false;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    big = big || vol[i] > 3;
+  if (__any(big) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flags, 1u);
+}
+
+// One thread per dword of the bricked volume (16 2-bit or 8 4-bit labels);
+// voxels of the padding beyond the volume are 0.
+__global__ void pack_volume_kernel(const uint8_t* __restrict__ vol, int Z, int Y, int X, int NBX,
+                                   int NBY, int bits, long long ndw, uint32_t* __restrict__ out) {
+  const long long d = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndw) return;
+  const int per = 32 / bits;
+  uint32_t word = 0;
+  for (int j = 0; j < per; ++j) {
+    const long long f = d * per + j;
+    const long long b = f >> 9;
+    const int w = (int)(f & 511);
+    const int bx = (int)(b % NBX);
+    const long long t = b / NBX;
+    const int by = (int)(t % NBY), bz = (int)(t / NBY);
+    const int z = bz * 8 + (w >> 6), y = by * 8 + ((w >> 3) & 7), x = bx * 8 + (w & 7);
+    uint32_t lab = 0;
+    if (z < Z && y < Y && x < X) lab = vol[((size_t)z * Y + y) * X + x];
+    const uint32_t code = bits == 2 ? lab : (lab <= 4u ? lab : 5u);
+    word |= code << (j * bits);
+  }
+  out[d] = word;
+}
+
 // ------------------------------------------------------------ host side
 
 // Grow-only page-locked host buffer (hipHostMalloc) for uploads that should
@@ -772,6 +863,13 @@ struct Context {
   hipStream_t stream = nullptr;
   uint8_t* d_vol = nullptr;
   int Z = 0, Y = 0, X = 0;
+  int layout_req = ANACOR_LAYOUT_AUTO;   // requested layout of the trace kernel
+  int layout = kLayoutRaw;               // layout in use (kLayout*)
+  uint8_t* d_pack = nullptr;             // bricked copy of the volume
+  int pack_bits = 0;                     // 2 or 4 once built
+  long long pack_bytes = 0;
+  int NBX = 0, NBY = 0, NBZ = 0;
+  int labels_over3 = -1;                 // some label outside 0..3 (-1: not scanned)
   int4* d_coords = nullptr;
   int N = 0;
   double vs[3] = {1, 1, 1};
@@ -856,20 +954,26 @@ static int choose_tile(const Context* c, int64_t R) {
 constexpr int kCap = 16;     // boundaries kept per lane before the re-walk fallback
 constexpr int kUnroll = 4;   // label gathers in flight per lane (2, 6, 8 measured slower)
 
-template <bool GUARD, bool FULL, bool WIDE, bool F64, bool PATHS>
+template <bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS>
 static void launch_trace(const TraceArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((trace_kernel<kCap, kUnroll, GUARD, FULL, WIDE, F64, PATHS>), dim3(grid),
+  hipLaunchKernelGGL((trace_kernel<kCap, kUnroll, GUARD, FULL, LAYOUT, F64, PATHS>), dim3(grid),
                      dim3(kBlock), 0, s, a);
 }
 
 using LaunchFn = void (*)(const TraceArgs&, int, hipStream_t);
 using LaunchBatchFn = void (*)(const BatchArgs&, int, hipStream_t);
 
-
-
+// Index bits: guard 1, full 2, f64 4, paths 8, layout 16 * (0..3).  Path
+// records are written from the raw layouts only.
 template <int I>
 static constexpr LaunchFn launcher() {
-  return &launch_trace<(I & 1) != 0, (I & 2) != 0, (I & 4) != 0, (I & 8) != 0, (I & 16) != 0>;
+  constexpr int layout = (I >> 4) & 3;
+  constexpr bool paths = (I & 8) != 0;
+  if constexpr (paths && layout >= kLayoutBrick2) {
+    return nullptr;
+  } else {
+    return &launch_trace<(I & 1) != 0, (I & 2) != 0, layout, (I & 4) != 0, paths>;
+  }
 }
 
 template <int... I>
@@ -877,13 +981,91 @@ static constexpr std::array<LaunchFn, sizeof...(I)> launch_table(std::integer_se
   return {launcher<I>()...};
 }
 
-static const std::array<LaunchFn, 32> kLaunch = launch_table(std::make_integer_sequence<int, 32>{});
+static const std::array<LaunchFn, 64> kLaunch = launch_table(std::make_integer_sequence<int, 64>{});
 
-static void dispatch_trace(const TraceArgs& a, int grid, bool guard, bool full, bool wide,
+static void dispatch_trace(const TraceArgs& a, int grid, bool guard, bool full, int layout,
                            bool f64, bool paths, hipStream_t s) {
-  const int i = (guard ? 1 : 0) | (full ? 2 : 0) | (wide ? 4 : 0) | (f64 ? 8 : 0) |
-                (paths ? 16 : 0);
+  const int i = (guard ? 1 : 0) | (full ? 2 : 0) | (f64 ? 4 : 0) | (paths ? 8 : 0) |
+                (layout << 4);
   kLaunch[i](a, grid, s);
+}
+
+// 24-bit address arithmetic unless a z-plane or the volume is too large for it.
+static bool raw_needs_wide(const Context* c) {
+  return (int64_t)c->Y * c->X >= (1 << 23) || c->Z >= (1 << 23);
+}
+
+// The bricked layouts need every factor of the address chain in 24 bits and
+// every partial sum in 31: brick constants below 2^23, padded volume below 2^30.
+static bool bricks_fit(const Context* c) {
+  const int64_t nbx = (c->X + 7) / 8, nby = (c->Y + 7) / 8, nbz = (c->Z + 7) / 8;
+  return 512 * nbx * nby < (1 << 23) && 512 * nbx * nby * nbz < (1 << 30) &&
+         c->X < (1 << 23) && c->Y < (1 << 23) && c->Z < (1 << 23);
+}
+
+// Choose (and if needed build) the volume layout of the trace kernel.
+// Automatic: bricks for volumes above 64 MB, which exceed the L2s and most of
+// the Infinity Cache in the raw layout; raw otherwise, where the kernel is
+// bound by VALU issue and the brick address arithmetic would only add to it.
+static int ensure_layout(Context* c) {
+  int want = c->layout_req;
+  if (want == ANACOR_LAYOUT_AUTO) {
+    const int64_t nvox = (int64_t)c->Z * c->Y * c->X;
+    const char* env = std::getenv("ANACOR_VOLUME_LAYOUT");
+    if (env && std::strcmp(env, "raw") == 0) want = ANACOR_LAYOUT_RAW;
+    else if (env && std::strcmp(env, "brick") == 0) want = ANACOR_LAYOUT_BRICK;
+    else want = nvox > (64ll << 20) ? ANACOR_LAYOUT_BRICK : ANACOR_LAYOUT_RAW;
+    if (want != ANACOR_LAYOUT_RAW && !bricks_fit(c)) want = ANACOR_LAYOUT_RAW;
+  }
+  if (want == ANACOR_LAYOUT_RAW) {
+    c->layout = raw_needs_wide(c) ? kLayoutWide : kLayoutRaw;
+    return ANACOR_OK;
+  }
+  if (!bricks_fit(c)) {
+    set_error(ANACOR_ERR_ARG, "volume %dx%dx%d too large for the bricked layout", c->Z, c->Y,
+              c->X);
+    return ANACOR_ERR_ARG;
+  }
+  const long long nvox = (long long)c->Z * c->Y * c->X;
+  if (c->labels_over3 < 0) {
+    unsigned* d_flag = nullptr;
+    unsigned h_flag = 0;
+    HIP_TRY(hipMalloc(&d_flag, sizeof(unsigned)));
+    HIP_TRY(hipMemsetAsync(d_flag, 0, sizeof(unsigned), c->stream));
+    hipLaunchKernelGGL(label_scan_kernel, dim3(2048), dim3(256), 0, c->stream, c->d_vol, nvox,
+                       d_flag);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(&h_flag, d_flag, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(d_flag));
+    c->labels_over3 = (h_flag & 1u) ? 1 : 0;
+  }
+  int bits = (want == ANACOR_LAYOUT_BRICK4 || c->labels_over3) ? 4 : 2;
+  if (want == ANACOR_LAYOUT_BRICK2 && c->labels_over3) {
+    set_error(ANACOR_ERR_ARG, "2-bit bricks need labels in 0..3");
+    return ANACOR_ERR_ARG;
+  }
+  if (c->pack_bits != bits) {
+    c->NBX = (c->X + 7) / 8;
+    c->NBY = (c->Y + 7) / 8;
+    c->NBZ = (c->Z + 7) / 8;
+    const long long nb = (long long)c->NBX * c->NBY * c->NBZ;
+    const long long bytes = nb * 512 * bits / 8;
+    if (c->d_pack) HIP_TRY(hipFree(c->d_pack));
+    c->d_pack = nullptr;
+    c->pack_bits = 0;
+    HIP_TRY(hipMalloc(&c->d_pack, (size_t)bytes));
+    const long long ndw = bytes / 4;
+    hipLaunchKernelGGL(pack_volume_kernel, dim3((unsigned)((ndw + 255) / 256)), dim3(256), 0,
+                       c->stream, c->d_vol, c->Z, c->Y, c->X, c->NBX, c->NBY, bits, ndw,
+                       (uint32_t*)c->d_pack);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->pack_bits = bits;
+    c->pack_bytes = bytes;
+  }
+  c->layout = bits == 2 ? kLayoutBrick2 : kLayoutBrick4;
+  return ANACOR_OK;
 }
 
 // Processing order of the reflections: sorted by the directions of their two
@@ -950,8 +1132,21 @@ static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flag
   HIP_TRY(hipMemsetAsync(c->d_err, 0, (size_t)R * sizeof(unsigned), c->stream));
   HIP_TRY(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
 
+  int layout = raw_needs_wide(c) ? kLayoutWide : kLayoutRaw;   // path records: raw layout
+  if (!paths) {
+    rc = ensure_layout(c);
+    if (rc) {
+      if (d_paths) (void)hipFree(d_paths);
+      return rc;
+    }
+    layout = c->layout;
+  }
   TraceArgs a;
-  a.vol = c->d_vol;
+  const bool bricked = layout == kLayoutBrick2 || layout == kLayoutBrick4;
+  a.vol = bricked ? c->d_pack : c->d_vol;
+  a.vol_bytes = bricked ? (int)c->pack_bytes : c->Z * c->Y * c->X;
+  a.cz = 512 * c->NBX * c->NBY - 512;
+  a.cy = 512 * c->NBX - 64;
   a.Z = c->Z; a.Y = c->Y; a.X = c->X;
   a.coords = c->d_coords;
   a.N = c->N;
@@ -972,9 +1167,7 @@ static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flag
   const int grid = (int)((nblk + 7) / 8 * 8);
 
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  // 24-bit address arithmetic unless a z-plane or the volume is too large for it
-  const bool wide = (int64_t)c->Y * c->X >= (1 << 23) || c->Z >= (1 << 23);
-  dispatch_trace(a, grid, guard, c->full != 0, wide, f64, paths, c->stream);
+  dispatch_trace(a, grid, guard, c->full != 0, layout, f64, paths, c->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   if (!paths) {
@@ -1131,6 +1324,10 @@ static int ctx_create(const uint8_t* vol, const int64_t shape[3], const int64_t*
     c->h_coords[(size_t)i] = make_int4((int)z, (int)y, (int)x, (int)i);
   }
   int blk[3] = {kDefaultBlock[0], kDefaultBlock[1], kDefaultBlock[2]};
+  // volumes that take the bricked layout by default: Morton-ordered waves,
+  // whose rays share bricks (measured on 1000x1000x800 with 100k sampled
+  // voxels: 164 ms against 306 ms for row-shaped waves)
+  if ((int64_t)nvox > (64ll << 20)) blk[0] = blk[1] = blk[2] = 0;
   if (const char* e = std::getenv("ANACOR_VOXEL_BLOCK"))
     if (std::sscanf(e, "%d,%d,%d", &blk[0], &blk[1], &blk[2]) != 3) blk[0] = blk[1] = blk[2] = 0;
   const int rc = upload_voxel_order(c, blk[0], blk[1], blk[2]);
@@ -1147,6 +1344,7 @@ static void ctx_destroy(Context* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->d_vol) (void)hipFree(c->d_vol);
+  if (c->d_pack) (void)hipFree(c->d_pack);
   if (c->d_coords) (void)hipFree(c->d_coords);
   if (c->d_rules) (void)hipFree(c->d_rules);
   if (c->d_partial) (void)hipFree(c->d_partial);
@@ -1463,6 +1661,27 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
 using namespace anacor;
 
 extern "C" {
+
+int anacor_ctx_set_layout(anacor_ctx* ctx, int layout) {
+  if (!ctx || !ctx->c || !(layout == ANACOR_LAYOUT_AUTO || layout == ANACOR_LAYOUT_RAW ||
+                           layout == ANACOR_LAYOUT_BRICK || layout == ANACOR_LAYOUT_BRICK2 ||
+                           layout == ANACOR_LAYOUT_BRICK4)) {
+    set_error(ANACOR_ERR_ARG, "bad context or layout %d", layout);
+    return ANACOR_ERR_ARG;
+  }
+  Context* c = ctx->c;
+  std::lock_guard<std::mutex> lk(c->mu_run);
+  HIP_TRY(hipSetDevice(c->device));
+  c->layout_req = layout;
+  return ensure_layout(c);
+}
+
+int anacor_ctx_get_layout(const anacor_ctx* ctx) {
+  if (!ctx || !ctx->c) return ANACOR_LAYOUT_RAW;
+  const int l = ctx->c->layout;
+  return l == kLayoutBrick2 ? ANACOR_LAYOUT_BRICK2
+                            : (l == kLayoutBrick4 ? ANACOR_LAYOUT_BRICK4 : ANACOR_LAYOUT_RAW);
+}
 
 int anacor_last_error(char* buf, size_t n) {
   if (buf && n) {

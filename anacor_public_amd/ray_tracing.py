@@ -199,6 +199,17 @@ class Tracer:
     def set_tile(self, voxels_per_tile: int):
         self._lib.anacor_ctx_set_tile(self.handle, int(voxels_per_tile))
 
+    def set_layout(self, layout: int):
+        """Volume layout of the trace kernel (_lib.LAYOUT_*): raw uint8, or 8x8x8
+        bricks of 2-bit / 4-bit labels.  Results are bit-identical in every
+        layout; bricks pay off when the volume outgrows the caches."""
+        _lib.check(self._lib.anacor_ctx_set_layout(self.handle, int(layout)), self._lib)
+
+    @property
+    def layout(self) -> int:
+        """Layout used by the most recent trace (_lib.LAYOUT_RAW, _BRICK2, _BRICK4)."""
+        return int(self._lib.anacor_ctx_get_layout(self.handle))
+
     def set_voxel_block(self, bz: int, by: int, bx: int):
         """Group crystal voxels into waves by (bz, by, bx) voxel blocks
         ((0, 0, 0): Morton order)."""

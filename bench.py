@@ -266,10 +266,20 @@ def main():
             return runner.run([w.s1 for w in wls], [w.omega for w in wls], wls[0].raw_xray,
                               wls[0].omega_axis, wls[0].F)
     else:
+        from anacor_public_amd import autotune
         from anacor_public_amd.ray_tracing import Tracer
         runner = Tracer.for_workload(wls[0])
         if args.tile:
             runner.set_tile(args.tile)
+        elif not args.no_autotune:
+            w = wls[0]
+            t1 = time.perf_counter()
+            pick = autotune.tune(runner, w.s1, w.omega, w.raw_xray, w.omega_axis, w.F, w.coords,
+                                 trial=128 if args.config == "large" else 512,
+                                 layouts=autotune.LAYOUTS, use_cache=False)
+            extra["autotune"] = {"layout": pick["layout"], "block": pick["block"],
+                                 "tile": pick["tile"]}
+            log(f"# autotune ({time.perf_counter() - t1:.1f}s): {extra['autotune']}")
 
         def call():
             return [runner.run_workload(wls[0])]
@@ -311,7 +321,9 @@ def main():
             f"max rel diff vs GPU {rel:.2e}")
 
     traffic = None
-    if name in TRAFFIC_BYTES_PER_STEP and not args.reflections:
+    # the calibrated traffic per step was measured with the raw layout
+    if (name in TRAFFIC_BYTES_PER_STEP and not args.reflections and args.config != "batched"
+            and runner.layout == 0):
         traffic = TRAFFIC_BYTES_PER_STEP[name] * vsteps
     pairs = sum(len(w.coords) * len(w.omega) for w in wls)
     rec = {
@@ -329,7 +341,9 @@ def main():
         "data": "synthetic (seeded four-material volumes and reflections)",
         "config": {"workload": name, "volumes": [list(w.volume.shape) for w in wls],
                    "n_voxels": N, "reflections": R, "path": "plain",
-                   "mean": "fp32 exp + compensated sum"},
+                   "mean": "fp32 exp + compensated sum",
+                   "volume_layout": ("raw" if args.config == "batched" else
+                                     {0: "raw", 2: "brick2", 4: "brick4"}[runner.layout])},
         "rays_per_s": 2.0 * pairs / (ms_per_step * 1e-3),
         "voxel_steps_per_s": vsteps / (ms_per_step * 1e-3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

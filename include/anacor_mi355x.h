@@ -105,6 +105,21 @@ void anacor_ctx_destroy(anacor_ctx *ctx);
 /* Voxels per workgroup tile (0 = automatic). */
 int anacor_ctx_set_tile(anacor_ctx *ctx, int voxels_per_tile);
 
+/* Volume layout of the trace kernel.  Never changes a result: every layout
+ * yields the same labels along every ray (bit-identical results).  Bricks are
+ * 8x8x8 voxels; 4-bit bricks store labels 0..4 as they are and any other value
+ * as 5, which the boundary rule cannot tell apart (only 0, "changed" and
+ * 0..4 matter to it).  The packed copy is built on the GPU on first use; the
+ * raw volume stays resident (path records and the batched path use it). */
+#define ANACOR_LAYOUT_AUTO   (-1)  /* bricks above 64 MB; ANACOR_VOLUME_LAYOUT=raw|brick overrides */
+#define ANACOR_LAYOUT_RAW    0     /* uint8 [z][y][x] */
+#define ANACOR_LAYOUT_BRICK  1     /* 2-bit bricks if every label is 0..3, else 4-bit */
+#define ANACOR_LAYOUT_BRICK2 2     /* 2-bit bricks (error unless every label is 0..3) */
+#define ANACOR_LAYOUT_BRICK4 4     /* 4-bit bricks */
+int anacor_ctx_set_layout(anacor_ctx *ctx, int layout);
+/* Layout used by the most recent trace: ANACOR_LAYOUT_RAW, _BRICK2 or _BRICK4. */
+int anacor_ctx_get_layout(const anacor_ctx *ctx);
+
 /* Grouping of the crystal voxels into waves: blocks of bz x by x bx voxels in
  * z, y, x order, lanes along x inside a block; (0, 0, 0) selects Morton
  * order.  Affects speed only, never results beyond the FP32 summation order. */
