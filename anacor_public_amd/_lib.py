@@ -42,6 +42,8 @@ EXPORTED = (
     "anacor_mlp_set_params", "anacor_mlp_set_scale", "anacor_mlp_set_features",
     "anacor_mlp_set_envelope", "anacor_mlp_features", "anacor_mlp_forward",
     "anacor_mlp_gradients", "anacor_mlp_train", "anacor_mlp_predict", "anacor_mlp_stats",
+    "ray_tracing_single_gridding", "nearest_neighbor_interpolate_overall", "anacor_grid_create",
+    "anacor_grid_destroy", "anacor_grid_map", "anacor_grid_run", "anacor_grid_stats",
 )
 
 MLP_INPUTS = 12
@@ -153,6 +155,23 @@ def _prototypes(lib):
     lib.anacor_mlp_predict.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, vp]
     lib.anacor_mlp_stats.restype = ct.c_int
     lib.anacor_mlp_stats.argtypes = [vp, ct.POINTER(MlpStats)]
+    lib.anacor_grid_create.restype = vp
+    lib.anacor_grid_create.argtypes = [vp, i64, i64, vp, vp, i32]
+    lib.anacor_grid_destroy.restype = None
+    lib.anacor_grid_destroy.argtypes = [vp]
+    lib.anacor_grid_map.restype = ct.c_int
+    lib.anacor_grid_map.argtypes = [vp, vp]
+    lib.anacor_grid_run.restype = ct.c_int
+    lib.anacor_grid_run.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, i64, vp, i64, i32, vp]
+    lib.anacor_grid_stats.restype = ct.c_int
+    lib.anacor_grid_stats.argtypes = [vp, ct.POINTER(dbl), ct.POINTER(dbl)]
+    # gridding drop-ins (utils/utils_gridding.py:104-118, :567-591)
+    lib.ray_tracing_single_gridding.restype = dbl
+    lib.ray_tracing_single_gridding.argtypes = [vp, vp, dbl, dbl, vp, vp, vp, vp, i32, i32, i32]
+    lib.nearest_neighbor_interpolate_overall.restype = ct.POINTER(dbl)
+    lib.nearest_neighbor_interpolate_overall.argtypes = [
+        i64, i64, vp, i64, vp, vp, vp, vp, vp, i64, vp, vp, i32, i32, vp, vp, vp, ct.c_size_t,
+        ct.c_size_t, dbl, dbl, dbl, dbl, i32]
     # drop-in entry points, with the argtypes the reference's binding uses
     # (utils/utils_mp.py:59-79, :82-94, :128-148)
     lib.ray_tracing_overall.restype = ct.POINTER(dbl)

@@ -240,6 +240,22 @@ void host_setup(const double* s1, const double* omega, int64_t R, const double* 
   for (auto& th : pool) th.join();
 }
 
+void host_setup_directions(const double* rotated_s1, const double* theta_phi, int64_t D,
+                           int64_t max_dim, RayRule* rays) {
+  for (int64_t i = 0; i < D; ++i) {
+    RayRule& r = rays[i];
+    std::memset(&r, 0, sizeof(r));
+    double d3[3];
+    to_myframe(rotated_s1 + 3 * i, d3);
+    r.dir[0] = d3[0];
+    r.dir[1] = d3[1];
+    r.dir[2] = d3[2];
+    face_rules(theta_phi[2 * i], theta_phi[2 * i + 1], r);
+    r.neg = 0u;
+    r.guard = needs_guard(r, max_dim) ? 1u : 0u;
+  }
+}
+
 void host_setup_rotated(const double rotated_s1[3], const double xray[3], int64_t max_dim,
                         ReflRule* rule, ReflGeom* geom) {
   ReflGeom g;

@@ -46,6 +46,13 @@ void host_setup(const double* s1, const double* omega, int64_t R,
                 const double* raw_xray, const double* omega_axis, const double* F,
                 int64_t max_dim, ReflRule* rules, ReflGeom* geom, int n_threads);
 
+// One ray per direction, as ray_tracing_single_gridding walks it
+// (ray_tracing.c:484-534): the exit-face direction from rotated_s1[D][3], the
+// stepping rules from the given angles theta_phi[D][2] (the diffracted-ray
+// convention, not negated).
+void host_setup_directions(const double* rotated_s1, const double* theta_phi, int64_t D,
+                           int64_t max_dim, RayRule* rays);
+
 // Same for already-rotated vectors (ray_tracing_single).
 void host_setup_rotated(const double rotated_s1[3], const double xray[3], int64_t max_dim,
                         ReflRule* rule, ReflGeom* geom);

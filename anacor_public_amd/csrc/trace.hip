@@ -49,6 +49,14 @@ struct anacor_ctx {
 };
 
 namespace anacor {
+struct Grid;
+}
+
+struct anacor_grid {
+  anacor::Grid* g;
+};
+
+namespace anacor {
 struct Batch;
 }
 
@@ -796,6 +804,184 @@ __global__ void batch_finish_kernel(const BatchSample* __restrict__ samples, int
   }
   const int refl = B.order[r];
   B.out[refl] = B.err[refl] ? __builtin_nan("") : sum / (double)B.N;
+}
+
+// ------------------------------------------------------------- gridding
+//
+// The reference's approximate path (utils/utils_gridding.py,
+// src/gridding_interpolation.c): an absorption map holding, for every grid
+// direction and crystal voxel, the sum of mu*L of one ray from the voxel
+// along that direction (ray_tracing_single_gridding, ray_tracing.c:484-534,
+// IsExp = 0, stored as float), then per reflection a look-up of the incident
+// and the diffracted direction and the mean of exp(-(m1 + m2)).
+
+struct GridArgs {
+  const uint8_t* vol;
+  int vol_bytes, cz, cy;
+  int Z, Y, X;
+  const int4* coords;          // (z, y, x, caller index)
+  int N;
+  const RayRule* rays;         // [D] one ray per direction
+  int D, tiles, vox_per_tile, n_blocks;
+  double vs[3];
+  double mu[4];
+  int is_exp;
+  float* map;                  // [D][N], voxels in the caller's order
+  double* map64;               // same, unrounded (nullable)
+  unsigned* err;               // [D]
+};
+
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT>
+__global__ __launch_bounds__(kBlock) void grid_map_kernel(GridArgs a) {
+  __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
+  const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
+  if (lb >= a.n_blocks) return;
+  const int d = lb / a.tiles;
+  const int tile = lb - d * a.tiles;
+  const Volume V = make_volume(a.vol, a.Z, a.Y, a.X, a.vol_bytes, a.cz, a.cy);
+  lds_u32* seg = (lds_u32*)(seg_lds + threadIdx.x);
+  const RayRule* rr = a.rays + d;
+  unsigned err = 0;
+  const int v_end = min(a.N, (tile + 1) * a.vox_per_tile);
+  for (int v = tile * a.vox_per_tile + (int)threadIdx.x; v < v_end; v += kBlock) {
+    const int4 co = a.coords[v];
+    double val = __builtin_nan("");
+    const int face = exit_face(co.x, co.y, co.z, rr->dir[0], rr->dir[1], rr->dir[2], a.Z, a.Y,
+                               a.X, false);
+    RayLane L;
+    if (face == 0) {
+      err |= ANACOR_RAY_ERR_FACE;
+    } else if (!make_lane<LAYOUT>(rr, face, co.x, co.y, co.z, V, L)) {
+      err |= ANACOR_RAY_ERR_ASSERT;
+    } else {
+      double Lr[4];
+      int len, nseg, last[3];
+      walk_ray<CAP, U, GUARD, FULL, LAYOUT>(L, V, a.vs, seg, len, nseg, last, Lr);
+      // cal_rate with zero incident lengths, same association (:1554-1583)
+      const double r = a.mu[0] * (0.0 + Lr[0]) + a.mu[1] * (0.0 + Lr[1]) +
+                       a.mu[2] * (0.0 + Lr[2]) + a.mu[3] * (0.0 + Lr[3]);
+      val = a.is_exp ? exp(-r) : r;
+    }
+    a.map[(size_t)d * a.N + co.w] = (float)val;
+    if (a.map64) a.map64[(size_t)d * a.N + co.w] = val;
+  }
+  if (err) atomicOr(&a.err[d], err);
+}
+
+// Look-up geometry of one reflection on the padded grid the reference builds
+// (utils_gridding.py:406-445: theta_extra columns and phi_extra rows copied
+// from the opposite ends, theta_list / phi_list over the padded extent).
+struct GridLookup {
+  const float* map;            // resident: [n_phi][n_theta][N] (the map above)
+  const double* map64;         // drop-in: [N][ny][nx], padded (reference layout)
+  int N, n_theta, n_phi, theta_extra, phi_extra;
+  const double* theta_list;    // [nx]
+  const double* phi_list;      // [ny]
+  int nx, ny;
+  double theta_min, theta_max, phi_min, phi_max;
+  const double* angles;        // [R][4]: theta_1, phi_1 (incident), theta, phi (diffracted)
+  const int* order;            // [R] processing position -> caller's index
+  int R;
+  double* out;                 // [R]
+  unsigned* err;               // [R]
+};
+
+// nearest_neighbor_custom (gridding_interpolation.c:16-32): truncated index
+// on the padded grid, clamped.
+__device__ __forceinline__ void nearest_index(const GridLookup& g, double theta, double phi,
+                                              int& it, int& ip) {
+  const double theta_step = (g.theta_max - g.theta_min) / (g.nx - 1);
+  const double phi_step = (g.phi_max - g.phi_min) / (g.ny - 1);
+  it = (int)((theta - g.theta_min) / theta_step);
+  ip = (int)((phi - g.phi_min) / phi_step);
+  it = it < 0 ? 0 : (it >= g.nx ? g.nx - 1 : it);
+  ip = ip < 0 ? 0 : (ip >= g.ny ? g.ny - 1 : ip);
+}
+
+// The interval index of GSL's bilinear interp2d (binary search of gsl_interp,
+// x_array[i] <= x < x_array[i+1]); false outside the grid, where GSL reports
+// a domain error.
+__device__ __forceinline__ bool interval(const double* xs, int n, double x, int& i) {
+  if (!(x >= xs[0] && x <= xs[n - 1])) return false;
+  int lo = 0, hi = n - 1;
+  while (hi > lo + 1) {
+    const int m = (hi + lo) / 2;
+    if (xs[m] > x) hi = m;
+    else lo = m;
+  }
+  i = lo;
+  return true;
+}
+
+template <bool RESIDENT>
+__device__ __forceinline__ double grid_value(const GridLookup& g, int v, int it, int ip) {
+  if (RESIDENT) {
+    int ut = (it - g.theta_extra) % g.n_theta;
+    int up = (ip - g.phi_extra) % g.n_phi;
+    ut += ut < 0 ? g.n_theta : 0;
+    up += up < 0 ? g.n_phi : 0;
+    return (double)g.map[((size_t)up * g.n_theta + ut) * g.N + v];
+  }
+  return g.map64[((size_t)v * g.ny + ip) * g.nx + it];
+}
+
+// One workgroup per reflection; METHOD 1 nearest, 2 bilinear.
+template <int METHOD, bool RESIDENT>
+__global__ __launch_bounds__(kBlock) void grid_lookup_kernel(GridLookup g) {
+  __shared__ double part[kBlock / kWave];
+  const int pos = (int)blockIdx.x;
+  if (pos >= g.R) return;
+  const int r = g.order ? g.order[pos] : pos;
+  const double* ang = g.angles + 4 * (size_t)r;
+  int it[2], ip[2];
+  double wt[2][4];
+  bool ok = true;
+  for (int k = 0; k < 2; ++k) {
+    const double th = ang[2 * k], ph = ang[2 * k + 1];
+    if (METHOD == 1) {
+      nearest_index(g, th, ph, it[k], ip[k]);
+    } else {
+      int xi = 0, yi = 0;
+      ok = ok && interval(g.theta_list, g.nx, th, xi) && interval(g.phi_list, g.ny, ph, yi);
+      if (!ok) { it[k] = ip[k] = 0; continue; }
+      it[k] = xi;
+      ip[k] = yi;
+      const double xmin = g.theta_list[xi], xmax = g.theta_list[xi + 1];
+      const double ymin = g.phi_list[yi], ymax = g.phi_list[yi + 1];
+      const double t = (th - xmin) / (xmax - xmin), u = (ph - ymin) / (ymax - ymin);
+      wt[k][0] = (1. - t) * (1. - u);   // zminmin
+      wt[k][1] = t * (1. - u);          // zmaxmin
+      wt[k][2] = (1. - t) * u;          // zminmax
+      wt[k][3] = t * u;                 // zmaxmax
+    }
+  }
+  double sum = 0.0;
+  if (ok) {
+    for (int v = (int)threadIdx.x; v < g.N; v += kBlock) {
+      double m[2];
+      for (int k = 0; k < 2; ++k) {
+        if (METHOD == 1) {
+          m[k] = grid_value<RESIDENT>(g, v, it[k], ip[k]);
+        } else {
+          const double zmm = grid_value<RESIDENT>(g, v, it[k], ip[k]);
+          const double zMm = grid_value<RESIDENT>(g, v, it[k] + 1, ip[k]);
+          const double zmM = grid_value<RESIDENT>(g, v, it[k], ip[k] + 1);
+          const double zMM = grid_value<RESIDENT>(g, v, it[k] + 1, ip[k] + 1);
+          m[k] = wt[k][0] * zmm + wt[k][1] * zMm + wt[k][2] * zmM + wt[k][3] * zMM;
+        }
+      }
+      sum += exp(-(m[0] + m[1]));
+    }
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+  if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / kWave; ++w) s += part[w];
+    g.out[r] = ok ? s / (double)g.N : __builtin_nan("");
+    if (!ok) g.err[r] = ANACOR_RAY_ERR_FACE;
+  }
 }
 
 // ------------------------------------------------------- volume packing
@@ -1662,6 +1848,269 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
 
 }  // namespace anacor
 
+namespace anacor {
+
+// ------------------------------------------------------- gridding (host)
+
+template <bool GUARD, bool FULL, int LAYOUT>
+static void launch_grid_map(const GridArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((grid_map_kernel<kCap, unroll_for(LAYOUT), GUARD, FULL, LAYOUT>),
+                     dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+using LaunchGridFn = void (*)(const GridArgs&, int, hipStream_t);
+
+template <int I>
+static constexpr LaunchGridFn grid_launcher() {
+  return &launch_grid_map<(I & 1) != 0, (I & 2) != 0, (I >> 2) & 3>;
+}
+
+template <int... I>
+static constexpr std::array<LaunchGridFn, sizeof...(I)> grid_table(
+    std::integer_sequence<int, I...>) {
+  return {grid_launcher<I>()...};
+}
+
+static const std::array<LaunchGridFn, 16> kLaunchGrid =
+    grid_table(std::make_integer_sequence<int, 16>{});
+
+struct Grid {
+  Context* c = nullptr;        // borrowed
+  int n_theta = 0, n_phi = 0, D = 0;
+  float* d_map = nullptr;      // [D][N]
+  double* d_angles = nullptr;  // lookup buffers
+  double* d_out = nullptr;
+  unsigned* d_err = nullptr;
+  int* d_order = nullptr;
+  size_t cap = 0;
+  double* d_lists = nullptr;
+  size_t lists_cap = 0;
+  float build_ms = 0.f, last_ms = 0.f;
+  std::vector<ReflRule> h_rules;
+  std::vector<ReflGeom> h_geom;
+  std::vector<double> h_angles;
+};
+
+static void grid_free(Grid* g) {
+  void* bufs[] = {g->d_map, g->d_angles, g->d_out, g->d_err, g->d_order, g->d_lists};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  delete g;
+}
+
+// Absorption maps of every crystal voxel of c for D directions.
+static int grid_build(Context* c, int64_t n_theta, int64_t n_phi, const double* rotated_s1,
+                      const double* theta_phi, int is_exp, Grid** out,
+                      double* d_map64 = nullptr) {
+  if (!c || n_theta < 1 || n_phi < 1 || !rotated_s1 || !theta_phi ||
+      n_theta * n_phi > (1 << 24)) {
+    set_error(ANACOR_ERR_ARG, "bad gridding arguments");
+    return ANACOR_ERR_ARG;
+  }
+  Grid* g = new Grid();
+  g->c = c;
+  g->n_theta = (int)n_theta;
+  g->n_phi = (int)n_phi;
+  g->D = (int)(n_theta * n_phi);
+  const int D = g->D, N = c->N;
+  std::vector<RayRule> rays((size_t)D);
+  host_setup_directions(rotated_s1, theta_phi, D, max_dim(c), rays.data());
+  bool guard = false;
+  for (const RayRule& r : rays) guard = guard || r.guard;
+  int rc = ensure_layout(c);
+  if (rc) { grid_free(g); return rc; }
+  RayRule* d_rays = nullptr;
+  unsigned* d_err = nullptr;
+  auto fail = [&](hipError_t e, const char* what) {
+    set_error(ANACOR_ERR_HIP, "gridding: %s failed: %s", what, hipGetErrorString(e));
+    if (d_rays) (void)hipFree(d_rays);
+    if (d_err) (void)hipFree(d_err);
+    grid_free(g);
+    return ANACOR_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&g->d_map, (size_t)D * std::max(N, 1) * sizeof(float))) != hipSuccess)
+    return fail(e, "hipMalloc(map)");
+  if ((e = hipMalloc(&d_rays, (size_t)D * sizeof(RayRule))) != hipSuccess)
+    return fail(e, "hipMalloc");
+  if ((e = hipMalloc(&d_err, (size_t)D * sizeof(unsigned))) != hipSuccess)
+    return fail(e, "hipMalloc");
+  if ((e = hipMemcpyAsync(d_rays, rays.data(), (size_t)D * sizeof(RayRule),
+                          hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+    return fail(e, "upload");
+  if ((e = hipMemsetAsync(d_err, 0, (size_t)D * sizeof(unsigned), c->stream)) != hipSuccess)
+    return fail(e, "hipMemsetAsync");
+  std::vector<unsigned> h_err((size_t)D, 0u);
+  if (N > 0) {
+    const int vpt = choose_tile(c, D);
+    const int64_t tiles = (N + vpt - 1) / vpt;
+    const int64_t nblk = (int64_t)D * tiles;
+    if (nblk > (int64_t)0x7fffffff - 8) {
+      (void)hipFree(d_rays);
+      (void)hipFree(d_err);
+      grid_free(g);
+      set_error(ANACOR_ERR_ARG, "gridding: too many workgroups");
+      return ANACOR_ERR_ARG;
+    }
+    GridArgs a;
+    const bool bricked = c->layout == kLayoutBrick2 || c->layout == kLayoutBrick4;
+    a.vol = bricked ? c->d_pack : c->d_vol;
+    a.vol_bytes = bricked ? (int)c->pack_bytes : c->Z * c->Y * c->X;
+    a.cz = 512 * c->NBX * c->NBY - 512;
+    a.cy = 512 * c->NBX - 64;
+    a.Z = c->Z; a.Y = c->Y; a.X = c->X;
+    a.coords = c->d_coords;
+    a.N = N;
+    a.rays = d_rays;
+    a.D = D;
+    a.tiles = (int)tiles;
+    a.vox_per_tile = vpt;
+    a.n_blocks = (int)nblk;
+    for (int i = 0; i < 3; ++i) a.vs[i] = c->vs[i];
+    for (int i = 0; i < 4; ++i) a.mu[i] = c->mu[i];
+    a.is_exp = is_exp;
+    a.map = g->d_map;
+    a.map64 = d_map64;
+    a.err = d_err;
+    if ((e = hipEventRecord(c->ev0, c->stream)) != hipSuccess) return fail(e, "event");
+    const int i = (guard ? 1 : 0) | (c->full ? 2 : 0) | (c->layout << 2);
+    kLaunchGrid[i](a, (int)((nblk + 7) / 8 * 8), c->stream);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "grid_map_kernel");
+    if ((e = hipEventRecord(c->ev1, c->stream)) != hipSuccess) return fail(e, "event");
+  }
+  if ((e = hipMemcpyAsync(h_err.data(), d_err, (size_t)D * sizeof(unsigned),
+                          hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
+    return fail(e, "download");
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail(e, "gridding");
+  if (N > 0) (void)hipEventElapsedTime(&g->build_ms, c->ev0, c->ev1);
+  (void)hipFree(d_rays);
+  (void)hipFree(d_err);
+  *out = g;
+  for (int d = 0; d < D; ++d)
+    if (h_err[(size_t)d]) {
+      set_error(ANACOR_ERR_GEOMETRY, "gridding direction %d: the reference aborts here; its "
+                "map entries are NaN", d);
+      return ANACOR_ERR_GEOMETRY;
+    }
+  return ANACOR_OK;
+}
+
+struct GridLists {
+  const double* theta_list;
+  int64_t nx;
+  const double* phi_list;
+  int64_t ny;
+  double theta_min, theta_max, phi_min, phi_max;
+};
+
+// Per-reflection angles (theta_1, phi_1, theta, phi) as
+// nearest_neighbor_interpolate_overall computes them (gridding_interpolation.c:
+// 120-146: rotation, then dials_2_thetaphi_22 of both rays).
+static void reflection_angles(const double* s1, const double* omega, int64_t R,
+                              const double* raw, const double* axis, const double* F,
+                              int64_t max_dim_, std::vector<ReflRule>& rules,
+                              std::vector<ReflGeom>& geom, std::vector<double>& ang, int threads) {
+  rules.resize((size_t)R);
+  geom.resize((size_t)R);
+  ang.resize((size_t)R * 4);
+  host_setup(s1, omega, R, raw, axis, F, max_dim_, rules.data(), geom.data(), threads);
+  for (int64_t i = 0; i < R; ++i) {
+    ang[4 * i + 0] = geom[(size_t)i].theta[0];
+    ang[4 * i + 1] = geom[(size_t)i].phi[0];
+    ang[4 * i + 2] = geom[(size_t)i].theta[1];
+    ang[4 * i + 3] = geom[(size_t)i].phi[1];
+  }
+}
+
+// Look-ups of R reflections with the given angles on the device; map is the
+// resident map (float) or, when map64 is set, the reference's padded layout.
+static int grid_lookup(hipStream_t stream, const float* map, const double* map64, int N,
+                       int n_theta, int n_phi, const GridLists& L, const double* h_angles,
+                       int64_t R, int method, double* out, double** d_bufs, size_t* cap,
+                       double** d_lists, size_t* lists_cap, float* ms) {
+  if (method != 1 && method != 2) {
+    set_error(ANACOR_ERR_ARG, "interpolation method %d (1 nearest, 2 bilinear)", method);
+    return ANACOR_ERR_ARG;
+  }
+  if (L.nx < 2 || L.ny < 2 || !L.theta_list || !L.phi_list) {
+    set_error(ANACOR_ERR_ARG, "bad grid lists");
+    return ANACOR_ERR_ARG;
+  }
+  const size_t need = (size_t)R * (4 * sizeof(double) + sizeof(double) + sizeof(unsigned)) + 64;
+  if (need > *cap) {
+    if (*d_bufs) HIP_TRY(hipFree(*d_bufs));
+    *d_bufs = nullptr;
+    HIP_TRY(hipMalloc(d_bufs, need));
+    *cap = need;
+  }
+  const size_t nl = (size_t)(L.nx + L.ny);
+  if (nl > *lists_cap) {
+    if (*d_lists) HIP_TRY(hipFree(*d_lists));
+    *d_lists = nullptr;
+    HIP_TRY(hipMalloc(d_lists, nl * sizeof(double)));
+    *lists_cap = nl;
+  }
+  double* d_ang = *d_bufs;
+  double* d_out = d_ang + 4 * (size_t)R;
+  unsigned* d_err = (unsigned*)(d_out + R);
+  HIP_TRY(hipMemcpyAsync(d_ang, h_angles, (size_t)R * 4 * sizeof(double), hipMemcpyHostToDevice,
+                         stream));
+  HIP_TRY(hipMemcpyAsync(*d_lists, L.theta_list, (size_t)L.nx * sizeof(double),
+                         hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(*d_lists + L.nx, L.phi_list, (size_t)L.ny * sizeof(double),
+                         hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemsetAsync(d_err, 0, (size_t)R * sizeof(unsigned), stream));
+  GridLookup g;
+  g.map = map;
+  g.map64 = map64;
+  g.N = N;
+  g.n_theta = n_theta;
+  g.n_phi = n_phi;
+  g.theta_extra = (int)((L.nx - n_theta) / 2);
+  g.phi_extra = (int)((L.ny - n_phi) / 2);
+  g.theta_list = *d_lists;
+  g.phi_list = *d_lists + L.nx;
+  g.nx = (int)L.nx;
+  g.ny = (int)L.ny;
+  g.theta_min = L.theta_min;
+  g.theta_max = L.theta_max;
+  g.phi_min = L.phi_min;
+  g.phi_max = L.phi_max;
+  g.angles = d_ang;
+  g.order = nullptr;
+  g.R = (int)R;
+  g.out = d_out;
+  g.err = d_err;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, stream));
+  const bool res = map != nullptr;
+  if (method == 1 && res) hipLaunchKernelGGL((grid_lookup_kernel<1, true>), dim3((unsigned)R), dim3(kBlock), 0, stream, g);
+  if (method == 2 && res) hipLaunchKernelGGL((grid_lookup_kernel<2, true>), dim3((unsigned)R), dim3(kBlock), 0, stream, g);
+  if (method == 1 && !res) hipLaunchKernelGGL((grid_lookup_kernel<1, false>), dim3((unsigned)R), dim3(kBlock), 0, stream, g);
+  if (method == 2 && !res) hipLaunchKernelGGL((grid_lookup_kernel<2, false>), dim3((unsigned)R), dim3(kBlock), 0, stream, g);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(e1, stream));
+  std::vector<unsigned> h_err((size_t)R);
+  HIP_TRY(hipMemcpyAsync(out, d_out, (size_t)R * sizeof(double), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipMemcpyAsync(h_err.data(), d_err, (size_t)R * sizeof(unsigned),
+                         hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (ms) (void)hipEventElapsedTime(ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int64_t i = 0; i < R; ++i)
+    if (h_err[(size_t)i]) {
+      set_error(ANACOR_ERR_GEOMETRY, "reflection %lld: direction outside the grid (result NaN)",
+                (long long)i);
+      return ANACOR_ERR_GEOMETRY;
+    }
+  return ANACOR_OK;
+}
+
+}  // namespace anacor
+
 // ===================================================================== C-ABI
 
 using namespace anacor;
@@ -1970,6 +2419,195 @@ double ray_tracing_single_mp(int64_t* coord_list, int64_t len_coord_list,
   (void)store_paths; (void)num_workers;
   return single_impl(coord_list, len_coord_list, rotated_s1, xray, voxel_size, coefficients,
                      label_list, shape, full_iteration, IsExp);
+}
+
+
+// ------------------------------------------------------------- gridding
+
+anacor_grid* anacor_grid_create(anacor_ctx* ctx, int64_t n_theta, int64_t n_phi,
+                                const double* rotated_s1, const double* theta_phi, int is_exp) {
+  clear_error();
+  if (!ctx || !ctx->c) {
+    set_error(ANACOR_ERR_ARG, "null context");
+    return nullptr;
+  }
+  Context* c = ctx->c;
+  std::lock_guard<std::mutex> lk(c->mu_run);
+  if (hipSetDevice(c->device) != hipSuccess) {
+    set_error(ANACOR_ERR_HIP, "hipSetDevice failed");
+    return nullptr;
+  }
+  Grid* g = nullptr;
+  const int rc = grid_build(c, n_theta, n_phi, rotated_s1, theta_phi, is_exp ? 1 : 0, &g);
+  if (rc != ANACOR_OK && rc != ANACOR_ERR_GEOMETRY) return nullptr;
+  return new anacor_grid{g};
+}
+
+void anacor_grid_destroy(anacor_grid* grid) {
+  if (!grid) return;
+  if (grid->g) {
+    (void)hipSetDevice(grid->g->c->device);
+    grid_free(grid->g);
+  }
+  delete grid;
+}
+
+int anacor_grid_map(const anacor_grid* grid, float* out) {
+  if (!grid || !grid->g || !out) {
+    set_error(ANACOR_ERR_ARG, "null grid or output");
+    return ANACOR_ERR_ARG;
+  }
+  const Grid* g = grid->g;
+  std::lock_guard<std::mutex> lk(g->c->mu_run);
+  HIP_TRY(hipSetDevice(g->c->device));
+  HIP_TRY(hipMemcpyAsync(out, g->d_map, (size_t)g->D * g->c->N * sizeof(float),
+                         hipMemcpyDeviceToHost, g->c->stream));
+  HIP_TRY(hipStreamSynchronize(g->c->stream));
+  return ANACOR_OK;
+}
+
+int anacor_grid_run(anacor_grid* grid, const double* s1, const double* omega, int64_t R,
+                    const double raw_xray[3], const double omega_axis[3], const double F[9],
+                    const double* theta_list, int64_t nx, const double* phi_list, int64_t ny,
+                    int method, double* out) {
+  clear_error();
+  if (!grid || !grid->g || R < 0 || (R > 0 && (!s1 || !omega || !out || !raw_xray ||
+                                                !omega_axis || !F))) {
+    set_error(ANACOR_ERR_ARG, "bad grid run arguments");
+    return ANACOR_ERR_ARG;
+  }
+  Grid* g = grid->g;
+  Context* c = g->c;
+  if (nx < g->n_theta || ny < g->n_phi || ((nx - g->n_theta) & 1) || ((ny - g->n_phi) & 1)) {
+    set_error(ANACOR_ERR_ARG, "padded lists (%lld x %lld) do not extend the %d x %d grid "
+              "evenly", (long long)nx, (long long)ny, g->n_theta, g->n_phi);
+    return ANACOR_ERR_ARG;
+  }
+  if (R == 0) return ANACOR_OK;
+  if (c->N == 0) {
+    for (int64_t i = 0; i < R; ++i) out[i] = __builtin_nan("");
+    return ANACOR_OK;
+  }
+  std::lock_guard<std::mutex> lk(c->mu_run);
+  HIP_TRY(hipSetDevice(c->device));
+  reflection_angles(s1, omega, R, raw_xray, omega_axis, F, max_dim(c), g->h_rules, g->h_geom,
+                    g->h_angles, host_threads(0));
+  GridLists L{theta_list, nx, phi_list, ny, *std::min_element(theta_list, theta_list + nx),
+              *std::max_element(theta_list, theta_list + nx),
+              *std::min_element(phi_list, phi_list + ny), *std::max_element(phi_list, phi_list + ny)};
+  return grid_lookup(c->stream, g->d_map, nullptr, c->N, g->n_theta, g->n_phi, L,
+                     g->h_angles.data(), R, method, out, &g->d_angles, &g->cap, &g->d_lists,
+                     &g->lists_cap, &g->last_ms);
+}
+
+int anacor_grid_stats(const anacor_grid* grid, double* build_ms, double* lookup_ms) {
+  if (!grid || !grid->g) return ANACOR_ERR_ARG;
+  if (build_ms) *build_ms = grid->g->build_ms;
+  if (lookup_ms) *lookup_ms = grid->g->last_ms;
+  return ANACOR_OK;
+}
+
+double* nearest_neighbor_interpolate_overall(
+    int64_t low, int64_t up, int64_t* coord_list, int64_t len_coord_list,
+    const double* scattering_vector_list, const double* omega_list, const double* raw_xray,
+    const double* omega_axis, const double* kp_rotation_matrix, int64_t len_result,
+    double* voxel_size, double* coefficients, int num_workers, int IsExp, double* theta_list,
+    double* phi_list, double* gridding_data, size_t nx, size_t ny, double theta_min,
+    double theta_max, double phi_min, double phi_max, int interpolation_method) {
+  (void)low; (void)up; (void)coord_list; (void)voxel_size; (void)coefficients; (void)IsExp;
+  clear_error();
+  double* result = (double*)std::malloc((size_t)std::max<int64_t>(len_result, 1) * sizeof(double));
+  if (!result) return nullptr;
+  for (int64_t i = 0; i < len_result; ++i) result[i] = __builtin_nan("");
+  if (len_result <= 0) return result;
+  if (!scattering_vector_list || !omega_list || !raw_xray || !omega_axis ||
+      !kp_rotation_matrix || !gridding_data || !theta_list || !phi_list || len_coord_list <= 0) {
+    set_error(ANACOR_ERR_ARG, "null argument");
+    return result;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error(ANACOR_ERR_NO_DEVICE, "no HIP device visible");
+    fprintf(stderr, "anacor: %s\n", g_err);
+    return result;
+  }
+  std::vector<ReflRule> rules;
+  std::vector<ReflGeom> geom;
+  std::vector<double> ang;
+  reflection_angles(scattering_vector_list, omega_list, len_result, raw_xray, omega_axis,
+                    kp_rotation_matrix, 1 << 20, rules, geom, ang, host_threads(num_workers));
+  hipStream_t st = nullptr;
+  double* d_map = nullptr;
+  double* d_bufs = nullptr;
+  double* d_lists = nullptr;
+  size_t cap = 0, lcap = 0;
+  const size_t n = (size_t)len_coord_list * ny * nx;
+  auto cleanup = [&]() {
+    if (d_map) (void)hipFree(d_map);
+    if (d_bufs) (void)hipFree(d_bufs);
+    if (d_lists) (void)hipFree(d_lists);
+    if (st) (void)hipStreamDestroy(st);
+  };
+  if (hipSetDevice(0) != hipSuccess || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) !=
+      hipSuccess || hipMalloc(&d_map, n * sizeof(double)) != hipSuccess ||
+      hipMemcpyAsync(d_map, gridding_data, n * sizeof(double), hipMemcpyHostToDevice, st) !=
+      hipSuccess) {
+    set_error(ANACOR_ERR_HIP, "gridding map upload failed");
+    fprintf(stderr, "anacor: %s\n", g_err);
+    cleanup();
+    return result;
+  }
+  GridLists L{theta_list, (int64_t)nx, phi_list, (int64_t)ny, theta_min, theta_max, phi_min,
+              phi_max};
+  const int rc = grid_lookup(st, nullptr, d_map, (int)len_coord_list, (int)nx, (int)ny, L,
+                             ang.data(), len_result, interpolation_method, result, &d_bufs, &cap,
+                             &d_lists, &lcap, nullptr);
+  if (rc != ANACOR_OK && rc != ANACOR_ERR_GEOMETRY) fprintf(stderr, "anacor: %s\n", g_err);
+  cleanup();
+  return result;
+}
+
+double ray_tracing_single_gridding(int64_t* coord, const double* rotated_s1, double theta,
+                                   double phi, double* voxel_size, double* coefficients,
+                                   int8_t*** label_list, int64_t* shape, int full_iteration,
+                                   int index, int IsExp) {
+  clear_error();
+  if (!coord || !rotated_s1 || !label_list || !shape) {
+    set_error(ANACOR_ERR_ARG, "null argument");
+    return __builtin_nan("");
+  }
+  // the reference asserts that the given angles are those of rotated_s1
+  ReflRule rule;
+  ReflGeom geo;
+  host_setup_rotated(rotated_s1, rotated_s1, 1 << 20, &rule, &geo);
+  if (!(std::fabs(geo.theta[1] - theta) < 1e-6) || !(std::fabs(geo.phi[1] - phi) < 1e-6)) {
+    set_error(ANACOR_ERR_GEOMETRY, "direction %d: theta/phi do not match rotated_s1", index);
+    return __builtin_nan("");
+  }
+  std::vector<uint8_t> storage;
+  const uint8_t* vol = flatten_table(label_list, shape, storage);
+  Context* c = nullptr;
+  if (ctx_create(vol, shape, coord, nullptr, 1, voxel_size, coefficients, full_iteration, 0,
+                 &c) != ANACOR_OK) {
+    fprintf(stderr, "anacor: ray_tracing_single_gridding: %s\n", g_err);
+    return __builtin_nan("");
+  }
+  const double tp[2] = {theta, phi};
+  double out = __builtin_nan("");
+  {
+    std::lock_guard<std::mutex> lk(c->mu_run);
+    Grid* g = nullptr;
+    double* d64 = nullptr;
+    if (hipSetDevice(c->device) == hipSuccess && hipMalloc(&d64, sizeof(double)) == hipSuccess &&
+        grid_build(c, 1, 1, rotated_s1, tp, ((char)IsExp == 1) ? 1 : 0, &g, d64) == ANACOR_OK) {
+      double v = 0.0;
+      if (hipMemcpy(&v, d64, sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) out = v;
+    }
+    if (g) grid_free(g);
+    if (d64) (void)hipFree(d64);
+  }
+  ctx_destroy(c);
+  return out;
 }
 
 }  // extern "C"

@@ -123,6 +123,31 @@ def rel_stats(a, ref):
             "p99": float(np.quantile(r, 0.99)), "mean": float(r.mean())}
 
 
+def gridding_baseline(tracer, sw, geo, A_true, steps: int, n_theta: int = 360,
+                      n_phi: int = 180) -> dict:
+    """The gridding path (absorption maps of every voxel on a 1-degree
+    theta/phi grid, built on the GPU, then look-ups) on the surrogate's
+    sample: error on the held-out set and rate on the inference set."""
+    from anacor_public_amd.gridding import Gridding
+    t0 = time.perf_counter()
+    out = {"grid": [n_theta, n_phi]}
+    with Gridding(tracer, n_theta, n_phi) as grid:
+        out["build_s"] = time.perf_counter() - t0
+        out["map_kernel_ms"] = grid.stats()["build_ms"]
+        for name, method in (("nearest", Gridding.NEAREST), ("bilinear", Gridding.BILINEAR)):
+            A = grid.run(sw.heldout_s1, sw.heldout_omega, *geo, method=method)
+            err = rel_stats(A, A_true)
+            grid.run(sw.infer_s1, sw.infer_omega, *geo, method=method)
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                grid.run(sw.infer_s1, sw.infer_omega, *geo, method=method)
+            rate = len(sw.infer_omega) * steps / (time.perf_counter() - t1)
+            out[name] = {"value": rate, "heldout_rel_error": err,
+                         "lookup_kernel_ms": grid.stats()["lookup_ms"]}
+            log(f"# gridding {n_theta}x{n_phi} {name}: {rate:.3e} A_h/s, held-out {err}")
+    return out
+
+
 def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int = 600,
                   cpu_budget: float = 0.0):
     """Fit the surrogate, judge it on the held-out set, time 1M inferences
@@ -183,6 +208,8 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
         }
         log(f"# surrogate: {value:.3e} A_h/s ({rec['speedup_vs_plain_gpu']:.1f}x the plain GPU "
             f"path's {plain_rate:.3e} on this sample), MLP {mms:.2f} ms per 1M, fallback {fb}")
+        # the reference's own approximation on the same sample: gridding maps
+        rec["gridding_baseline"] = gridding_baseline(tr, sw, geo, A_true, steps)
         if cpu_budget > 0:
             cpu, _ = cpu_baseline([W.Workload(wl.name, wl.volume, wl.coords, sw.infer_s1[:4096],
                                               sw.infer_omega[:4096])], cpu_budget)

@@ -198,6 +198,56 @@ int anacor_batch_run(anacor_batch *batch, const double *const *s1, const double 
 /* Statistics of the most recent batch run (kernel_ms covers the whole graph). */
 int anacor_batch_stats(const anacor_batch *batch, anacor_stats_t *stats);
 
+/* ------------------------------------------------------------ gridding path */
+
+/* Replaces ray_tracing_single_gridding, AnACor/src/ray_tracing.c:484-534
+ * (ctypes binding utils/utils_gridding.py:104-118, per-voxel loop :162-231):
+ * sum of mu*L of one ray from coord along rotated_s1 walked with the given
+ * angles (IsExp = 1: its exp(-.)).  NaN (with ANACOR_ERR_GEOMETRY) where the
+ * reference's asserts abort.  One ray per call; anacor_grid_create builds a
+ * whole map in one launch. */
+double ray_tracing_single_gridding(int64_t *coord, const double *rotated_s1, double theta,
+                                   double phi, double *voxel_size, double *coefficients,
+                                   int8_t ***label_list, int64_t *shape, int full_iteration,
+                                   int index, int IsExp);
+
+/* Replaces nearest_neighbor_interpolate_overall, AnACor/src/gridding_interpolation.c:
+ * 98-161 (binding utils/utils_gridding.py:567-591, call :627-641): per
+ * reflection the mean over voxels of exp(-(m1 + m2)) where m1, m2 are the
+ * padded map gridding_data[N][ny][nx] looked up at the incident and diffracted
+ * (theta, phi): interpolation_method 1 = the reference's truncated
+ * nearest-neighbour index, 2 = bilinear as GSL's interp2d (the GSL path of
+ * gridding_interpolation.c:53-78).  Returns malloc'd double[len_result]. */
+double *nearest_neighbor_interpolate_overall(
+    int64_t low, int64_t up, int64_t *coord_list, int64_t len_coord_list,
+    const double *scattering_vector_list, const double *omega_list, const double *raw_xray,
+    const double *omega_axis, const double *kp_rotation_matrix, int64_t len_result,
+    double *voxel_size, double *coefficients, int num_workers, int IsExp, double *theta_list,
+    double *phi_list, double *gridding_data, size_t nx, size_t ny, double theta_min,
+    double theta_max, double phi_min, double phi_max, int interpolation_method);
+
+/* A resident absorption map of the crystal voxels of ctx for n_theta x n_phi
+ * directions (phi-major, as utils_gridding.py:254-259 orders them):
+ * rotated_s1[D][3] and their (theta, phi)[D][2].  Built by one kernel that
+ * walks every (direction, voxel) ray with the tracer's exact rules; entries
+ * are floats as the reference's .npy maps.  The map is [D][N] (voxels in
+ * the order given to anacor_ctx_create), the reference's absorption_map. */
+typedef struct anacor_grid anacor_grid;
+anacor_grid *anacor_grid_create(anacor_ctx *ctx, int64_t n_theta, int64_t n_phi,
+                                const double *rotated_s1, const double *theta_phi, int is_exp);
+void anacor_grid_destroy(anacor_grid *grid);
+int anacor_grid_map(const anacor_grid *grid, float *out);
+/* The correction of R reflections by look-up (method 1 nearest, 2 bilinear)
+ * on the padded grid the lists describe (theta_list[nx], phi_list[ny]: the
+ * n_theta x n_phi grid extended by (nx - n_theta) / 2 columns and
+ * (ny - n_phi) / 2 rows from the opposite ends, utils_gridding.py:406-445). */
+int anacor_grid_run(anacor_grid *grid, const double *s1, const double *omega, int64_t R,
+                    const double raw_xray[3], const double omega_axis[3], const double F[9],
+                    const double *theta_list, int64_t nx, const double *phi_list, int64_t ny,
+                    int method, double *out);
+/* Duration of the map build and of the latest look-up (HIP events). */
+int anacor_grid_stats(const anacor_grid *grid, double *build_ms, double *lookup_ms);
+
 /* Host-side geometry of R reflections (diagnostics): theta_phi[R][4] =
  * (theta, phi) of the incident then the diffracted ray; d_table[R][2][6][3] =
  * per-face per-increment deltas (z, y, x).  Either output may be NULL. */

@@ -514,3 +514,52 @@ double oracle_single(const int64_t *coords, int64_t n_coords,
     free(segs.cls);
     return m;
 }
+
+/* ray_tracing_single_gridding, ray_tracing.c:484-534: one ray from `coord`
+ * along rotated_s1 (the diffracted-ray convention, L1 = 0) walked with the
+ * given angles; cal_rate with zero incident lengths.  NaN where the reference
+ * would abort. */
+double oracle_single_ray(const int64_t coord[3], const double rotated_s1[3], double theta,
+                         double phi, const double *voxel_size, const double *coefficients,
+                         const uint8_t *volume, const int64_t shape[3], int full_iteration,
+                         int is_exp, oracle_ray_t *ray)
+{
+    double dir[3];
+    oracle_myframe(rotated_s1, dir);
+    const int64_t face = oracle_cube_face(coord, dir, shape, 0);
+    if (face == 0) return NAN;
+    oracle_rule_t rule;
+    oracle_face_rule(theta, phi, face, &rule);
+    if (!rule.valid) return NAN;
+    seg_buf segs = {NULL, NULL, 0};
+    oracle_ray_t rr;
+    memset(&rr, 0, sizeof(rr));
+    trace_ray(coord, &rule, face, (const int8_t *)volume, shape, voxel_size, full_iteration,
+              &segs, &rr);
+    free(segs.posi);
+    free(segs.cls);
+    if (ray) *ray = rr;
+    const double zero[4] = {0.0, 0.0, 0.0, 0.0};
+    return rate(zero, rr.L, coefficients, is_exp);
+}
+
+/* The absorption map of utils/utils_gridding.py:144-231: for D directions
+ * (rotated_s1[D][3] with their angles theta_phi[D][2]) and every crystal
+ * voxel, oracle_single_ray stored as float, map[D][n_coords]. */
+void oracle_gridding_map(const int64_t *coords, int64_t n_coords, const double *rotated_s1,
+                         const double *theta_phi, int64_t D, const double *voxel_size,
+                         const double *coefficients, const uint8_t *volume,
+                         const int64_t shape[3], int full_iteration, int is_exp,
+                         int n_threads, float *map)
+{
+    if (n_threads < 1) n_threads = 1;
+#pragma omp parallel for num_threads(n_threads) schedule(dynamic, 1)
+    for (int64_t d = 0; d < D; d++) {
+        for (int64_t k = 0; k < n_coords; k++) {
+            map[d * n_coords + k] = (float)oracle_single_ray(
+                coords + 3 * k, rotated_s1 + 3 * d, theta_phi[2 * d], theta_phi[2 * d + 1],
+                voxel_size, coefficients, volume, shape, full_iteration, is_exp, NULL);
+        }
+    }
+}
+

@@ -83,6 +83,15 @@ int64_t oracle_trace(const int64_t *coords, int64_t n_coords,
 
 /* Equivalent of ray_tracing_single (ray_tracing.c:166-292): already rotated
  * vectors; returns the mean (NaN on an error). */
+double oracle_single_ray(const int64_t coord[3], const double rotated_s1[3], double theta,
+                         double phi, const double *voxel_size, const double *coefficients,
+                         const uint8_t *volume, const int64_t shape[3], int full_iteration,
+                         int is_exp, oracle_ray_t *ray);
+void   oracle_gridding_map(const int64_t *coords, int64_t n_coords, const double *rotated_s1,
+                           const double *theta_phi, int64_t D, const double *voxel_size,
+                           const double *coefficients, const uint8_t *volume,
+                           const int64_t shape[3], int full_iteration, int is_exp,
+                           int n_threads, float *map);
 double oracle_single(const int64_t *coords, int64_t n_coords,
                      const double rotated_s1[3], const double xray[3],
                      const double *voxel_size, const double *coefficients,

@@ -91,6 +91,14 @@ class Oracle:
                                                 ct.c_void_p, ct.c_void_p, ct.c_void_p]
         lib.oracle_kp_rotation.restype = None
         lib.oracle_kp_rotation.argtypes = [ct.c_void_p, ct.c_double, ct.c_void_p]
+        lib.oracle_single_ray.restype = ct.c_double
+        lib.oracle_single_ray.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_double,
+                                          ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                          ct.c_int, ct.c_int, ct.c_void_p]
+        lib.oracle_gridding_map.restype = None
+        lib.oracle_gridding_map.argtypes = [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p,
+                                            ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                            ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p]
         self.lib = lib
 
     def trace(self, wl, n_threads: int = 0, rays: bool = False, R: int | None = None):
@@ -173,6 +181,33 @@ class Oracle:
         self.lib.oracle_kp_rotation(_ptr(a), float(theta), _ptr(out))
         return out
 
+    def single_ray(self, wl, coord, rotated_s1, theta, phi, is_exp: int = 0) -> float:
+        """ray_tracing_single_gridding restated: one ray's sum of mu*L (or exp)."""
+        vol = np.ascontiguousarray(wl.volume).view(np.uint8)
+        shape = np.array(vol.shape, dtype=np.int64)
+        return float(self.lib.oracle_single_ray(
+            _ptr(np.ascontiguousarray(coord, dtype=np.int64)),
+            _ptr(np.ascontiguousarray(rotated_s1, dtype=np.float64)), float(theta), float(phi),
+            _ptr(np.ascontiguousarray(wl.voxel_size, dtype=np.float64)),
+            _ptr(np.ascontiguousarray(wl.coefficients, dtype=np.float64)), _ptr(vol),
+            _ptr(shape), int(wl.full_iteration), int(is_exp), None))
+
+    def gridding_map(self, wl, rotated_s1, theta_phi, is_exp: int = 0,
+                     n_threads: int = 0) -> np.ndarray:
+        """float32 [D][n_coords] absorption map of the workload's voxels."""
+        vol = np.ascontiguousarray(wl.volume).view(np.uint8)
+        shape = np.array(vol.shape, dtype=np.int64)
+        coords = np.ascontiguousarray(wl.coords, dtype=np.int64)
+        dirs = np.ascontiguousarray(rotated_s1, dtype=np.float64).reshape(-1, 3)
+        tp = np.ascontiguousarray(theta_phi, dtype=np.float64).reshape(-1, 2)
+        out = np.empty((len(dirs), len(coords)), dtype=np.float32)
+        self.lib.oracle_gridding_map(
+            _ptr(coords), len(coords), _ptr(dirs), _ptr(tp), len(dirs),
+            _ptr(np.ascontiguousarray(wl.voxel_size, dtype=np.float64)),
+            _ptr(np.ascontiguousarray(wl.coefficients, dtype=np.float64)), _ptr(vol), _ptr(shape),
+            int(wl.full_iteration), int(is_exp), int(n_threads or os.cpu_count() or 1), _ptr(out))
+        return out
+
 
 def label_table(volume: np.ndarray):
     """The int8*** row-pointer table that ``python_2_c_3d`` builds
@@ -217,7 +252,29 @@ class Reference:
                                             ct.c_int]
         lib.free.restype = None
         lib.free.argtypes = [ct.c_void_p]
+        lib.ray_tracing_single_gridding.restype = ct.c_double
+        lib.ray_tracing_single_gridding.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_double,
+                                                    ct.c_double, ct.c_void_p, ct.c_void_p,
+                                                    ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int,
+                                                    ct.c_int]
         self.lib = lib
+
+    def single_gridding(self, wl, coord, rotated_s1, theta, phi, is_exp: int = 0,
+                        table=None) -> float:
+        """The reference's ray_tracing_single_gridding (ray_tracing.c:484-534)."""
+        keep = None
+        if table is None:
+            table, keep = label_table(np.ascontiguousarray(wl.volume).view(np.int8))
+        shape = np.array(np.asarray(wl.volume).shape, dtype=np.int64)
+        with _stdout_silenced():
+            v = self.lib.ray_tracing_single_gridding(
+                _ptr(np.ascontiguousarray(coord, dtype=np.int64)),
+                _ptr(np.ascontiguousarray(rotated_s1, dtype=np.float64)), float(theta), float(phi),
+                _ptr(np.ascontiguousarray(wl.voxel_size, dtype=np.float64)),
+                _ptr(np.ascontiguousarray(wl.coefficients, dtype=np.float64)), table, _ptr(shape),
+                int(wl.full_iteration), 0, int(is_exp))
+        del keep
+        return float(v)
 
     def overall(self, wl, n_threads: int = 0, R: int | None = None) -> np.ndarray:
         s1 = np.ascontiguousarray(wl.s1, dtype=np.float64)
