@@ -83,19 +83,24 @@ def test_files_end_to_end(gpu, oracle, tmp_path):
     wl = W.small_config(R=40)
     rng = np.random.default_rng(1)
     xyz = np.stack([rng.uniform(0, 100, 40), rng.uniform(0, 100, 40), wl.omega], axis=1)
-    _write_refl(tmp_path / "refl.json", wl.s1, xyz, rng)
+    # the stacker takes every *refl*.json of its folder, as the reference's
+    # does, so the inputs live apart from the correction chunks
+    inp, out_dir = tmp_path / "in", tmp_path / "out"
+    inp.mkdir()
+    out_dir.mkdir()
+    _write_refl(inp / "refl.json", wl.s1, xyz, rng)
     gon = {"axes": [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], wl.omega_axis.tolist()],
            "angles": [0.0, 0.0, 0.0]}
-    with open(tmp_path / "expt.json", "w") as f:
+    with open(inp / "expt.json", "w") as f:
         json.dump([gon, {"direction": (-wl.raw_xray).tolist()}], f)
-    e = dials_io.load_experiment(str(tmp_path / "expt.json"))
+    e = dials_io.load_experiment(str(inp / "expt.json"))
     np.testing.assert_array_equal(e.F, np.eye(3).reshape(-1))
     with Tracer.for_workload(wl) as t:
         direct = t.run_workload(wl)
         for low, up in ((0, 20), (20, -1)):
-            r = dials_io.load_reflections(str(tmp_path / "refl.json"), low=low, up=up)
-            dials_io.save_corrections(str(tmp_path), "ds", up,
+            r = dials_io.load_reflections(str(inp / "refl.json"), low=low, up=up)
+            dials_io.save_corrections(str(out_dir), "ds", up,
                                       t.run(r.s1, r.omega, e.raw_xray, e.omega_axis, e.F))
-    out = np.array(dials_io.stack_corrections(str(tmp_path), "ds"))
+    out = np.array(dials_io.stack_corrections(str(out_dir), "ds"))
     np.testing.assert_array_equal(out, direct)
     np.testing.assert_allclose(out, oracle.trace(wl)["result"], rtol=1e-6, atol=0)
