@@ -44,6 +44,7 @@ EXPORTED = (
     "anacor_mlp_gradients", "anacor_mlp_train", "anacor_mlp_predict", "anacor_mlp_stats",
     "ray_tracing_single_gridding", "nearest_neighbor_interpolate_overall", "anacor_grid_create",
     "anacor_grid_destroy", "anacor_grid_map", "anacor_grid_run", "anacor_grid_stats",
+    "ib_test", "anacor_ctx_bisection", "anacor_ctx_bisection_rotated",
 )
 
 MLP_INPUTS = 12
@@ -60,7 +61,8 @@ assert PATH_DTYPE.itemsize == 120
 
 class Stats(ct.Structure):
     _fields_ = [("kernel_ms", ct.c_double), ("voxel_steps", ct.c_uint64),
-                ("workgroups", ct.c_int64), ("n_voxels", ct.c_int64), ("host_ms", ct.c_double)]
+                ("workgroups", ct.c_int64), ("n_voxels", ct.c_int64), ("host_ms", ct.c_double),
+                ("outside_reads", ct.c_uint64)]
 
 
 class Sample(ct.Structure):
@@ -184,6 +186,14 @@ def _prototypes(lib):
     lib.ray_tracing_single.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, i32, i64, i32]
     lib.ray_tracing_single_mp.restype = dbl
     lib.ray_tracing_single_mp.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, i32, i64, i32, i32]
+    # iterative bisection (binding mp_worker.py:390-404; its trailing IsExp
+    # argument is dropped here, the callee ignores it)
+    lib.ib_test.restype = dbl
+    lib.ib_test.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, i32, i64, i32]
+    lib.anacor_ctx_bisection.restype = ct.c_int
+    lib.anacor_ctx_bisection.argtypes = [vp, vp, vp, i64, vp, vp, vp, i32, i32, vp]
+    lib.anacor_ctx_bisection_rotated.restype = ct.c_int
+    lib.anacor_ctx_bisection_rotated.argtypes = [vp, vp, vp, i64, i32, i32, vp, vp]
     lib.free.restype = None
     lib.free.argtypes = [vp]
     return lib

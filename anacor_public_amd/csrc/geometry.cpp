@@ -240,6 +240,51 @@ void host_setup(const double* s1, const double* omega, int64_t R, const double* 
   for (auto& th : pool) th.join();
 }
 
+void ib_rules(const ReflGeom* geom, int64_t R, IbRule* rules) {
+  // increments (bisection.c:21-148), each expression as the reference
+  // writes it; a face whose assert fails keeps zero ratios and no valid bit
+  for (int64_t i = 0; i < R; ++i) {
+    for (int k = 0; k < 2; ++k) {
+      IbRay& ray = rules[i].ray[k];
+      std::memset(&ray, 0, sizeof(ray));
+      std::memcpy(ray.dir, geom[i].dir[k], 3 * sizeof(double));
+      ray.neg = k == 0 ? 1u : 0u;
+      const double t = geom[i].theta[k], p = geom[i].phi[k];
+      const double at = std::fabs(t), ap = std::fabs(p);
+      const bool narrow = at < kPi / 2;
+      auto set = [&](int face, bool ok, double rz, double ry, double rx) {
+        if (!ok) return;
+        ray.r[face - 1][0] = rz;
+        ray.r[face - 1][1] = ry;
+        ray.r[face - 1][2] = rx;
+        ray.valid |= 1u << (face - 1);
+      };
+      // face 4 (y -> 0) and 5 (y -> y_max), :39-84
+      if (narrow) {
+        set(4, t > 0, std::sin(p) / std::tan(at), 1.0, -std::cos(ap) / std::tan(at));
+        set(5, t < 0, std::sin(p) / std::tan(at), -1.0, -std::cos(ap) / std::tan(at));
+      } else {
+        const double tt = std::tan(kPi - at);
+        set(4, t > 0, std::sin(-p) / tt, 1.0, std::cos(ap) / tt);
+        set(5, t < 0, -std::sin(p) / tt, -1.0, std::cos(ap) / tt);
+      }
+      // face 6 (x -> 0) and 1 (x -> x_max), :85-106
+      set(6, at <= kPi / 2, std::tan(p), std::tan(t) / std::cos(p), -1.0);
+      set(1, at >= kPi / 2, -std::tan(p), std::tan(kPi - t) / std::cos(ap), 1.0);
+      // face 2 (z -> 0) and 3 (z -> z_max), :107-142
+      if (narrow) {
+        const double rx = -1 / std::tan(ap), ry = std::tan(t) / std::sin(ap);
+        set(2, p < 0, -1.0, ry, rx);
+        set(3, p > 0, 1.0, ry, rx);
+      } else {
+        const double rx = 1 / std::tan(ap), ry = std::tan(kPi - t) / std::sin(ap);
+        set(2, p > 0, -1.0, ry, rx);
+        set(3, p < 0, 1.0, ry, rx);
+      }
+    }
+  }
+}
+
 void host_setup_directions(const double* rotated_s1, const double* theta_phi, int64_t D,
                            int64_t max_dim, RayRule* rays) {
   for (int64_t i = 0; i < D; ++i) {

@@ -39,6 +39,26 @@ struct ReflGeom {
   double dir[2][3];
 };
 
+// One ray of the bisection traversal (increments, bisection.c:21-148): the
+// cube_face direction, and per exit face the ratios with which the sample
+// point at distance m along the face axis is (z + m*r[0], y - m*r[1],
+// x + m*r[2]).  240 bytes.
+struct alignas(16) IbRay {
+  double dir[4];        // myframe (z, y, x) direction, [3] unused
+  double r[6][4];       // per face f = 1..6 (index f-1): (rz, ry, rx), [3] unused
+  uint32_t valid;       // bit f-1: the reference's assert for face f holds
+  uint32_t neg;         // 1 for the incident ray (cube_face's L1)
+  uint32_t pad[2];
+};
+static_assert(sizeof(IbRay) == 240, "IbRay layout");
+
+struct IbRule {
+  IbRay ray[2];         // [0] incident, [1] diffracted
+};
+
+// Bisection rules of R reflections from their angles and directions.
+void ib_rules(const ReflGeom* geom, int64_t R, IbRule* rules);
+
 // Fill rules[R] (and geom[R] when non-null) for R reflections, using up to
 // n_threads host threads.  max_dim is the largest volume extent, used to set
 // RayRule::guard.  Thread-safe, allocates nothing.

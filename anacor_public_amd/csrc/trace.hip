@@ -1077,6 +1077,12 @@ struct Context {
   std::vector<int> h_order;         // processing position -> caller's index
   Pinned<int> h_order_pin;          // its upload staging
   int* d_order = nullptr;
+  // bisection traversal
+  IbRule* d_ib = nullptr;
+  size_t ib_cap = 0;
+  Pinned<IbRule> h_ib;
+  unsigned long long* d_oob = nullptr;
+  unsigned long long last_oob = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_kernel_ms = 0.f;
   unsigned long long last_steps = 0;
@@ -1265,7 +1271,8 @@ static int ensure_layout(Context* c) {
 // workgroups resident on an XCD at one time trace nearly parallel rays whose
 // label gathers share that XCD's L2.  Results are written back through the
 // inverse permutation, so the order is invisible to callers.
-static void order_reflections(Context* c, const ReflRule* rules, int64_t R) {
+template <class DirOf>   // DirOf(i, k): direction (z, y, x) of ray k of reflection i
+static void order_by_direction(Context* c, int64_t R, DirOf dir_of) {
   c->h_order.resize((size_t)R);
   for (int64_t i = 0; i < R; ++i) c->h_order[(size_t)i] = (int)i;
   const char* env = std::getenv("ANACOR_SORT_REFLECTIONS");
@@ -1278,8 +1285,8 @@ static void order_reflections(Context* c, const ReflRule* rules, int64_t R) {
   };
   std::vector<uint64_t> key((size_t)R);
   for (int64_t i = 0; i < R; ++i) {
-    const double* a = rules[i].ray[0].dir;
-    const double* b = rules[i].ray[1].dir;
+    const double* a = dir_of(i, 0);
+    const double* b = dir_of(i, 1);
     const uint64_t ka = (spread_bits(q(a[0])) << 2) | (spread_bits(q(a[1])) << 1) | spread_bits(q(a[2]));
     const uint64_t kb = (spread_bits(q(b[0])) << 2) | (spread_bits(q(b[1])) << 1) | spread_bits(q(b[2]));
     key[(size_t)i] = (ka << 21) | kb;
@@ -1288,10 +1295,15 @@ static void order_reflections(Context* c, const ReflRule* rules, int64_t R) {
                    [&](int x, int y) { return key[(size_t)x] < key[(size_t)y]; });
 }
 
+static void order_reflections(Context* c, const ReflRule* rules, int64_t R) {
+  order_by_direction(c, R, [rules](int64_t i, int k) { return rules[i].ray[k].dir; });
+}
+
 static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flags, double* out,
                      PathRecord* paths_out) {
   const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
   const bool paths = paths_out != nullptr;
+  c->last_oob = 0;
   if (R == 0) return ANACOR_OK;
   if (c->N == 0) {
     for (int64_t i = 0; i < R; ++i) out ? (void)(out[i] = __builtin_nan("")) : (void)0;
@@ -1544,6 +1556,8 @@ static void ctx_destroy(Context* c) {
   if (c->d_out) (void)hipFree(c->d_out);
   if (c->d_order) (void)hipFree(c->d_order);
   if (c->d_steps) (void)hipFree(c->d_steps);
+  if (c->d_ib) (void)hipFree(c->d_ib);
+  if (c->d_oob) (void)hipFree(c->d_oob);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -2109,6 +2123,365 @@ static int grid_lookup(hipStream_t stream, const float* map, const double* map64
   return ANACOR_OK;
 }
 
+// ------------------------------------------------------ iterative bisection
+//
+// The reference's alternative traversal (ib_test, ray_tracing.c:25-160, with
+// bisection.c:21-471): per ray, instead of visiting every voxel to the exit
+// plane, bisect the distance along the exit face's axis for the outer air
+// boundary, the crystal's outer boundary and, between them, the loop (class
+// 2) and bubble (class 4) boundaries, each to a resolution of one step.
+// One (reflection, voxel) pair per lane as in trace_kernel; a lane's label
+// reads are a chain of dependent gathers (about 8 per search, 4 to 6 searches
+// per ray), so the kernel is bound by gather latency, not by bytes.
+//
+// Positions are FP64 values of floor(): integers a double holds exactly, so
+// differences and lengths equal the reference's int64 arithmetic.  A sample
+// point outside the volume reads as vacuum (0); the reference reads its row
+// table there unchecked.  Such reads are counted (anacor_stats_t).
+
+struct IbArgs {
+  const uint8_t* vol;          // raw volume or the packed one
+  int vol_bytes;
+  int cz, cy;                  // brick constants (bricked layouts)
+  int Z, Y, X;
+  const int4* coords;          // (z, y, x, caller index), wave-grouped order
+  int N;
+  const IbRule* rules;         // caller's order
+  int R;
+  int tiles, vox_per_tile, n_blocks;
+  double vs[3];
+  double mu[4];
+  int is_exp;
+  int num_cls;
+  float2* partial32;
+  double* partial64;
+  unsigned* err;
+  unsigned long long* reads;   // label reads
+  unsigned long long* oob;     // of which outside the volume
+  double* lengths;             // [R][N][2][4] in the caller's voxel order, or null
+  const int* order;
+};
+
+// Label at integer-valued FP64 position (z, y, x); 0 outside the volume.
+template <int LAYOUT>
+__device__ __forceinline__ int ib_label(const Volume& V, double z, double y, double x,
+                                        unsigned& oob) {
+  if (!(z >= 0.0 && z < (double)V.Z && y >= 0.0 && y < (double)V.Y && x >= 0.0 &&
+        x < (double)V.X)) {
+    ++oob;
+    return 0;
+  }
+  const int iz = (int)z, iy = (int)y, ix = (int)x;
+  if (LAYOUT == kLayoutRaw || LAYOUT == kLayoutWide)
+    return load_label(V, (iz * V.Y + iy) * V.X + ix);
+  const int f = iz * 64 + (iz >> 3) * V.cz + iy * 8 + (iy >> 3) * V.cy + ix + (ix >> 3) * 504;
+  return load_packed<LAYOUT>(V, f);
+}
+
+struct IbPoint {
+  double p[3];                 // z, y, x
+  double mid;
+};
+
+// bisection (bisection.c:156-235): halve [shortest, longest] to one step.
+// outer = 0 pulls `longest` in while the sample holds `cls`; outer = 1 pushes
+// `shortest` out while it does.  Returns the voxel plus floor(mid) steps.
+template <int LAYOUT>
+__device__ __forceinline__ IbPoint ib_search(const Volume& V, const double c[3],
+                                             const double r[3], double longest, double shortest,
+                                             bool outer, int cls, unsigned& reads,
+                                             unsigned& oob) {
+  double diff = longest - shortest;
+  double mid = (longest + shortest) / 2.0;
+  while (diff > 1.0) {
+    mid = (longest + shortest) / 2.0;
+    const int label = ib_label<LAYOUT>(V, __builtin_floor(c[0] + mid * r[0]),
+                                       __builtin_floor(c[1] - mid * r[1]),
+                                       __builtin_floor(c[2] + mid * r[2]), oob);
+    if ((label == cls) == outer) shortest = mid;
+    else longest = mid;
+    diff = longest - shortest;
+    ++reads;
+  }
+  const double m = __builtin_floor(mid);
+  IbPoint q;
+  q.p[0] = __builtin_floor(c[0] + m * r[0]);
+  q.p[1] = __builtin_floor(c[1] - m * r[1]);
+  q.p[2] = __builtin_floor(c[2] + m * r[2]);
+  q.mid = mid;
+  return q;
+}
+
+__device__ __forceinline__ double ib_l1(const double a[3], const double b[3]) {
+  return __builtin_fabs(a[0] - b[0]) + __builtin_fabs(a[1] - b[1]) + __builtin_fabs(a[2] - b[2]);
+}
+
+// difference_length (bisection.c:244-253), pow(v, 2) as v*v.
+__device__ __forceinline__ double ib_dist(const double a[3], const double b[3],
+                                          const double vs[3]) {
+  const double dz = (a[0] - b[0]) * vs[0];
+  const double dy = (a[1] - b[1]) * vs[1];
+  const double dx = (a[2] - b[2]) * vs[2];
+  return __builtin_sqrt(dz * dz + dy * dy + dx * dx);
+}
+
+// iterative_bisection (bisection.c:298-471) and cal_path_bisection (:255-296)
+// of one ray from voxel c: L = [li, lo, cr, bu].
+template <int LAYOUT>
+__device__ __forceinline__ void ib_ray(const Volume& V, const double c[3], const double r[3],
+                                       double air_longest, int num_cls, const double vs[3],
+                                       double L[4], unsigned& reads, unsigned& oob) {
+  const IbPoint A = ib_search<LAYOUT>(V, c, r, air_longest, 0.0, false, 0, reads, oob);
+  const IbPoint C = ib_search<LAYOUT>(V, c, r, A.mid, 0.0, true, 3, reads, oob);
+  // the pairs appended after (voxel, A, C): at most two
+  double q[4][3];
+  int n = 0;
+  do {
+    const IbPoint Li = ib_search<LAYOUT>(V, c, r, A.mid, C.mid, false, 2, reads, oob);
+    if (ib_l1(Li.p, A.p) / 3 > 1) {
+      const IbPoint Lo = ib_search<LAYOUT>(V, c, r, A.mid, Li.mid, true, 2, reads, oob);
+      if (ib_l1(Li.p, Lo.p) / 3 <= 2) break;     // a segmentation artefact ends the search
+      for (int a = 0; a < 3; ++a) { q[n][a] = Li.p[a]; q[n + 1][a] = Lo.p[a]; }
+      n += 2;
+    }
+    if (num_cls > 3) {
+      const IbPoint Bi = ib_search<LAYOUT>(V, c, r, A.mid, C.mid, false, 4, reads, oob);
+      if (ib_l1(Bi.p, A.p) > 1) {
+        const IbPoint Bo = ib_search<LAYOUT>(V, c, r, A.mid, Bi.mid, true, 4, reads, oob);
+        if (ib_l1(Bi.p, Bo.p) / 3 <= 2) break;
+        for (int a = 0; a < 3; ++a) { q[n][a] = Bi.p[a]; q[n + 1][a] = Bo.p[a]; }
+        n += 2;
+      }
+    }
+  } while (false);
+  // with one pair it is measured as the loop, with two only the second, as
+  // the bubble (cal_path_bisection reads points 3-4 or 5-6)
+  const double total = ib_dist(c, A.p, vs);
+  const double cr = ib_dist(c, C.p, vs);
+  double lo = 0.0, bu = 0.0;
+  if (n == 2) lo = ib_dist(q[0], q[1], vs);
+  else if (n == 4) bu = ib_dist(q[2], q[3], vs);
+  L[0] = total - lo - bu - cr - 0.0;
+  L[1] = lo;
+  L[2] = cr;
+  L[3] = bu;
+}
+
+template <int LAYOUT, bool F64, bool LENGTHS>
+__global__ __launch_bounds__(kBlock) void ib_kernel(IbArgs a) {
+  __shared__ ReduceLds red;
+  const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
+  if (lb >= a.n_blocks) return;
+  const int pos = lb / a.tiles;
+  const int tile = lb - pos * a.tiles;
+  const int refl = a.order[pos];
+  const Volume V = make_volume(a.vol, a.Z, a.Y, a.X, a.vol_bytes, a.cz, a.cy);
+  const IbRule* rule = a.rules + refl;
+  TileAcc acc = {0.f, 0.f, 0.0, 0ull, 0u};
+  unsigned oob = 0;
+  const int v_begin = tile * a.vox_per_tile;
+  const int v_end = min(a.N, v_begin + a.vox_per_tile);
+  for (int v = v_begin + (int)threadIdx.x; v < v_end; v += kBlock) {
+    const int4 co = a.coords[v];
+    const double c[3] = {(double)co.x, (double)co.y, (double)co.z};
+    double L[2][4];
+    unsigned verr = 0, reads = 0;
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+      const IbRay& ray = rule->ray[k];
+      const int face = exit_face(co.x, co.y, co.z, ray.dir[0], ray.dir[1], ray.dir[2], a.Z, a.Y,
+                                 a.X, ray.neg != 0);
+      L[k][0] = L[k][1] = L[k][2] = L[k][3] = 0.0;
+      if (face == 0) { verr |= ANACOR_RAY_ERR_FACE; continue; }
+      if (!((ray.valid >> (face - 1)) & 1u)) { verr |= ANACOR_RAY_ERR_ASSERT; continue; }
+      // AirLongest of increments: steps from the voxel to the face's last plane
+      double air;
+      switch (face) {
+        case 4: air = (double)co.y; break;
+        case 5: air = (double)(a.Y - 1 - co.y); break;
+        case 6: air = (double)co.z; break;
+        case 1: air = (double)(a.X - 1 - co.z); break;
+        case 2: air = (double)co.x; break;
+        default: air = (double)(a.Z - 1 - co.x); break;
+      }
+      const double r[3] = {ray.r[face - 1][0], ray.r[face - 1][1], ray.r[face - 1][2]};
+      ib_ray<LAYOUT>(V, c, r, air, a.num_cls, a.vs, L[k], reads, oob);
+    }
+    acc.err |= verr;
+    acc.steps += reads;
+    if (LENGTHS) {
+      double* out = a.lengths + ((size_t)refl * a.N + co.w) * 8;
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[k * 4 + j] = L[k][j];
+    }
+    // cal_rate (ray_tracing_basic.c:1554-1583), same association order
+    const double rsum = a.mu[0] * (L[0][0] + L[1][0]) + a.mu[1] * (L[0][1] + L[1][1]) +
+                        a.mu[2] * (L[0][2] + L[1][2]) + a.mu[3] * (L[0][3] + L[1][3]);
+    if (F64) {
+      acc.acc64 += a.is_exp ? exp(-rsum) : rsum;
+    } else {
+      const float val = a.is_exp ? exp_neg(rsum) : (float)rsum;
+      const float y = val - acc.kc;
+      const float t = acc.ks + y;
+      acc.kc = (t - acc.ks) - y;
+      acc.ks = t;
+    }
+  }
+  if (oob) atomicAdd(a.oob, (unsigned long long)oob);
+  reduce_tile<F64, false>(acc, red, &a.err[refl], a.reads, a.partial32 + lb, a.partial64 + lb);
+}
+
+template <int LAYOUT, bool F64, bool LENGTHS>
+static void launch_ib(const IbArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((ib_kernel<LAYOUT, F64, LENGTHS>), dim3((unsigned)grid), dim3(kBlock), 0, s,
+                     a);
+}
+
+static void dispatch_ib(const IbArgs& a, int grid, int layout, bool f64, bool lengths,
+                        hipStream_t s) {
+  using Fn = void (*)(const IbArgs&, int, hipStream_t);
+  static const Fn table[3][2][2] = {
+      {{launch_ib<kLayoutRaw, false, false>, launch_ib<kLayoutRaw, false, true>},
+       {launch_ib<kLayoutRaw, true, false>, launch_ib<kLayoutRaw, true, true>}},
+      {{launch_ib<kLayoutBrick2, false, false>, launch_ib<kLayoutBrick2, false, true>},
+       {launch_ib<kLayoutBrick2, true, false>, launch_ib<kLayoutBrick2, true, true>}},
+      {{launch_ib<kLayoutBrick4, false, false>, launch_ib<kLayoutBrick4, false, true>},
+       {launch_ib<kLayoutBrick4, true, false>, launch_ib<kLayoutBrick4, true, true>}}};
+  // the raw and wide layouts share one 32-bit address form here
+  const int li = layout == kLayoutBrick2 ? 1 : layout == kLayoutBrick4 ? 2 : 0;
+  table[li][f64 ? 1 : 0][lengths ? 1 : 0](a, grid, s);
+}
+
+// First failing reflection of a run (its result is already NaN) as the
+// context's error; the error words are read back only when some result is NaN.
+static int report_ray_errors(Context* c, const double* out, int64_t R) {
+  int64_t bad = -1;
+  for (int64_t i = 0; i < R && bad < 0; ++i)
+    if (out[i] != out[i]) bad = i;
+  if (bad < 0) return ANACOR_OK;
+  std::vector<unsigned> herr((size_t)R);
+  HIP_TRY(hipMemcpy(herr.data(), c->d_err, (size_t)R * sizeof(unsigned), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < R; ++i)
+    if (herr[(size_t)i]) {
+      set_error(ANACOR_ERR_GEOMETRY,
+                "reflection %lld: %s (the reference aborts here); result set to NaN",
+                (long long)i,
+                (herr[(size_t)i] & ANACOR_RAY_ERR_FACE) ? "no exit face (cube_face)"
+                                                        : "exit face inconsistent with beam angles");
+      return ANACOR_ERR_GEOMETRY;
+    }
+  return ANACOR_OK;
+}
+
+// Bisection traversal of R reflections given their angles and directions.
+static int ib_run(Context* c, const ReflGeom* geom, int64_t R, int num_cls, int flags,
+                  double* out, double* lengths) {
+  const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
+  c->last_oob = 0;
+  c->last_steps = 0;
+  if (R == 0) return ANACOR_OK;
+  if (c->N == 0) {
+    for (int64_t i = 0; i < R; ++i) out[i] = __builtin_nan("");
+    return ANACOR_OK;
+  }
+  const int vpt = choose_tile(c, R);
+  const int64_t tiles = (c->N + vpt - 1) / vpt;
+  const int64_t nblk = R * tiles;
+  if (nblk > (int64_t)0x7fffffff - 8) {
+    set_error(ANACOR_ERR_ARG, "too many workgroups (%lld)", (long long)nblk);
+    return ANACOR_ERR_ARG;
+  }
+  int rc = ensure_run_buffers(c, R, nblk, f64);
+  if (rc) return rc;
+  if ((size_t)R > c->ib_cap) {
+    if (c->d_ib) HIP_TRY(hipFree(c->d_ib));
+    c->d_ib = nullptr;
+    HIP_TRY(hipMalloc(&c->d_ib, (size_t)R * sizeof(IbRule)));
+    c->ib_cap = (size_t)R;
+  }
+  if (!c->d_oob) HIP_TRY(hipMalloc(&c->d_oob, sizeof(unsigned long long)));
+  HIP_TRY(c->h_ib.reserve((size_t)R));
+  ib_rules(geom, R, c->h_ib.data());
+  order_by_direction(c, R, [geom](int64_t i, int k) { return geom[i].dir[k]; });
+  HIP_TRY(c->h_order_pin.reserve((size_t)R));
+  for (int64_t i = 0; i < R; ++i) c->h_order_pin[(size_t)i] = c->h_order[(size_t)i];
+  rc = ensure_layout(c);
+  if (rc) return rc;
+  double* d_len = nullptr;
+  if (lengths) HIP_TRY(hipMalloc(&d_len, (size_t)R * c->N * 8 * sizeof(double)));
+
+  HIP_TRY(hipMemcpyAsync(c->d_ib, c->h_ib.data(), (size_t)R * sizeof(IbRule),
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order_pin.data(), (size_t)R * sizeof(int),
+                         hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemsetAsync(c->d_err, 0, (size_t)R * sizeof(unsigned), c->stream));
+  HIP_TRY(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
+  HIP_TRY(hipMemsetAsync(c->d_oob, 0, sizeof(unsigned long long), c->stream));
+
+  IbArgs a;
+  const int layout = c->layout;
+  const bool bricked = layout == kLayoutBrick2 || layout == kLayoutBrick4;
+  a.vol = bricked ? c->d_pack : c->d_vol;
+  a.vol_bytes = bricked ? (int)c->pack_bytes : c->Z * c->Y * c->X;
+  a.cz = 512 * c->NBX * c->NBY - 512;
+  a.cy = 512 * c->NBX - 64;
+  a.Z = c->Z; a.Y = c->Y; a.X = c->X;
+  a.coords = c->d_coords;
+  a.N = c->N;
+  a.rules = c->d_ib;
+  a.R = (int)R;
+  a.tiles = (int)tiles;
+  a.vox_per_tile = vpt;
+  a.n_blocks = (int)nblk;
+  for (int i = 0; i < 3; ++i) a.vs[i] = c->vs[i];
+  for (int i = 0; i < 4; ++i) a.mu[i] = c->mu[i];
+  a.is_exp = (flags & ANACOR_FLAG_NO_EXP) ? 0 : 1;
+  a.num_cls = num_cls;
+  a.partial32 = (float2*)c->d_partial;
+  a.partial64 = (double*)c->d_partial;
+  a.err = c->d_err;
+  a.reads = c->d_steps;
+  a.oob = c->d_oob;
+  a.lengths = d_len;
+  a.order = c->d_order;
+  const int grid = (int)((nblk + 7) / 8 * 8);
+
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  dispatch_ib(a, grid, layout, f64, lengths != nullptr, c->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, c->stream,
+                     (const float2*)c->d_partial, (const double*)c->d_partial, c->d_err,
+                     c->d_order, (int)R, (int)tiles, c->N, f64 ? 1 : 0, c->d_out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, c->d_out, (size_t)R * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  if (lengths)
+    HIP_TRY(hipMemcpyAsync(lengths, d_len, (size_t)R * c->N * 8 * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(&c->last_steps, c->d_steps, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(&c->last_oob, c->d_oob, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipEventElapsedTime(&c->last_kernel_ms, c->ev0, c->ev1));
+  c->last_blocks = (int)nblk;
+  if (d_len) HIP_TRY(hipFree(d_len));
+  return report_ray_errors(c, out, R);
+}
+
+// Rotated vectors (rotated_s1[R][3], xray[R][3]) to angles and directions, as
+// ib_test computes them (ray_tracing.c:65-80).
+static void rotated_geometry(const double* rotated_s1, const double* xray, int64_t R,
+                             int64_t max_dim_, std::vector<ReflGeom>& geom) {
+  geom.resize((size_t)std::max<int64_t>(R, 1));
+  ReflRule scratch;
+  for (int64_t i = 0; i < R; ++i)
+    host_setup_rotated(rotated_s1 + 3 * i, xray + 3 * i, max_dim_, &scratch, &geom[(size_t)i]);
+}
+
 }  // namespace anacor
 
 // ===================================================================== C-ABI
@@ -2215,6 +2588,7 @@ int anacor_ctx_stats(const anacor_ctx* ctx, anacor_stats_t* st) {
   st->workgroups = ctx->c->last_blocks;
   st->n_voxels = ctx->c->N;
   st->host_ms = ctx->c->last_host_ms;
+  st->outside_reads = ctx->c->last_oob;
   return ANACOR_OK;
 }
 
@@ -2257,6 +2631,7 @@ int anacor_batch_stats(const anacor_batch* batch, anacor_stats_t* st) {
   for (Context* c : batch->b->samples) n += c->N;
   st->n_voxels = n;
   st->host_ms = batch->b->last_host_ms;
+  st->outside_reads = 0;
   return ANACOR_OK;
 }
 
@@ -2421,6 +2796,90 @@ double ray_tracing_single_mp(int64_t* coord_list, int64_t len_coord_list,
                      label_list, shape, full_iteration, IsExp);
 }
 
+
+// ---- iterative bisection
+
+static int ib_check_num_cls(int num_cls) {
+  if (num_cls < 1) {
+    set_error(ANACOR_ERR_ARG, "num_cls must be positive");
+    return ANACOR_ERR_ARG;
+  }
+  return ANACOR_OK;
+}
+
+int anacor_ctx_bisection(anacor_ctx* ctx, const double* s1, const double* omega, int64_t R,
+                         const double raw_xray[3], const double omega_axis[3], const double F[9],
+                         int num_cls, int flags, double* out) {
+  clear_error();
+  if (!ctx || R < 0 ||
+      (R > 0 && (!s1 || !omega || !raw_xray || !omega_axis || !F || !out))) {
+    set_error(ANACOR_ERR_ARG, "bad arguments");
+    return ANACOR_ERR_ARG;
+  }
+  if (ib_check_num_cls(num_cls)) return ANACOR_ERR_ARG;
+  Context* c = ctx->c;
+  std::lock_guard<std::mutex> lk(c->mu_run);
+  HIP_TRY(hipSetDevice(c->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  HIP_TRY(c->h_rules.reserve((size_t)std::max<int64_t>(R, 1)));
+  std::vector<ReflGeom> geom((size_t)std::max<int64_t>(R, 1));
+  host_setup(s1, omega, R, raw_xray, omega_axis, F, max_dim(c), c->h_rules.data(), geom.data(),
+             host_threads(0));
+  c->last_host_ms = ms_since(t0);
+  return ib_run(c, geom.data(), R, num_cls, flags, out, nullptr);
+}
+
+int anacor_ctx_bisection_rotated(anacor_ctx* ctx, const double* rotated_s1, const double* xray,
+                                 int64_t R, int num_cls, int flags, double* out,
+                                 double* lengths) {
+  clear_error();
+  if (!ctx || R < 0 || (R > 0 && (!rotated_s1 || !xray || !out))) {
+    set_error(ANACOR_ERR_ARG, "bad arguments");
+    return ANACOR_ERR_ARG;
+  }
+  if (ib_check_num_cls(num_cls)) return ANACOR_ERR_ARG;
+  Context* c = ctx->c;
+  std::lock_guard<std::mutex> lk(c->mu_run);
+  HIP_TRY(hipSetDevice(c->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<ReflGeom> geom;
+  rotated_geometry(rotated_s1, xray, R, max_dim(c), geom);
+  c->last_host_ms = ms_since(t0);
+  return ib_run(c, geom.data(), R, num_cls, flags, out, lengths);
+}
+
+double ib_test(int64_t* coord_list, int64_t len_coord_list, double* rotated_s1, double* xray,
+               double* voxel_size, double* coefficients, int8_t*** label_list, int64_t* shape,
+               int full_iteration, int64_t store_paths, int num_cls) {
+  (void)store_paths;
+  clear_error();
+  if (!label_list || !shape || !rotated_s1 || !xray) {
+    set_error(ANACOR_ERR_ARG, "null argument");
+    return __builtin_nan("");
+  }
+  std::vector<uint8_t> storage;
+  const uint8_t* vol = flatten_table(label_list, shape, storage);
+  Context* c = nullptr;
+  if (ctx_create(vol, shape, coord_list, nullptr, len_coord_list, voxel_size, coefficients,
+                 full_iteration, 0, &c) != ANACOR_OK) {
+    fprintf(stderr, "anacor: ib_test: %s\n", g_err);
+    return __builtin_nan("");
+  }
+  double out = __builtin_nan("");
+  {
+    std::lock_guard<std::mutex> lk(c->mu_run);
+    if (hipSetDevice(c->device) == hipSuccess) {
+      std::vector<ReflGeom> geom;
+      rotated_geometry(rotated_s1, xray, 1, max_dim(c), geom);
+      const int rc = num_cls < 1 ? ib_check_num_cls(num_cls)
+                                 : ib_run(c, geom.data(), 1, num_cls, 0, &out, nullptr);
+      if (rc != ANACOR_OK && rc != ANACOR_ERR_GEOMETRY)
+        fprintf(stderr, "anacor: ib_test: %s\n", g_err);
+    }
+  }
+  ctx_destroy(c);
+  return out;
+}
 
 // ------------------------------------------------------------- gridding
 

@@ -150,6 +150,25 @@ def ray_tracing_single_mp(coord_list, len_coord_list, rotated_s1, xray, voxel_si
     return float(v)
 
 
+def ib_test(coord_list, len_coord_list, rotated_s1, xray, voxel_size, coefficients, label_list,
+            shape, full_iteration=0, store_paths=0, num_cls=3, IsExp=1) -> float:
+    """Drop-in for ib_test (ray_tracing.c:25-160): the bisection traversal of
+    one reflection given its rotated vectors.  IsExp is accepted and ignored,
+    as by the reference (its binding passes it, the C function has no such
+    parameter and always averages exp(-sum mu*L))."""
+    lib = _lib.load()
+    table = label_list if hasattr(label_list, "ptr") else LabelTable(label_list)
+    coords = np.ascontiguousarray(coord_list, dtype=np.int64)
+    v = lib.ib_test(_p(coords), int(len_coord_list), _p(_f64(rotated_s1)), _p(_f64(xray)),
+                    _p(_f64(voxel_size)), _p(_f64(coefficients)), table.ptr,
+                    _p(np.ascontiguousarray(shape, dtype=np.int64)), int(full_iteration),
+                    int(store_paths), int(num_cls))
+    code, msg = _lib.last_error(lib)
+    if code not in (_lib.ANACOR_OK, _lib.ANACOR_ERR_GEOMETRY):
+        raise _lib.AnacorError(code, msg)
+    return float(v)
+
+
 def last_error():
     return _lib.last_error()
 
@@ -240,12 +259,47 @@ class Tracer:
         _lib.check(rc, self._lib)
         return out.reshape(R, self.n_voxels)
 
+    def bisection(self, s1, omega, raw_xray, omega_axis, F, num_cls: int = 3,
+                  f64_mean: bool = False, is_exp: bool = True,
+                  strict: bool = False) -> np.ndarray:
+        """The reference's iterative-bisection traversal (ib_test after the
+        rotation of ray_tracing_overall) for every reflection in one launch."""
+        s1 = _f64(s1).reshape(-1, 3)
+        om = _f64(omega)
+        R = len(om)
+        out = np.empty(R, dtype=np.float64)
+        flags = (_lib.FLAG_F64_MEAN if f64_mean else 0) | (0 if is_exp else _lib.FLAG_NO_EXP)
+        rc = self._lib.anacor_ctx_bisection(self.handle, _p(s1), _p(om), R, _p(_f64(raw_xray)),
+                                            _p(_f64(omega_axis)), _p(_f64(F)), int(num_cls),
+                                            flags, _p(out))
+        _lib.check(rc, self._lib, allow=(_lib.ANACOR_OK,) if strict else
+                   (_lib.ANACOR_OK, _lib.ANACOR_ERR_GEOMETRY))
+        return out
+
+    def bisection_rotated(self, rotated_s1, xray, num_cls: int = 3, f64_mean: bool = False,
+                          is_exp: bool = True, lengths: bool = False):
+        """ib_test for R reflections given as rotated vectors ([R][3] each).
+        With lengths=True also returns [R, N, 2, 4] per-voxel, per-ray
+        [li, lo, cr, bu] (voxels in the order given to the constructor)."""
+        s1 = _f64(rotated_s1).reshape(-1, 3)
+        xr = _f64(xray).reshape(-1, 3)
+        if len(s1) != len(xr):
+            raise ValueError("rotated_s1 and xray differ in length")
+        R = len(s1)
+        out = np.empty(R, dtype=np.float64)
+        L = np.empty((R, self.n_voxels, 2, 4), dtype=np.float64) if lengths else None
+        flags = (_lib.FLAG_F64_MEAN if f64_mean else 0) | (0 if is_exp else _lib.FLAG_NO_EXP)
+        rc = self._lib.anacor_ctx_bisection_rotated(self.handle, _p(s1), _p(xr), R, int(num_cls),
+                                                    flags, _p(out), _p(L) if lengths else None)
+        _lib.check(rc, self._lib, allow=(_lib.ANACOR_OK, _lib.ANACOR_ERR_GEOMETRY))
+        return (out, L) if lengths else out
+
     def stats(self) -> dict:
         st = _lib.Stats()
         self._lib.anacor_ctx_stats(self.handle, ct.byref(st))
         return dict(kernel_ms=st.kernel_ms, voxel_steps=int(st.voxel_steps),
                     workgroups=int(st.workgroups), n_voxels=int(st.n_voxels),
-                    host_ms=st.host_ms)
+                    host_ms=st.host_ms, outside_reads=int(st.outside_reads))
 
     @classmethod
     def _borrow(cls, handle: int, n_voxels: int, shape) -> "Tracer":

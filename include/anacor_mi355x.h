@@ -158,6 +158,7 @@ typedef struct {
     int64_t  workgroups;     /* workgroups of the trace launch */
     int64_t  n_voxels;
     double   host_ms;        /* host setup of the reflections (rotation, angles, face rules, order) */
+    uint64_t outside_reads;  /* bisection runs: label reads outside the volume (read as vacuum) */
 } anacor_stats_t;
 int anacor_ctx_stats(const anacor_ctx *ctx, anacor_stats_t *stats);
 
@@ -247,6 +248,35 @@ int anacor_grid_run(anacor_grid *grid, const double *s1, const double *omega, in
                     int method, double *out);
 /* Duration of the map build and of the latest look-up (HIP events). */
 int anacor_grid_stats(const anacor_grid *grid, double *build_ms, double *lookup_ms);
+
+/* ------------------------------------------------- iterative bisection path */
+
+/* Replaces ib_test, AnACor/src/ray_tracing.c:25-160 with bisection.c:21-471
+ * (ctypes binding mp_worker.py:390-404, call :509-514; the binding's extra
+ * trailing IsExp argument is ignored, as by the reference): the reference's
+ * alternative traversal, which bisects each ray for its air, crystal, loop
+ * and (num_cls > 3) bubble boundaries instead of walking it; mean over the
+ * voxels of exp(-sum mu*L) for one reflection given its rotated vectors.
+ * Labels outside the volume read as vacuum (the reference reads its row table
+ * unchecked there).  NaN (ANACOR_ERR_GEOMETRY) where the reference asserts. */
+double ib_test(int64_t *coord_list, int64_t len_coord_list, double *rotated_s1, double *xray,
+               double *voxel_size, double *coefficients, int8_t ***label_list, int64_t *shape,
+               int full_iteration, int64_t store_paths, int num_cls);
+
+/* The same for R reflections in one launch on a resident volume: s1 [R][3]
+ * and omega [R] rotated as ray_tracing_overall rotates them (the loop of
+ * mp_worker.py:491-514).  flags: ANACOR_FLAG_F64_MEAN, ANACOR_FLAG_NO_EXP. */
+int anacor_ctx_bisection(anacor_ctx *ctx, const double *s1, const double *omega, int64_t R,
+                         const double raw_xray[3], const double omega_axis[3], const double F[9],
+                         int num_cls, int flags, double *out);
+
+/* On already rotated vectors (rotated_s1 [R][3], xray [R][3]), as ib_test
+ * takes them.  lengths (optional, host) receives [R][n_coords][2][4]: per
+ * voxel (in the order given to anacor_ctx_create) and ray ([0] incident, [1]
+ * diffracted) the lengths [li, lo, cr, bu] of cal_path_bisection. */
+int anacor_ctx_bisection_rotated(anacor_ctx *ctx, const double *rotated_s1, const double *xray,
+                                 int64_t R, int num_cls, int flags, double *out,
+                                 double *lengths);
 
 /* Host-side geometry of R reflections (diagnostics): theta_phi[R][4] =
  * (theta, phi) of the incident then the diffracted ray; d_table[R][2][6][3] =

@@ -563,3 +563,297 @@ void oracle_gridding_map(const int64_t *coords, int64_t n_coords, const double *
     }
 }
 
+
+/* ------------------------------------------------------ iterative bisection
+ *
+ * ib_test (ray_tracing.c:25-160) with increments, bisection,
+ * iterative_bisection and cal_path_bisection (bisection.c:21-471), resolution
+ * 1.0 (ray_tracing.c:77).  Positions are held as doubles: every value is an
+ * integer (floor) that a double holds exactly, so the differences and lengths
+ * equal the reference's int64 arithmetic.
+ *
+ * The reference indexes its row-pointer table without a bounds check; a label
+ * outside the volume reads as 0 (vacuum) here.  diag->oob counts such reads,
+ * diag->undefined those for which the reference's read is not vacuum: either
+ * its row-major wrap-around (label_list[z][y][x] == flat[(z*Y + y)*X + x]
+ * while z and z*Y + y index the tables) lands on a non-zero label, or it
+ * leaves the tables.  Where undefined == 0 the two agree.
+ */
+
+/* increments, bisection.c:21-148: the ratios (z, y, x) with which bisection
+ * steps ("z + m*rz", "y - m*ry", "x + m*rx") for exit face `face`.  Returns 0
+ * where the reference's assert fails or the face is not 1..6 (it aborts, or
+ * goes on with uninitialised ratios). */
+int oracle_ib_ratios(int64_t face, double theta, double phi, double r[3])
+{
+    const double at = fabs(theta), ap = fabs(phi);
+    const int narrow = at < ORACLE_PI / 2;
+    switch (face) {
+    case 4:                                   /* :39-61, y -> 0 */
+        if (!(theta > 0)) return 0;
+        if (narrow) {
+            r[2] = -cos(ap) / tan(at);
+            r[0] = sin(phi) / tan(at);
+        } else {
+            r[2] = cos(ap) / tan(ORACLE_PI - at);
+            r[0] = sin(-phi) / tan(ORACLE_PI - at);
+        }
+        r[1] = 1;
+        return 1;
+    case 5:                                   /* :62-84, y -> y_max */
+        if (!(theta < 0)) return 0;
+        if (narrow) {
+            r[2] = -cos(ap) / tan(at);
+            r[0] = sin(phi) / tan(at);
+        } else {
+            r[2] = cos(ap) / tan(ORACLE_PI - at);
+            r[0] = -sin(phi) / tan(ORACLE_PI - at);
+        }
+        r[1] = -1;
+        return 1;
+    case 6:                                   /* :85-95, x -> 0 */
+        if (!(at <= ORACLE_PI / 2)) return 0;
+        r[2] = -1;
+        r[1] = tan(theta) / cos(phi);
+        r[0] = tan(phi);
+        return 1;
+    case 1:                                   /* :96-106, x -> x_max */
+        if (!(at >= ORACLE_PI / 2)) return 0;
+        r[2] = 1;
+        r[1] = tan(ORACLE_PI - theta) / cos(ap);
+        r[0] = -tan(phi);
+        return 1;
+    case 2:                                   /* :107-124, z -> 0 */
+    case 3:                                   /* :125-142, z -> z_max */
+        if (narrow) {
+            if (face == 2 ? !(phi < 0) : !(phi > 0)) return 0;
+            r[2] = -1 / tan(ap);
+            r[1] = tan(theta) / sin(ap);
+        } else {
+            if (face == 2 ? !(phi > 0) : !(phi < 0)) return 0;
+            r[2] = 1 / tan(ap);
+            r[1] = tan(ORACLE_PI - theta) / sin(ap);
+        }
+        r[0] = face == 2 ? -1 : 1;
+        return 1;
+    default:
+        return 0;
+    }
+}
+
+/* AirLongest of increments: the distance, in steps of the face axis, from
+ * the voxel to the last plane before the face (extent - 1 at the top). */
+static double ib_air_longest(int64_t face, const int64_t c[3], const int64_t shape[3])
+{
+    switch (face) {
+    case 4: return (double)c[1];
+    case 5: return (double)(shape[1] - 1 - c[1]);
+    case 6: return (double)c[2];
+    case 1: return (double)(shape[2] - 1 - c[2]);
+    case 2: return (double)c[0];
+    default: return (double)(shape[0] - 1 - c[0]);
+    }
+}
+
+static int ib_label(const int8_t *vol, const int64_t shape[3], double z, double y, double x,
+                    oracle_ib_diag_t *diag)
+{
+    const double Z = (double)shape[0], Y = (double)shape[1], X = (double)shape[2];
+    if (z >= 0 && z < Z && y >= 0 && y < Y && x >= 0 && x < X)
+        return vol[((int64_t)z * shape[1] + (int64_t)y) * shape[2] + (int64_t)x];
+    if (diag) {
+        diag->oob++;
+        int defined = 0, v = 0;
+        if (z >= 0 && z < Z) {
+            const double row = z * Y + y;
+            const double flat = row * X + x;
+            if (row >= 0 && row < Z * Y && flat >= 0 && flat < Z * Y * X) {
+                defined = 1;
+                v = vol[(int64_t)flat];
+            }
+        }
+        if (!defined || v != 0) diag->undefined++;
+    }
+    return 0;
+}
+
+typedef struct { double p[3]; double mid; } ib_point_t;
+
+/* bisection, bisection.c:156-235: halve [shortest, longest] until it is at
+ * most one step wide.  `outer` = 0 moves `longest` in while the sample point
+ * holds class `cls` (the point where cls begins, seen from outside); outer = 1
+ * moves `shortest` out while it does (where cls ends, seen from the voxel).
+ * The point returned is the voxel plus floor(mid) steps. */
+static ib_point_t ib_search(double longest, double shortest, const double r[3],
+                            const int64_t c[3], const int8_t *vol, const int64_t shape[3],
+                            int outer, int cls, oracle_ib_diag_t *diag)
+{
+    const double z = (double)c[0], y = (double)c[1], x = (double)c[2];
+    double diff = longest - shortest;
+    double mid = (longest + shortest) / 2.0;
+    while (diff > 1.0) {
+        mid = (longest + shortest) / 2.0;
+        const int label = ib_label(vol, shape, floor(z + mid * r[0]), floor(y - mid * r[1]),
+                                   floor(x + mid * r[2]), diag);
+        const int hit = (int8_t)label == cls;
+        if (hit == outer)
+            shortest = mid;
+        else
+            longest = mid;
+        diff = longest - shortest;
+        if (diag) diag->reads++;
+    }
+    const double m = floor(mid);
+    ib_point_t out = {{floor(z + m * r[0]), floor(y - m * r[1]), floor(x + m * r[2])}, mid};
+    return out;
+}
+
+/* |a - b|_1 summed z, y, x as the reference's three fabs terms. */
+static double ib_l1(const double *a, const double *b)
+{
+    return fabs(a[0] - b[0]) + fabs(a[1] - b[1]) + fabs(a[2] - b[2]);
+}
+
+/* difference_length, bisection.c:244-253 (pow(v, 2) folded to v*v). */
+static double ib_dist(const double *a, const double *b, const double *vs)
+{
+    const double dz = (a[0] - b[0]) * vs[0];
+    const double dy = (a[1] - b[1]) * vs[1];
+    const double dx = (a[2] - b[2]) * vs[2];
+    return sqrt(dz * dz + dy * dy + dx * dx);
+}
+
+/* iterative_bisection (bisection.c:298-471) then cal_path_bisection
+ * (:255-296) for one ray: L = [li, lo, cr, bu].  Returns an ORACLE_* code. */
+static int ib_ray(double theta, double phi, const int64_t c[3], int64_t face,
+                  const int8_t *vol, const int64_t shape[3], int num_cls,
+                  const double *vs, double L[4], oracle_ib_diag_t *diag)
+{
+    double r[3];
+    if (face == 0) return ORACLE_ERR_FACE;
+    if (!oracle_ib_ratios(face, theta, phi, r)) return ORACLE_ERR_ASSERT;
+    double pts[7][3];
+    int n = 1;
+    pts[0][0] = (double)c[0];
+    pts[0][1] = (double)c[1];
+    pts[0][2] = (double)c[2];
+    const ib_point_t A = ib_search(ib_air_longest(face, c, shape), 0, r, c, vol, shape, 0, 0,
+                                   diag);
+    const ib_point_t C = ib_search(A.mid, 0, r, c, vol, shape, 1, 3, diag);
+    memcpy(pts[n++], A.p, sizeof(A.p));
+    memcpy(pts[n++], C.p, sizeof(C.p));
+    do {
+        /* loop (class 2): inner boundary, then the outer one; a pair closer
+         * than two voxels per axis on average ends the search (:355-402) */
+        const ib_point_t Li = ib_search(A.mid, C.mid, r, c, vol, shape, 0, 2, diag);
+        if (ib_l1(Li.p, A.p) / 3 > 1) {
+            const ib_point_t Lo = ib_search(A.mid, Li.mid, r, c, vol, shape, 1, 2, diag);
+            if (ib_l1(Li.p, Lo.p) / 3 <= 2) break;
+            memcpy(pts[n++], Li.p, sizeof(Li.p));
+            memcpy(pts[n++], Lo.p, sizeof(Lo.p));
+        }
+        /* bubble (class 4), num_cls > 3 only; its first test has no /3
+         * (:405-448) */
+        if (num_cls > 3) {
+            const ib_point_t Bi = ib_search(A.mid, C.mid, r, c, vol, shape, 0, 4, diag);
+            if (ib_l1(Bi.p, A.p) > 1) {
+                const ib_point_t Bo = ib_search(A.mid, Bi.mid, r, c, vol, shape, 1, 4, diag);
+                if (ib_l1(Bi.p, Bo.p) / 3 <= 2) break;
+                memcpy(pts[n++], Bi.p, sizeof(Bi.p));
+                memcpy(pts[n++], Bo.p, sizeof(Bo.p));
+            }
+        }
+    } while (0);
+    /* cal_path_bisection: with seven points the loop length stays 0 and the
+     * bubble pair is measured; with five, whichever pair was found is taken
+     * as the loop */
+    const double total = ib_dist(pts[0], pts[1], vs);
+    const double cr = ib_dist(pts[0], pts[2], vs);
+    double lo = 0, bu = 0;
+    const double air = 0;
+    if (n == 5)
+        lo = ib_dist(pts[3], pts[4], vs);
+    else if (n == 7)
+        bu = ib_dist(pts[5], pts[6], vs);
+    L[0] = total - lo - bu - cr - air;
+    L[1] = lo;
+    L[2] = cr;
+    L[3] = bu;
+    return ORACLE_OK;
+}
+
+static double ib_core(const int64_t *coords, int64_t n, const double theta[2],
+                      const double phi[2], const double dir[2][3], const double *vs,
+                      const double *mu, const int8_t *vol, const int64_t shape[3],
+                      int num_cls, int is_exp, double *lengths, oracle_ib_diag_t *diag,
+                      int *err)
+{
+    double sum = 0.0;
+    *err = ORACLE_OK;
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t c[3] = {coords[i * 3], coords[i * 3 + 1], coords[i * 3 + 2]};
+        const int64_t face[2] = {oracle_cube_face(c, dir[0], shape, 1),
+                                 oracle_cube_face(c, dir[1], shape, 0)};
+        double L[2][4];
+        for (int k = 0; k < 2; k++) {
+            const int e = ib_ray(theta[k], phi[k], c, face[k], vol, shape, num_cls, vs, L[k],
+                                 diag);
+            if (e) {
+                *err = e;
+                return NAN;
+            }
+        }
+        if (lengths) memcpy(lengths + i * 8, L, sizeof(L));
+        sum += rate(L[0], L[1], mu, is_exp);
+    }
+    return sum / n;
+}
+
+double oracle_ib_test(const int64_t *coords, int64_t n_coords, const double rotated_s1[3],
+                      const double xray[3], const double *voxel_size,
+                      const double *coefficients, const uint8_t *volume,
+                      const int64_t shape[3], int num_cls, int is_exp, double *lengths,
+                      oracle_ib_diag_t *diag)
+{
+    double theta[2], phi[2], dir[2][3];
+    oracle_thetaphi(xray, 1, &theta[0], &phi[0]);
+    oracle_thetaphi(rotated_s1, 0, &theta[1], &phi[1]);
+    oracle_myframe(xray, dir[0]);
+    oracle_myframe(rotated_s1, dir[1]);
+    int err = 0;
+    return ib_core(coords, n_coords, theta, phi, (const double(*)[3])dir, voxel_size,
+                   coefficients, (const int8_t *)volume, shape, num_cls, is_exp, lengths, diag,
+                   &err);
+}
+
+int64_t oracle_ib_overall(const int64_t *coords, int64_t n_coords, const double *s1,
+                          const double *omega, int64_t R, const double *raw_xray,
+                          const double *omega_axis, const double *F, const double *voxel_size,
+                          const double *coefficients, const uint8_t *volume,
+                          const int64_t shape[3], int num_cls, int is_exp, int n_threads,
+                          double *result, oracle_ib_diag_t *diag)
+{
+    int64_t failed = 0, oob = 0, undef = 0, reads = 0;
+    if (n_threads < 1) n_threads = 1;
+#pragma omp parallel for num_threads(n_threads) schedule(dynamic, 1) \
+    reduction(+ : failed, oob, undef, reads)
+    for (int64_t i = 0; i < R; i++) {
+        oracle_refl_t g;
+        setup_one(&s1[i * 3], omega[i], raw_xray, omega_axis, F, &g);
+        oracle_ib_diag_t d = {0, 0, 0};
+        int err = 0;
+        result[i] = ib_core(coords, n_coords, g.theta, g.phi, (const double(*)[3])g.dir,
+                            voxel_size, coefficients, (const int8_t *)volume, shape, num_cls,
+                            is_exp, NULL, &d, &err);
+        if (err) failed++;
+        oob += d.oob;
+        undef += d.undefined;
+        reads += d.reads;
+    }
+    if (diag) {
+        diag->oob = oob;
+        diag->undefined = undef;
+        diag->reads = reads;
+    }
+    return failed;
+}

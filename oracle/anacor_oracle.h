@@ -98,6 +98,34 @@ double oracle_single(const int64_t *coords, int64_t n_coords,
                      const uint8_t *volume, const int64_t shape[3],
                      int full_iteration, int is_exp, oracle_ray_t *rays, int64_t *steps);
 
+/*
+ * Iterative bisection (ib_test, ray_tracing.c:25-160; bisection.c:21-471).
+ * Labels outside the volume read as 0.  Diagnostics: label reads made, reads
+ * outside the volume, and those of them whose value in the reference (its
+ * unchecked row-pointer table) would not be 0 or is not defined.
+ */
+typedef struct {
+    int64_t reads;
+    int64_t oob;
+    int64_t undefined;
+} oracle_ib_diag_t;
+
+int     oracle_ib_ratios(int64_t face, double theta, double phi, double r[3]);
+/* ib_test on already rotated vectors; lengths (optional) [n_coords][2][4]. */
+double  oracle_ib_test(const int64_t *coords, int64_t n_coords, const double rotated_s1[3],
+                       const double xray[3], const double *voxel_size,
+                       const double *coefficients, const uint8_t *volume,
+                       const int64_t shape[3], int num_cls, int is_exp, double *lengths,
+                       oracle_ib_diag_t *diag);
+/* Rotation (ray_tracing.c:393-407) then ib_test per reflection; returns the
+ * number of reflections on which the reference aborts (result NaN). */
+int64_t oracle_ib_overall(const int64_t *coords, int64_t n_coords, const double *s1,
+                          const double *omega, int64_t R, const double *raw_xray,
+                          const double *omega_axis, const double *F, const double *voxel_size,
+                          const double *coefficients, const uint8_t *volume,
+                          const int64_t shape[3], int num_cls, int is_exp, int n_threads,
+                          double *result, oracle_ib_diag_t *diag);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,10 @@ REFL_DTYPE = np.dtype([("xray", np.float64, 3), ("s1", np.float64, 3), ("theta",
                        ("phi", np.float64, 2), ("dir", np.float64, (2, 3))])
 
 
+class IbDiag(ct.Structure):
+    _fields_ = [("reads", ct.c_int64), ("oob", ct.c_int64), ("undefined", ct.c_int64)]
+
+
 class OracleRule(ct.Structure):
     _fields_ = [("d", ct.c_double * 3), ("rounding", ct.c_int32), ("axis", ct.c_int32),
                 ("to_max", ct.c_int32), ("inclusive", ct.c_int32), ("valid", ct.c_int32),
@@ -99,7 +103,67 @@ class Oracle:
         lib.oracle_gridding_map.argtypes = [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p,
                                             ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                             ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p]
+        lib.oracle_ib_ratios.restype = ct.c_int
+        lib.oracle_ib_ratios.argtypes = [ct.c_int64, ct.c_double, ct.c_double, ct.c_void_p]
+        lib.oracle_ib_test.restype = ct.c_double
+        lib.oracle_ib_test.argtypes = [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p,
+                                       ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                       ct.c_int, ct.c_int, ct.c_void_p, ct.POINTER(IbDiag)]
+        lib.oracle_ib_overall.restype = ct.c_int64
+        lib.oracle_ib_overall.argtypes = [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p,
+                                          ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                          ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                          ct.c_int, ct.c_int, ct.c_int, ct.c_void_p,
+                                          ct.POINTER(IbDiag)]
         self.lib = lib
+
+    def ib_ratios(self, face, theta, phi):
+        """increments (bisection.c:21-148): (rz, ry, rx) or None where the
+        reference's assert fails."""
+        r = np.zeros(3)
+        ok = self.lib.oracle_ib_ratios(int(face), float(theta), float(phi), _ptr(r))
+        return r if ok else None
+
+    def ib_test(self, wl, coords, rotated_s1, xray, num_cls=3, is_exp=1, lengths=False):
+        """ib_test (ray_tracing.c:25-160) on rotated vectors.  Returns
+        (mean, diag dict[, lengths [n][2][4]])."""
+        coords = np.ascontiguousarray(coords, dtype=np.int64)
+        vol = np.ascontiguousarray(wl.volume).view(np.uint8)
+        shape = np.array(vol.shape, dtype=np.int64)
+        L = np.zeros((len(coords), 2, 4)) if lengths else None
+        d = IbDiag()
+        m = self.lib.oracle_ib_test(_ptr(coords), len(coords),
+                                    _ptr(np.ascontiguousarray(rotated_s1, dtype=np.float64)),
+                                    _ptr(np.ascontiguousarray(xray, dtype=np.float64)),
+                                    _ptr(np.ascontiguousarray(wl.voxel_size, dtype=np.float64)),
+                                    _ptr(np.ascontiguousarray(wl.coefficients, dtype=np.float64)),
+                                    _ptr(vol), _ptr(shape), int(num_cls), int(is_exp),
+                                    _ptr(L) if lengths else None, ct.byref(d))
+        diag = dict(reads=d.reads, oob=d.oob, undefined=d.undefined)
+        return (m, diag, L) if lengths else (m, diag)
+
+    def ib_overall(self, wl, num_cls=3, is_exp=1, n_threads=0, R=None):
+        """Rotation then ib_test for every reflection of a Workload."""
+        s1 = np.ascontiguousarray(wl.s1, dtype=np.float64)
+        om = np.ascontiguousarray(wl.omega, dtype=np.float64)
+        if R is not None:
+            s1, om = s1[:R], om[:R]
+        coords = np.ascontiguousarray(wl.coords, dtype=np.int64)
+        vol = np.ascontiguousarray(wl.volume).view(np.uint8)
+        shape = np.array(vol.shape, dtype=np.int64)
+        out = np.empty(len(om))
+        d = IbDiag()
+        failed = self.lib.oracle_ib_overall(
+            _ptr(coords), len(coords), _ptr(s1), _ptr(om), len(om),
+            _ptr(np.ascontiguousarray(wl.raw_xray, dtype=np.float64)),
+            _ptr(np.ascontiguousarray(wl.omega_axis, dtype=np.float64)),
+            _ptr(np.ascontiguousarray(wl.F, dtype=np.float64).reshape(-1)),
+            _ptr(np.ascontiguousarray(wl.voxel_size, dtype=np.float64)),
+            _ptr(np.ascontiguousarray(wl.coefficients, dtype=np.float64)), _ptr(vol), _ptr(shape),
+            int(num_cls), int(is_exp), int(n_threads or os.cpu_count() or 1), _ptr(out),
+            ct.byref(d))
+        return dict(result=out, failed=int(failed), reads=d.reads, oob=d.oob,
+                    undefined=d.undefined)
 
     def trace(self, wl, n_threads: int = 0, rays: bool = False, R: int | None = None):
         """A_h per reflection (and optionally per-ray records) for a Workload."""
@@ -257,7 +321,30 @@ class Reference:
                                                     ct.c_double, ct.c_void_p, ct.c_void_p,
                                                     ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int,
                                                     ct.c_int]
+        lib.ib_test.restype = ct.c_double
+        lib.ib_test.argtypes = [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int64,
+                                ct.c_int]
         self.lib = lib
+
+    def ib_test(self, wl, coords, rotated_s1, xray, num_cls: int = 3, table=None) -> float:
+        """The reference's ib_test (ray_tracing.c:25-160).  It reads labels
+        without bounds checks and asserts on inconsistent angles: call it in a
+        process that may die."""
+        keep = None
+        if table is None:
+            table, keep = label_table(np.ascontiguousarray(wl.volume).view(np.int8))
+        shape = np.array(np.asarray(wl.volume).shape, dtype=np.int64)
+        coords = np.ascontiguousarray(coords, dtype=np.int64)
+        with _stdout_silenced():
+            v = self.lib.ib_test(_ptr(coords), len(coords),
+                                 _ptr(np.ascontiguousarray(rotated_s1, dtype=np.float64)),
+                                 _ptr(np.ascontiguousarray(xray, dtype=np.float64)),
+                                 _ptr(np.ascontiguousarray(wl.voxel_size, dtype=np.float64)),
+                                 _ptr(np.ascontiguousarray(wl.coefficients, dtype=np.float64)),
+                                 table, _ptr(shape), int(wl.full_iteration), 0, int(num_cls))
+        del keep
+        return float(v)
 
     def single_gridding(self, wl, coord, rotated_s1, theta, phi, is_exp: int = 0,
                         table=None) -> float:

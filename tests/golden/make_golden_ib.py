@@ -1,0 +1,159 @@
+"""Write tests/golden/golden_ib.npz from the reference's own C code.
+
+The reference's bisection traversal -- ``ib_test`` (ray_tracing.c:25-160) and,
+per ray, ``iterative_bisection`` + ``cal_path_bisection`` (bisection.c:
+255-471) -- compiled in place by ``make -C oracle ref`` into oracle/_ref, on a
+64^3 volume of all five classes: the 64^3 plain construction with an 8-voxel
+loop slab and three bubbles (class 4) in the solvent.  For each of 24
+reflections (rotated as ray_tracing_overall rotates them) and num_cls 3 and
+4: the mean over all crystal voxels, and the per-ray lengths [li, lo, cr, bu]
+with the number of path points for every 8th voxel.  Each reflection runs in
+a subprocess first, because the reference asserts (and reads its label table
+without bounds checks) on some geometries; one that dies is recorded as such.
+
+Usage:  python tests/golden/make_golden_ib.py   (needs /root/reference)
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from anacor_public_amd import workload as W  # noqa: E402
+from oracle.oracle import REF_SO, Oracle, build, label_table  # noqa: E402
+
+R = 24
+NUM_CLS = (3, 4)
+VOXEL_STRIDE = 8
+COEFFICIENTS = np.array([0.0123, 0.0117, 0.0145, 0.0201])
+
+
+def golden_volume() -> np.ndarray:
+    vol = W.make_volume(W.VolumeSpec((64, 64, 64), 28.0, 8, (12.0, 9.0, 7.0), 1))
+    zz, yy, xx = np.meshgrid(np.arange(64), np.arange(64), np.arange(64), indexing="ij")
+    for cz, cy, cx, r in ((20, 32, 14, 5.0), (44, 20, 46, 4.0), (32, 52, 40, 6.0)):
+        inside = ((zz - cz) ** 2 + (yy - cy) ** 2 + (xx - cx) ** 2 <= r * r) & (vol == 1)
+        vol[inside] = 4
+    return vol
+
+
+def case():
+    vol = golden_volume()
+    coords = W.crystal_voxels(vol)
+    s1, omega = W.make_reflections(R, 11)
+    geo = Oracle().reflection_setup(s1, omega, W.RAW_XRAY, W.OMEGA_AXIS, W.F_IDENTITY)
+    return vol, coords, s1, omega, geo
+
+
+class PathIB(ct.Structure):
+    _fields_ = [("path", ct.POINTER(ct.c_int64)), ("classes", ct.POINTER(ct.c_int8)),
+                ("boundary_list", ct.POINTER(ct.c_int64)), ("length", ct.c_int64)]
+
+
+class ThetaPhi(ct.Structure):
+    _fields_ = [("theta", ct.c_double), ("phi", ct.c_double)]
+
+
+def reference_lib():
+    lib = ct.CDLL(REF_SO)
+    vp = ct.c_void_p
+    lib.ib_test.restype = ct.c_double
+    lib.ib_test.argtypes = [vp, ct.c_int64, vp, vp, vp, vp, vp, vp, ct.c_int, ct.c_int64, ct.c_int]
+    lib.iterative_bisection.restype = PathIB
+    lib.iterative_bisection.argtypes = [ct.c_double, ct.c_double, vp, ct.c_int, vp, vp,
+                                        ct.c_double, ct.c_int]
+    lib.cal_path_bisection.restype = ct.POINTER(ct.c_double)
+    lib.cal_path_bisection.argtypes = [PathIB, vp]
+    lib.dials_2_thetaphi_22.restype = ThetaPhi
+    lib.dials_2_thetaphi_22.argtypes = [vp, ct.c_int64]
+    lib.dials_2_myframe.restype = None
+    lib.dials_2_myframe.argtypes = [vp, vp]
+    lib.cube_face.restype = ct.c_int64
+    lib.cube_face.argtypes = [vp, vp, vp, ct.c_int]
+    return lib
+
+
+def one_reflection(i: int, out_path: str) -> None:
+    """Everything the fixture holds for reflection i, with the reference."""
+    vol, coords, _, _, geo = case()
+    lib = reference_lib()
+    libc = ct.CDLL(None)
+    libc.free.argtypes = [ct.c_void_p]
+    table, keep = label_table(np.ascontiguousarray(vol).view(np.int8))
+    shape = np.array(vol.shape, dtype=np.int64)
+    vs = np.ones(3)
+    mu = COEFFICIENTS.copy()
+    s1r = np.ascontiguousarray(geo["s1"][i])
+    xr = np.ascontiguousarray(geo["xray"][i])
+    c = np.ascontiguousarray(coords, dtype=np.int64)
+    p = lambda a: a.ctypes.data_as(ct.c_void_p)  # noqa: E731
+    means = [lib.ib_test(p(c), len(c), p(s1r.copy()), p(xr.copy()), p(vs), p(mu), table, p(shape),
+                         0, 0, n) for n in NUM_CLS]
+    # per ray: angles and directions exactly as ib_test derives them (:65-80)
+    tp = [lib.dials_2_thetaphi_22(p(xr.copy()), 1), lib.dials_2_thetaphi_22(p(s1r.copy()), 0)]
+    dirs = [np.zeros(3), np.zeros(3)]
+    lib.dials_2_myframe(p(xr.copy()), p(dirs[0]))
+    lib.dials_2_myframe(p(s1r.copy()), p(dirs[1]))
+    sub = np.arange(0, len(c), VOXEL_STRIDE)
+    L = np.zeros((len(NUM_CLS), len(sub), 2, 4))
+    npts = np.zeros((len(NUM_CLS), len(sub), 2), dtype=np.int64)
+    for j, v in enumerate(sub):
+        cv = np.ascontiguousarray(c[v])
+        faces = [lib.cube_face(p(cv), p(dirs[0]), p(shape), 1),
+                 lib.cube_face(p(cv), p(dirs[1]), p(shape), 0)]
+        for a, n in enumerate(NUM_CLS):
+            for k in range(2):
+                path = lib.iterative_bisection(tp[k].theta, tp[k].phi, p(cv), int(faces[k]), table,
+                                               p(shape), 1.0, n)
+                nums = lib.cal_path_bisection(path, p(vs))
+                L[a, j, k] = np.ctypeslib.as_array(nums, shape=(4,))
+                npts[a, j, k] = path.length
+                for ptr in (nums, path.path, path.classes, path.boundary_list):
+                    libc.free(ct.cast(ptr, ct.c_void_p))
+    del keep
+    np.savez(out_path, means=np.array(means), L=L, npts=npts)
+
+
+def main() -> None:
+    build(with_reference=True)
+    vol, coords, s1, omega, geo = case()
+    means = np.full((len(NUM_CLS), R), np.nan)
+    sub = np.arange(0, len(coords), VOXEL_STRIDE)
+    L = np.full((len(NUM_CLS), R, len(sub), 2, 4), np.nan)
+    npts = np.zeros((len(NUM_CLS), R, len(sub), 2), dtype=np.int64)
+    ok = np.zeros(R, dtype=bool)
+    with tempfile.TemporaryDirectory() as tmp:
+        for i in range(R):
+            f = os.path.join(tmp, f"r{i}.npz")
+            rc = subprocess.run([sys.executable, __file__, "--one", str(i), f],
+                                capture_output=True).returncode
+            if rc != 0:
+                print(f"reflection {i}: the reference died (rc {rc})")
+                continue
+            d = np.load(f)
+            ok[i] = True
+            means[:, i] = d["means"]
+            L[:, i] = d["L"]
+            npts[:, i] = d["npts"]
+    print(f"{int(ok.sum())} of {R} reflections; path points:",
+          {int(k): int(v) for k, v in zip(*np.unique(npts[:, ok], return_counts=True))})
+    np.savez_compressed(os.path.join(HERE, "golden_ib.npz"), volume=vol, coords=coords,
+                        coefficients=COEFFICIENTS, s1=s1, omega=omega, raw_xray=W.RAW_XRAY,
+                        omega_axis=W.OMEGA_AXIS, F=W.F_IDENTITY, rotated_s1=geo["s1"],
+                        xray=geo["xray"], num_cls=np.array(NUM_CLS), ok=ok, means=means,
+                        voxel_subset=sub, lengths=L, n_points=npts)
+
+
+if __name__ == "__main__":
+    if len(sys.argv) > 3 and sys.argv[1] == "--one":
+        one_reflection(int(sys.argv[2]), sys.argv[3])
+        sys.exit(0)
+    main()
