@@ -2163,15 +2163,18 @@ struct IbArgs {
 };
 
 // Label at integer-valued FP64 position (z, y, x); 0 outside the volume.
+// The positions are finite or infinite, never NaN (finite voxel, positive
+// mid, finite or infinite ratio), so the saturating conversion keeps every
+// outside position outside and the range test runs on integers.
 template <int LAYOUT>
 __device__ __forceinline__ int ib_label(const Volume& V, double z, double y, double x,
                                         unsigned& oob) {
-  if (!(z >= 0.0 && z < (double)V.Z && y >= 0.0 && y < (double)V.Y && x >= 0.0 &&
-        x < (double)V.X)) {
+  const int iz = cvt_trunc_sat(z), iy = cvt_trunc_sat(y), ix = cvt_trunc_sat(x);
+  if (!((unsigned)iz < (unsigned)V.Z && (unsigned)iy < (unsigned)V.Y &&
+        (unsigned)ix < (unsigned)V.X)) {
     ++oob;
     return 0;
   }
-  const int iz = (int)z, iy = (int)y, ix = (int)x;
   if (LAYOUT == kLayoutRaw || LAYOUT == kLayoutWide)
     return load_label(V, (iz * V.Y + iy) * V.X + ix);
   const int f = iz * 64 + (iz >> 3) * V.cz + iy * 8 + (iy >> 3) * V.cy + ix + (ix >> 3) * 504;
@@ -2226,49 +2229,63 @@ __device__ __forceinline__ double ib_dist(const double a[3], const double b[3],
 }
 
 // iterative_bisection (bisection.c:298-471) and cal_path_bisection (:255-296)
-// of one ray from voxel c: L = [li, lo, cr, bu].
+// of one ray from voxel c: L = [li, lo, cr, bu].  Each boundary pair is
+// measured as soon as it is found, so only its length stays live.
 template <int LAYOUT>
 __device__ __forceinline__ void ib_ray(const Volume& V, const double c[3], const double r[3],
                                        double air_longest, int num_cls, const double vs[3],
                                        double L[4], unsigned& reads, unsigned& oob) {
   const IbPoint A = ib_search<LAYOUT>(V, c, r, air_longest, 0.0, false, 0, reads, oob);
-  const IbPoint C = ib_search<LAYOUT>(V, c, r, A.mid, 0.0, true, 3, reads, oob);
-  // the pairs appended after (voxel, A, C): at most two
-  double q[4][3];
-  int n = 0;
+  const double total = ib_dist(c, A.p, vs);
+  double cr;
+  double cmid;
+  {
+    const IbPoint C = ib_search<LAYOUT>(V, c, r, A.mid, 0.0, true, 3, reads, oob);
+    cr = ib_dist(c, C.p, vs);
+    cmid = C.mid;
+  }
+  // lengths of the pairs appended after (voxel, A, C), in order: at most two
+  double pair0 = 0.0, pair1 = 0.0;
+  int pairs = 0;
   do {
-    const IbPoint Li = ib_search<LAYOUT>(V, c, r, A.mid, C.mid, false, 2, reads, oob);
+    const IbPoint Li = ib_search<LAYOUT>(V, c, r, A.mid, cmid, false, 2, reads, oob);
     if (ib_l1(Li.p, A.p) / 3 > 1) {
       const IbPoint Lo = ib_search<LAYOUT>(V, c, r, A.mid, Li.mid, true, 2, reads, oob);
       if (ib_l1(Li.p, Lo.p) / 3 <= 2) break;     // a segmentation artefact ends the search
-      for (int a = 0; a < 3; ++a) { q[n][a] = Li.p[a]; q[n + 1][a] = Lo.p[a]; }
-      n += 2;
+      pair0 = ib_dist(Li.p, Lo.p, vs);
+      pairs = 1;
     }
     if (num_cls > 3) {
-      const IbPoint Bi = ib_search<LAYOUT>(V, c, r, A.mid, C.mid, false, 4, reads, oob);
+      const IbPoint Bi = ib_search<LAYOUT>(V, c, r, A.mid, cmid, false, 4, reads, oob);
       if (ib_l1(Bi.p, A.p) > 1) {
         const IbPoint Bo = ib_search<LAYOUT>(V, c, r, A.mid, Bi.mid, true, 4, reads, oob);
         if (ib_l1(Bi.p, Bo.p) / 3 <= 2) break;
-        for (int a = 0; a < 3; ++a) { q[n][a] = Bi.p[a]; q[n + 1][a] = Bo.p[a]; }
-        n += 2;
+        const double d = ib_dist(Bi.p, Bo.p, vs);
+        if (pairs == 0) pair0 = d;
+        else pair1 = d;
+        ++pairs;
       }
     }
   } while (false);
   // with one pair it is measured as the loop, with two only the second, as
   // the bubble (cal_path_bisection reads points 3-4 or 5-6)
-  const double total = ib_dist(c, A.p, vs);
-  const double cr = ib_dist(c, C.p, vs);
-  double lo = 0.0, bu = 0.0;
-  if (n == 2) lo = ib_dist(q[0], q[1], vs);
-  else if (n == 4) bu = ib_dist(q[2], q[3], vs);
+  const double lo = pairs == 1 ? pair0 : 0.0;
+  const double bu = pairs == 2 ? pair1 : 0.0;
   L[0] = total - lo - bu - cr - 0.0;
   L[1] = lo;
   L[2] = cr;
   L[3] = bu;
 }
 
+// Two lanes per (reflection, voxel): the even lane bisects the incident ray,
+// the odd lane the diffracted one, so each lane's chain of dependent gathers
+// is one ray long and a wave keeps twice as many in flight.  The pair swaps
+// its four lengths (lane ^ 1) and the even lane applies cal_rate.
+// Six waves per SIMD (at most 80 VGPRs, no spills): the kernel waits on
+// gathers, so resident waves matter more than registers.
 template <int LAYOUT, bool F64, bool LENGTHS>
-__global__ __launch_bounds__(kBlock) void ib_kernel(IbArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
+void ib_kernel(IbArgs a) {
   __shared__ ReduceLds red;
   const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
   if (lb >= a.n_blocks) return;
@@ -2276,24 +2293,24 @@ __global__ __launch_bounds__(kBlock) void ib_kernel(IbArgs a) {
   const int tile = lb - pos * a.tiles;
   const int refl = a.order[pos];
   const Volume V = make_volume(a.vol, a.Z, a.Y, a.X, a.vol_bytes, a.cz, a.cy);
-  const IbRule* rule = a.rules + refl;
+  const int k = (int)threadIdx.x & 1;
+  const IbRay& ray = a.rules[refl].ray[k];
   TileAcc acc = {0.f, 0.f, 0.0, 0ull, 0u};
   unsigned oob = 0;
   const int v_begin = tile * a.vox_per_tile;
   const int v_end = min(a.N, v_begin + a.vox_per_tile);
-  for (int v = v_begin + (int)threadIdx.x; v < v_end; v += kBlock) {
+  for (int v = v_begin + ((int)threadIdx.x >> 1); v < v_end; v += kBlock / 2) {
     const int4 co = a.coords[v];
     const double c[3] = {(double)co.x, (double)co.y, (double)co.z};
-    double L[2][4];
+    double L[4] = {0.0, 0.0, 0.0, 0.0};
     unsigned verr = 0, reads = 0;
-#pragma unroll 1
-    for (int k = 0; k < 2; ++k) {
-      const IbRay& ray = rule->ray[k];
-      const int face = exit_face(co.x, co.y, co.z, ray.dir[0], ray.dir[1], ray.dir[2], a.Z, a.Y,
-                                 a.X, ray.neg != 0);
-      L[k][0] = L[k][1] = L[k][2] = L[k][3] = 0.0;
-      if (face == 0) { verr |= ANACOR_RAY_ERR_FACE; continue; }
-      if (!((ray.valid >> (face - 1)) & 1u)) { verr |= ANACOR_RAY_ERR_ASSERT; continue; }
+    const int face = exit_face(co.x, co.y, co.z, ray.dir[0], ray.dir[1], ray.dir[2], a.Z, a.Y,
+                               a.X, ray.neg != 0);
+    if (face == 0) {
+      verr = ANACOR_RAY_ERR_FACE;
+    } else if (!((ray.valid >> (face - 1)) & 1u)) {
+      verr = ANACOR_RAY_ERR_ASSERT;
+    } else {
       // AirLongest of increments: steps from the voxel to the face's last plane
       double air;
       switch (face) {
@@ -2305,28 +2322,34 @@ __global__ __launch_bounds__(kBlock) void ib_kernel(IbArgs a) {
         default: air = (double)(a.Z - 1 - co.x); break;
       }
       const double r[3] = {ray.r[face - 1][0], ray.r[face - 1][1], ray.r[face - 1][2]};
-      ib_ray<LAYOUT>(V, c, r, air, a.num_cls, a.vs, L[k], reads, oob);
+      ib_ray<LAYOUT>(V, c, r, air, a.num_cls, a.vs, L, reads, oob);
     }
     acc.err |= verr;
     acc.steps += reads;
     if (LENGTHS) {
-      double* out = a.lengths + ((size_t)refl * a.N + co.w) * 8;
+      double* out = a.lengths + ((size_t)refl * a.N + co.w) * 8 + k * 4;
 #pragma unroll
-      for (int k = 0; k < 2; ++k)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) out[k * 4 + j] = L[k][j];
+      for (int j = 0; j < 4; ++j) out[j] = L[j];
     }
-    // cal_rate (ray_tracing_basic.c:1554-1583), same association order
-    const double rsum = a.mu[0] * (L[0][0] + L[1][0]) + a.mu[1] * (L[0][1] + L[1][1]) +
-                        a.mu[2] * (L[0][2] + L[1][2]) + a.mu[3] * (L[0][3] + L[1][3]);
-    if (F64) {
-      acc.acc64 += a.is_exp ? exp(-rsum) : rsum;
-    } else {
-      const float val = a.is_exp ? exp_neg(rsum) : (float)rsum;
-      const float y = val - acc.kc;
-      const float t = acc.ks + y;
-      acc.kc = (t - acc.ks) - y;
-      acc.ks = t;
+    double M[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) M[j] = __shfl_xor(L[j], 1);
+    // a pair with an error contributes nothing: its reflection ends as NaN
+    verr |= __shfl_xor(verr, 1);
+    if (k == 0 && verr == 0) {
+      // cal_rate (ray_tracing_basic.c:1554-1583), incident + diffracted in
+      // the reference's association order
+      const double rsum = a.mu[0] * (L[0] + M[0]) + a.mu[1] * (L[1] + M[1]) +
+                          a.mu[2] * (L[2] + M[2]) + a.mu[3] * (L[3] + M[3]);
+      if (F64) {
+        acc.acc64 += a.is_exp ? exp(-rsum) : rsum;
+      } else {
+        const float val = a.is_exp ? exp_neg(rsum) : (float)rsum;
+        const float y = val - acc.kc;
+        const float t = acc.ks + y;
+        acc.kc = (t - acc.ks) - y;
+        acc.ks = t;
+      }
     }
   }
   if (oob) atomicAdd(a.oob, (unsigned long long)oob);
@@ -2407,7 +2430,21 @@ static int ib_run(Context* c, const ReflGeom* geom, int64_t R, int num_cls, int 
   order_by_direction(c, R, [geom](int64_t i, int k) { return geom[i].dir[k]; });
   HIP_TRY(c->h_order_pin.reserve((size_t)R));
   for (int64_t i = 0; i < R; ++i) c->h_order_pin[(size_t)i] = c->h_order[(size_t)i];
-  rc = ensure_layout(c);
+  // Automatic layout: bricks whenever they fit.  The bisection's gathers are
+  // scattered along the ray, so a 4x or 2x smaller volume keeps more of it in
+  // the L2s; the walk's choice (raw up to 64 MB) does not carry over.
+  if (c->layout_req == ANACOR_LAYOUT_AUTO && bricks_fit(c)) {
+    const char* env = std::getenv("ANACOR_VOLUME_LAYOUT");
+    if (!(env && std::strcmp(env, "raw") == 0)) {
+      c->layout_req = ANACOR_LAYOUT_BRICK;
+      rc = ensure_layout(c);
+      c->layout_req = ANACOR_LAYOUT_AUTO;
+    } else {
+      rc = ensure_layout(c);
+    }
+  } else {
+    rc = ensure_layout(c);
+  }
   if (rc) return rc;
   double* d_len = nullptr;
   if (lengths) HIP_TRY(hipMalloc(&d_len, (size_t)R * c->N * 8 * sizeof(double)));
