@@ -169,6 +169,13 @@ def ib_test(coord_list, len_coord_list, rotated_s1, xray, voxel_size, coefficien
     return float(v)
 
 
+def num_classes(volume) -> int:
+    """Number of labels in the volume minus one (the vacuum), as the
+    reference's driver counts num_cls for ib_test (main_lite.py:153)."""
+    vol = np.ascontiguousarray(volume).view(np.uint8).reshape(-1)
+    return int(np.count_nonzero(np.bincount(vol, minlength=256))) - 1
+
+
 def last_error():
     return _lib.last_error()
 

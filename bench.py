@@ -10,6 +10,10 @@ starts.  Prints one JSON line (absorption factors per second) with the trace
 kernel's roofline position and the reference C/OpenMP path timed on a bounded
 sample of the same workload on this host's cores.
 
+The reference's iterative-bisection traversal (ib_test) is timed on the same
+workload in a "bisection" record of the default run, with the reference's
+ib_test on this host's cores beside it.
+
 The learned surrogate (config "surrogate", and a "surrogate" record in the
 default run) is trained on 200k reflections the tracer computes for batched
 sample 2 (100^3, 20,089 crystal voxels), judged by the tracer on 10k held-out
@@ -217,6 +221,92 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
     return rec
 
 
+def bisection_cpu_baseline(wl, num_cls: int, budget_s: float):
+    """The reference's ib_test (oracle/_ref), one reflection per call as its
+    driver makes them (mp_worker.py:509-514), on a thread per host core (the
+    calls release the GIL); A_h/s on the first reflections of the workload.
+    Falls back to the FP64 restatement when the reference build is absent."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    cores = cpu_cores()
+    orc = O.Oracle()
+    geo = orc.reflection_setup(wl.s1, wl.omega, wl.raw_xray, wl.omega_axis, wl.F)
+    try:
+        ref = O.Reference()
+        table, keep = O.label_table(np.ascontiguousarray(wl.volume).view(np.int8))
+        kind = "reference"
+
+        def one(i):
+            return ref.ib_test(wl, wl.coords, geo["s1"][i], geo["xray"][i], num_cls, table=table)
+    except (FileNotFoundError, OSError):
+        keep, kind = None, "port"
+
+        def one(i):
+            return orc.ib_test(wl, wl.coords, geo["s1"][i], geo["xray"][i], num_cls)[0]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as pool:
+        first = list(pool.map(one, range(cores)))
+    per = (time.perf_counter() - t0) / cores
+    R = int(min(len(wl.omega), max(cores, budget_s / max(per, 1e-9))))
+    R = max(cores, (R // cores) * cores)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as pool:
+        out = np.array(list(pool.map(one, range(R))))
+    dt = time.perf_counter() - t0
+    del keep, first
+    return dict(value=R / dt, unit="absorption factors/s", cores=cores, kind=kind,
+                sample=f"first {R}/{len(wl.omega)} reflections, one ib_test call each, "
+                       f"{dt:.1f} s"), out
+
+
+def run_bisection(steps: int, warmup: int, cpu_budget: float, walk_value: float):
+    """The reference's iterative-bisection traversal (ib_test) over the same
+    256^3 workload: every reflection in one launch; num_cls from the volume's
+    labels as main_lite.py:153 counts them."""
+    from anacor_public_amd import workload as W
+    from anacor_public_amd.ray_tracing import Tracer, num_classes
+    wl = W.single_gpu_config()
+    ncls = num_classes(wl.volume)
+    R = len(wl.omega)
+    with Tracer.for_workload(wl) as t:
+        def call():
+            return t.bisection(wl.s1, wl.omega, wl.raw_xray, wl.omega_axis, wl.F, ncls)
+        for _ in range(warmup):
+            call()
+        kms, reads = [], 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            A = call()
+            st = t.stats()
+            kms.append(st["kernel_ms"])
+            reads = st["voxel_steps"]
+        elapsed = time.perf_counter() - t0
+        layout = {0: "raw", 2: "brick2", 4: "brick4"}.get(t.layout, str(t.layout))
+        outside = st["outside_reads"]
+    assert np.all(np.isfinite(A)) and np.all((A > 0) & (A <= 1))
+    value = R * steps / elapsed
+    k_ms = float(np.mean(kms))
+    achieved = reads / (k_ms * 1e-3) / 1e9      # 1 algorithmic byte per label read
+    rec = {"value": value, "unit": "absorption factors/s", "workload": wl.name,
+           "num_cls": ncls, "ms_per_step": 1e3 * elapsed / steps, "kernel_ms": k_ms,
+           "label_reads_per_launch": int(reads), "outside_reads": int(outside),
+           "label_reads_per_s": reads / (k_ms * 1e-3), "volume_layout": layout,
+           "speedup_vs_plain_walk": value / walk_value,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "ib_kernel", "kernel_ms": k_ms, "bytes_per_launch": int(reads)}}
+    log(f"# bisection: {value:.3e} A_h/s ({rec['speedup_vs_plain_walk']:.2f}x the walk), "
+        f"kernel {k_ms:.2f} ms, {reads} label reads, layout {layout}")
+    if cpu_budget > 0:
+        cpu, cpu_out = bisection_cpu_baseline(wl, ncls, cpu_budget)
+        cpu["max_rel_diff_vs_gpu"] = float(np.max(np.abs(A[:len(cpu_out)] - cpu_out) / cpu_out))
+        rec["cpu_baseline"] = cpu
+        rec["speedup_vs_cpu"] = value / cpu["value"]
+        log(f"# bisection cpu baseline ({cpu['kind']}, {cpu['cores']} threads): "
+            f"{cpu['value']:.2f} A_h/s, max rel diff vs GPU {cpu['max_rel_diff_vs_gpu']:.2e}")
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +316,8 @@ def main():
                     choices=["single", "small", "large", "batched", "surrogate"])
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the surrogate record of the default run")
+    ap.add_argument("--no-bisection", action="store_true",
+                    help="skip the iterative-bisection record of the default run")
     ap.add_argument("--epochs", type=int, default=600, help="surrogate training epochs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -383,6 +475,9 @@ def main():
     }
     rec.update(extra)
     runner.close()
+    if args.config == "single" and not args.no_bisection and not args.reflections:
+        rec["bisection"] = run_bisection(args.steps, args.warmup,
+                                         0.0 if args.no_cpu_baseline else 8.0, value)
     if args.config == "single" and not args.no_surrogate and not args.reflections:
         rec["surrogate"] = run_surrogate(args.steps, args.warmup, epochs=args.epochs)
     print(json.dumps(rec), flush=True)

@@ -336,13 +336,14 @@ class Reference:
             table, keep = label_table(np.ascontiguousarray(wl.volume).view(np.int8))
         shape = np.array(np.asarray(wl.volume).shape, dtype=np.int64)
         coords = np.ascontiguousarray(coords, dtype=np.int64)
-        with _stdout_silenced():
-            v = self.lib.ib_test(_ptr(coords), len(coords),
-                                 _ptr(np.ascontiguousarray(rotated_s1, dtype=np.float64)),
-                                 _ptr(np.ascontiguousarray(xray, dtype=np.float64)),
-                                 _ptr(np.ascontiguousarray(wl.voxel_size, dtype=np.float64)),
-                                 _ptr(np.ascontiguousarray(wl.coefficients, dtype=np.float64)),
-                                 table, _ptr(shape), int(wl.full_iteration), 0, int(num_cls))
+        # ib_test prints nothing (its test_mode is 0), so unlike the other
+        # entries it runs with stdout as it is: callable from several threads
+        v = self.lib.ib_test(_ptr(coords), len(coords),
+                             _ptr(np.ascontiguousarray(rotated_s1, dtype=np.float64)),
+                             _ptr(np.ascontiguousarray(xray, dtype=np.float64)),
+                             _ptr(np.ascontiguousarray(wl.voxel_size, dtype=np.float64)),
+                             _ptr(np.ascontiguousarray(wl.coefficients, dtype=np.float64)),
+                             table, _ptr(shape), int(wl.full_iteration), 0, int(num_cls))
         del keep
         return float(v)
 
