@@ -2166,8 +2166,12 @@ struct IbArgs {
 // The positions are finite or infinite, never NaN (finite voxel, positive
 // mid, finite or infinite ratio), so the saturating conversion keeps every
 // outside position outside and the range test runs on integers.
+// The addresses use full-rate 24-bit multiply-adds (v_mad_i32_i24) wherever
+// the host has checked that every factor fits: the raw layout below 2^23
+// voxels per z plane, the bricked layouts always (bricks_fit); the wide raw
+// layout uses 32-bit multiplies.  yx = Y * X.
 template <int LAYOUT>
-__device__ __forceinline__ int ib_label(const Volume& V, double z, double y, double x,
+__device__ __forceinline__ int ib_label(const Volume& V, int yx, double z, double y, double x,
                                         unsigned& oob) {
   const int iz = cvt_trunc_sat(z), iy = cvt_trunc_sat(y), ix = cvt_trunc_sat(x);
   if (!((unsigned)iz < (unsigned)V.Z && (unsigned)iy < (unsigned)V.Y &&
@@ -2175,9 +2179,10 @@ __device__ __forceinline__ int ib_label(const Volume& V, double z, double y, dou
     ++oob;
     return 0;
   }
-  if (LAYOUT == kLayoutRaw || LAYOUT == kLayoutWide)
-    return load_label(V, (iz * V.Y + iy) * V.X + ix);
-  const int f = iz * 64 + (iz >> 3) * V.cz + iy * 8 + (iy >> 3) * V.cy + ix + (ix >> 3) * 504;
+  if (LAYOUT == kLayoutWide) return load_label(V, iz * yx + iy * V.X + ix);
+  if (LAYOUT == kLayoutRaw) return load_label(V, mad24(iz, yx, mad24(iy, V.X, ix)));
+  const int f = mad24(iz, 64, mad24(iz >> 3, V.cz, mad24(iy, 8, mad24(iy >> 3, V.cy,
+                mad24(ix >> 3, 504, ix)))));
   return load_packed<LAYOUT>(V, f);
 }
 
@@ -2190,7 +2195,7 @@ struct IbPoint {
 // outer = 0 pulls `longest` in while the sample holds `cls`; outer = 1 pushes
 // `shortest` out while it does.  Returns the voxel plus floor(mid) steps.
 template <int LAYOUT>
-__device__ __forceinline__ IbPoint ib_search(const Volume& V, const double c[3],
+__device__ __forceinline__ IbPoint ib_search(const Volume& V, int yx, const double c[3],
                                              const double r[3], double longest, double shortest,
                                              bool outer, int cls, unsigned& reads,
                                              unsigned& oob) {
@@ -2198,7 +2203,7 @@ __device__ __forceinline__ IbPoint ib_search(const Volume& V, const double c[3],
   double mid = (longest + shortest) / 2.0;
   while (diff > 1.0) {
     mid = (longest + shortest) / 2.0;
-    const int label = ib_label<LAYOUT>(V, __builtin_floor(c[0] + mid * r[0]),
+    const int label = ib_label<LAYOUT>(V, yx, __builtin_floor(c[0] + mid * r[0]),
                                        __builtin_floor(c[1] - mid * r[1]),
                                        __builtin_floor(c[2] + mid * r[2]), oob);
     if ((label == cls) == outer) shortest = mid;
@@ -2232,15 +2237,15 @@ __device__ __forceinline__ double ib_dist(const double a[3], const double b[3],
 // of one ray from voxel c: L = [li, lo, cr, bu].  Each boundary pair is
 // measured as soon as it is found, so only its length stays live.
 template <int LAYOUT>
-__device__ __forceinline__ void ib_ray(const Volume& V, const double c[3], const double r[3],
+__device__ __forceinline__ void ib_ray(const Volume& V, int yx, const double c[3], const double r[3],
                                        double air_longest, int num_cls, const double vs[3],
                                        double L[4], unsigned& reads, unsigned& oob) {
-  const IbPoint A = ib_search<LAYOUT>(V, c, r, air_longest, 0.0, false, 0, reads, oob);
+  const IbPoint A = ib_search<LAYOUT>(V, yx, c, r, air_longest, 0.0, false, 0, reads, oob);
   const double total = ib_dist(c, A.p, vs);
   double cr;
   double cmid;
   {
-    const IbPoint C = ib_search<LAYOUT>(V, c, r, A.mid, 0.0, true, 3, reads, oob);
+    const IbPoint C = ib_search<LAYOUT>(V, yx, c, r, A.mid, 0.0, true, 3, reads, oob);
     cr = ib_dist(c, C.p, vs);
     cmid = C.mid;
   }
@@ -2248,17 +2253,17 @@ __device__ __forceinline__ void ib_ray(const Volume& V, const double c[3], const
   double pair0 = 0.0, pair1 = 0.0;
   int pairs = 0;
   do {
-    const IbPoint Li = ib_search<LAYOUT>(V, c, r, A.mid, cmid, false, 2, reads, oob);
+    const IbPoint Li = ib_search<LAYOUT>(V, yx, c, r, A.mid, cmid, false, 2, reads, oob);
     if (ib_l1(Li.p, A.p) / 3 > 1) {
-      const IbPoint Lo = ib_search<LAYOUT>(V, c, r, A.mid, Li.mid, true, 2, reads, oob);
+      const IbPoint Lo = ib_search<LAYOUT>(V, yx, c, r, A.mid, Li.mid, true, 2, reads, oob);
       if (ib_l1(Li.p, Lo.p) / 3 <= 2) break;     // a segmentation artefact ends the search
       pair0 = ib_dist(Li.p, Lo.p, vs);
       pairs = 1;
     }
     if (num_cls > 3) {
-      const IbPoint Bi = ib_search<LAYOUT>(V, c, r, A.mid, cmid, false, 4, reads, oob);
+      const IbPoint Bi = ib_search<LAYOUT>(V, yx, c, r, A.mid, cmid, false, 4, reads, oob);
       if (ib_l1(Bi.p, A.p) > 1) {
-        const IbPoint Bo = ib_search<LAYOUT>(V, c, r, A.mid, Bi.mid, true, 4, reads, oob);
+        const IbPoint Bo = ib_search<LAYOUT>(V, yx, c, r, A.mid, Bi.mid, true, 4, reads, oob);
         if (ib_l1(Bi.p, Bo.p) / 3 <= 2) break;
         const double d = ib_dist(Bi.p, Bo.p, vs);
         if (pairs == 0) pair0 = d;
@@ -2322,7 +2327,7 @@ void ib_kernel(IbArgs a) {
         default: air = (double)(a.Z - 1 - co.x); break;
       }
       const double r[3] = {ray.r[face - 1][0], ray.r[face - 1][1], ray.r[face - 1][2]};
-      ib_ray<LAYOUT>(V, c, r, air, a.num_cls, a.vs, L, reads, oob);
+      ib_ray<LAYOUT>(V, a.Y * a.X, c, r, air, a.num_cls, a.vs, L, reads, oob);
     }
     acc.err |= verr;
     acc.steps += reads;
@@ -2365,15 +2370,17 @@ static void launch_ib(const IbArgs& a, int grid, hipStream_t s) {
 static void dispatch_ib(const IbArgs& a, int grid, int layout, bool f64, bool lengths,
                         hipStream_t s) {
   using Fn = void (*)(const IbArgs&, int, hipStream_t);
-  static const Fn table[3][2][2] = {
+  static const Fn table[4][2][2] = {
       {{launch_ib<kLayoutRaw, false, false>, launch_ib<kLayoutRaw, false, true>},
        {launch_ib<kLayoutRaw, true, false>, launch_ib<kLayoutRaw, true, true>}},
+      {{launch_ib<kLayoutWide, false, false>, launch_ib<kLayoutWide, false, true>},
+       {launch_ib<kLayoutWide, true, false>, launch_ib<kLayoutWide, true, true>}},
       {{launch_ib<kLayoutBrick2, false, false>, launch_ib<kLayoutBrick2, false, true>},
        {launch_ib<kLayoutBrick2, true, false>, launch_ib<kLayoutBrick2, true, true>}},
       {{launch_ib<kLayoutBrick4, false, false>, launch_ib<kLayoutBrick4, false, true>},
        {launch_ib<kLayoutBrick4, true, false>, launch_ib<kLayoutBrick4, true, true>}}};
-  // the raw and wide layouts share one 32-bit address form here
-  const int li = layout == kLayoutBrick2 ? 1 : layout == kLayoutBrick4 ? 2 : 0;
+  const int li = layout == kLayoutWide ? 1 : layout == kLayoutBrick2 ? 2
+                                                                  : layout == kLayoutBrick4 ? 3 : 0;
   table[li][f64 ? 1 : 0][lengths ? 1 : 0](a, grid, s);
 }
 
