@@ -208,11 +208,13 @@ __device__ __forceinline__ int clamp_int(int v, int hi) {
   return r;
 }
 
-template <bool GUARD>
+// BIAS = false drops the add of a zero bias (floor rounding): w + 0.0
+// converts to the same integer as w, -0.0 included.
+template <bool GUARD, bool BIAS = true>
 __device__ __forceinline__ int axis_index(double base, double incd, double d, double bias,
                                           int hi) {
   const double w = base + incd * d;
-  int v = cvt_trunc_sat(w + bias);
+  int v = cvt_trunc_sat(BIAS ? w + bias : w);
   if (GUARD && !(w < 0x1p63)) v = 0;
   return clamp_int(v, hi);
 }
@@ -237,11 +239,11 @@ struct RayLane {
   int ca, cb, cp, sp;    // bricked layouts: brick constants of a, b, p and stride of p
 };
 
-template <bool GUARD, bool WIDE>
+template <bool GUARD, bool WIDE, bool BIAS = true>
 __device__ __forceinline__ int lane_offset(const RayLane& L, int inc) {
   const double incd = (double)inc;
-  const int ia = axis_index<GUARD>(L.ba, incd, L.da, L.bias, L.ha);
-  const int ib = axis_index<GUARD>(L.bb, incd, L.db, L.bias, L.hb);
+  const int ia = axis_index<GUARD, BIAS>(L.ba, incd, L.da, L.bias, L.ha);
+  const int ib = axis_index<GUARD, BIAS>(L.bb, incd, L.db, L.bias, L.hb);
   const int ip = min(inc, L.incmax);
   if (WIDE) return ia * L.sa + ib * L.sb + ip * L.pstep + L.pbase;
   // every factor fits 24 bits (checked on the host): full-rate v_mad_i32_i24
@@ -295,13 +297,13 @@ __device__ __forceinline__ int load_packed(const Volume& V, int f) {
 }
 
 // Label at increment inc of the lane's ray in the given layout.
-template <bool GUARD, int LAYOUT>
+template <bool GUARD, int LAYOUT, bool BIAS = true>
 __device__ __forceinline__ int label_at(const RayLane& L, const Volume& V, int inc) {
   if (LAYOUT == kLayoutRaw || LAYOUT == kLayoutWide)
-    return load_label(V, lane_offset<GUARD, LAYOUT == kLayoutWide>(L, inc));
+    return load_label(V, lane_offset<GUARD, LAYOUT == kLayoutWide, BIAS>(L, inc));
   const double incd = (double)inc;
-  const int ia = axis_index<GUARD>(L.ba, incd, L.da, L.bias, L.ha);
-  const int ib = axis_index<GUARD>(L.bb, incd, L.db, L.bias, L.hb);
+  const int ia = axis_index<GUARD, BIAS>(L.ba, incd, L.da, L.bias, L.ha);
+  const int ib = axis_index<GUARD, BIAS>(L.bb, incd, L.db, L.bias, L.hb);
   const int ip = mad24(L.pdir, min(inc, L.incmax), L.p0);
   // every factor fits 24 bits and every partial sum 31 (checked on the host)
   const int f = mad24(ia, L.sa, mad24(ia >> 3, L.ca, mad24(ib, L.sb, mad24(ib >> 3, L.cb,
@@ -316,10 +318,10 @@ __device__ __forceinline__ int label_at(const RayLane& L, const Volume& V, int i
 // len_path_2; out receives [li, lo, cr, bu].
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
-template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT>
-__device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, const double vs[3],
-                                         lds_u32* seg, int& len_out, int& nseg_out,
-                                         int last_zyx[3], double out[4]) {
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool BIAS>
+__device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
+                                              const double vs[3], lds_u32* seg, int& len_out,
+                                              int& nseg_out, int last_zyx[3], double out[4]) {
   // taken steps (increments 0 .. cnt-1 passed), last label, vacuum run
   int cnt = 0, prev = 0, va = 0;
   bool done = L.limit <= 0;
@@ -342,7 +344,7 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
     int lab[U];
 #pragma unroll
     for (int u = 0; u < U; ++u)  // offsets stay inside the volume (clamped)
-      lab[u] = label_at<GUARD, LAYOUT>(L, V, inc0 + u);
+      lab[u] = label_at<GUARD, LAYOUT, BIAS>(L, V, inc0 + u);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int inc = inc0 + u;
@@ -422,6 +424,21 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
   }
   add(cls, len - start);
   out[0] = li; out[1] = lo; out[2] = cr; out[3] = bu;
+}
+
+// The stepping loop in two forms: floor rounding (zero bias, one FP64 add per
+// axis and step fewer) and round-half-away.  The rounding is fixed per ray
+// and almost always the same across a wave, so the branch rarely diverges.
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT>
+__device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, const double vs[3],
+                                         lds_u32* seg, int& len_out, int& nseg_out,
+                                         int last_zyx[3], double out[4]) {
+  if (L.bias != 0.0)
+    walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, true>(L, V, vs, seg, len_out, nseg_out, last_zyx,
+                                                     out);
+  else
+    walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, false>(L, V, vs, seg, len_out, nseg_out,
+                                                      last_zyx, out);
 }
 
 // Set up the lane's ray for exit face `face` (1..6) of rule `rr`; false when
