@@ -47,7 +47,7 @@ MLP_FLOPS_PER_ROW = 2 * (12 * 256 + 3 * 256 * 256 + 256)   # 2 x 199,936 MACs
 # L2<->fabric bytes per voxel step of trace_kernel on the 256^3 workload:
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a 1 GiB copy) over
 # 1.017e11 voxel steps, profiles/trace_kernel_plain-256cube.md.
-TRAFFIC_BYTES_PER_STEP = {"plain-256^3-stride2": 0.885}
+TRAFFIC_BYTES_PER_STEP = {"plain-256^3-stride2": 0.883}
 METRIC = "absorption factors/sec (rays/s) at 1 GPU, plain vs surrogate at stated rel. error"
 
 
