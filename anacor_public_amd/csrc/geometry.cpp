@@ -217,6 +217,92 @@ void setup_range(const double* s1, const double* omega, int64_t lo, int64_t hi,
   }
 }
 
+// The bisection ray of one beam vector (increments, bisection.c:21-148):
+// angles and direction as for the walk, then each ratio expression as the
+// reference writes it.  Each libm value is computed once per distinct
+// argument (bit-identical arguments give bit-identical results).  A face
+// whose assert fails keeps zero ratios and no valid bit.
+void ib_ray(const double* v, bool incident, IbRay& ray) {
+  std::memset(&ray, 0, sizeof(ray));
+  double t, p, d3[3];
+  beam_angles(v, incident, &t, &p);
+  to_myframe(v, d3);
+  ray.dir[0] = d3[0];
+  ray.dir[1] = d3[1];
+  ray.dir[2] = d3[2];
+  ray.neg = incident ? 1u : 0u;
+  const double at = std::fabs(t), ap = std::fabs(p);
+  const bool narrow = at < kPi / 2;
+  const bool p_is_ap = same_bits(p, ap), t_is_at = same_bits(t, at);
+  auto set = [&](int face, bool ok, double rz, double ry, double rx) {
+    if (!ok) return;
+    ray.r[face - 1][0] = rz;
+    ray.r[face - 1][1] = ry;
+    ray.r[face - 1][2] = rx;
+    ray.valid |= 1u << (face - 1);
+  };
+  const double tan_p = std::tan(p), cos_p = std::cos(p), sin_p = std::sin(p);
+  const double tan_t = std::tan(t), tan_pi_t = std::tan(kPi - t);
+  const double cos_ap = p_is_ap ? cos_p : std::cos(ap);
+  const double tan_ap = p_is_ap ? tan_p : std::tan(ap);
+  const double sin_ap = p_is_ap ? sin_p : std::sin(ap);
+  // faces 4 (y -> 0) and 5 (y -> y_max), :39-84
+  if (narrow) {
+    const double tt = t_is_at ? tan_t : std::tan(at);
+    set(4, t > 0, sin_p / tt, 1.0, -cos_ap / tt);
+    set(5, t < 0, sin_p / tt, -1.0, -cos_ap / tt);
+  } else {
+    const double tt = t_is_at ? tan_pi_t : std::tan(kPi - at);
+    const double mp = -p;
+    const double sin_mp = same_bits(mp, ap) ? sin_ap : std::sin(mp);
+    set(4, t > 0, sin_mp / tt, 1.0, cos_ap / tt);
+    set(5, t < 0, -sin_p / tt, -1.0, cos_ap / tt);
+  }
+  // faces 6 (x -> 0) and 1 (x -> x_max), :85-106
+  set(6, at <= kPi / 2, tan_p, tan_t / cos_p, -1.0);
+  set(1, at >= kPi / 2, -tan_p, tan_pi_t / cos_ap, 1.0);
+  // faces 2 (z -> 0) and 3 (z -> z_max), :107-142
+  if (narrow) {
+    const double rx = -1 / tan_ap, ry = tan_t / sin_ap;
+    set(2, p < 0, -1.0, ry, rx);
+    set(3, p > 0, 1.0, ry, rx);
+  } else {
+    const double rx = 1 / tan_ap, ry = tan_pi_t / sin_ap;
+    set(2, p > 0, -1.0, ry, rx);
+    set(3, p < 0, 1.0, ry, rx);
+  }
+}
+
+void rotate_one(const double* s1, double omega, const double* raw_xray,
+                const double* omega_axis, const double* F, double* xr, double* sv) {
+  double rw[9], m[9], tr[9];
+  rodrigues(omega_axis, omega, rw);
+  mul3(rw, F, m, 3);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) tr[b * 3 + a] = m[a * 3 + b];
+  mul3(tr, raw_xray, xr, 1);
+  mul3(tr, s1, sv, 1);
+}
+
+// Run body(lo, hi) over [0, R) on up to n_threads threads.
+template <class Body>
+void parallel_range(int64_t R, int n_threads, Body body) {
+  int nt = n_threads > 0 ? n_threads : 1;
+  if (R < 512) nt = 1;
+  if (nt == 1) {
+    body((int64_t)0, R);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const int64_t chunk = (R + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const int64_t lo = t * chunk, hi = lo + chunk < R ? lo + chunk : R;
+    if (lo >= hi) break;
+    pool.emplace_back(body, lo, hi);
+  }
+  for (auto& th : pool) th.join();
+}
+
 }  // namespace
 
 void host_setup(const double* s1, const double* omega, int64_t R, const double* raw_xray,
@@ -240,49 +326,28 @@ void host_setup(const double* s1, const double* omega, int64_t R, const double* 
   for (auto& th : pool) th.join();
 }
 
-void ib_rules(const ReflGeom* geom, int64_t R, IbRule* rules) {
-  // increments (bisection.c:21-148), each expression as the reference
-  // writes it; a face whose assert fails keeps zero ratios and no valid bit
-  for (int64_t i = 0; i < R; ++i) {
-    for (int k = 0; k < 2; ++k) {
-      IbRay& ray = rules[i].ray[k];
-      std::memset(&ray, 0, sizeof(ray));
-      std::memcpy(ray.dir, geom[i].dir[k], 3 * sizeof(double));
-      ray.neg = k == 0 ? 1u : 0u;
-      const double t = geom[i].theta[k], p = geom[i].phi[k];
-      const double at = std::fabs(t), ap = std::fabs(p);
-      const bool narrow = at < kPi / 2;
-      auto set = [&](int face, bool ok, double rz, double ry, double rx) {
-        if (!ok) return;
-        ray.r[face - 1][0] = rz;
-        ray.r[face - 1][1] = ry;
-        ray.r[face - 1][2] = rx;
-        ray.valid |= 1u << (face - 1);
-      };
-      // face 4 (y -> 0) and 5 (y -> y_max), :39-84
-      if (narrow) {
-        set(4, t > 0, std::sin(p) / std::tan(at), 1.0, -std::cos(ap) / std::tan(at));
-        set(5, t < 0, std::sin(p) / std::tan(at), -1.0, -std::cos(ap) / std::tan(at));
-      } else {
-        const double tt = std::tan(kPi - at);
-        set(4, t > 0, std::sin(-p) / tt, 1.0, std::cos(ap) / tt);
-        set(5, t < 0, -std::sin(p) / tt, -1.0, std::cos(ap) / tt);
-      }
-      // face 6 (x -> 0) and 1 (x -> x_max), :85-106
-      set(6, at <= kPi / 2, std::tan(p), std::tan(t) / std::cos(p), -1.0);
-      set(1, at >= kPi / 2, -std::tan(p), std::tan(kPi - t) / std::cos(ap), 1.0);
-      // face 2 (z -> 0) and 3 (z -> z_max), :107-142
-      if (narrow) {
-        const double rx = -1 / std::tan(ap), ry = std::tan(t) / std::sin(ap);
-        set(2, p < 0, -1.0, ry, rx);
-        set(3, p > 0, 1.0, ry, rx);
-      } else {
-        const double rx = 1 / std::tan(ap), ry = std::tan(kPi - t) / std::sin(ap);
-        set(2, p > 0, -1.0, ry, rx);
-        set(3, p < 0, 1.0, ry, rx);
-      }
+void ib_setup(const double* s1, const double* omega, int64_t R, const double* raw_xray,
+              const double* omega_axis, const double* F, IbRule* rules, int n_threads) {
+  if (R <= 0) return;
+  parallel_range(R, n_threads, [=](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; ++i) {
+      double xr[3], sv[3];
+      rotate_one(s1 + 3 * i, omega[i], raw_xray, omega_axis, F, xr, sv);
+      ib_ray(xr, true, rules[i].ray[0]);
+      ib_ray(sv, false, rules[i].ray[1]);
     }
-  }
+  });
+}
+
+void ib_setup_rotated(const double* rotated_s1, const double* xray, int64_t R, IbRule* rules,
+                      int n_threads) {
+  if (R <= 0) return;
+  parallel_range(R, n_threads, [=](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; ++i) {
+      ib_ray(xray + 3 * i, true, rules[i].ray[0]);
+      ib_ray(rotated_s1 + 3 * i, false, rules[i].ray[1]);
+    }
+  });
 }
 
 void host_setup_directions(const double* rotated_s1, const double* theta_phi, int64_t D,

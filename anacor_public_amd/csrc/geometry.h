@@ -56,8 +56,15 @@ struct IbRule {
   IbRay ray[2];         // [0] incident, [1] diffracted
 };
 
-// Bisection rules of R reflections from their angles and directions.
-void ib_rules(const ReflGeom* geom, int64_t R, IbRule* rules);
+// Bisection rules of R reflections: rotation as host_setup, then angles,
+// directions and the per-face ratios of both rays; up to n_threads threads.
+void ib_setup(const double* s1, const double* omega, int64_t R, const double* raw_xray,
+              const double* omega_axis, const double* F, IbRule* rules, int n_threads);
+
+// The same for already rotated vectors (rotated_s1[R][3], xray[R][3]), as
+// ib_test takes them (ray_tracing.c:65-80).
+void ib_setup_rotated(const double* rotated_s1, const double* xray, int64_t R, IbRule* rules,
+                      int n_threads);
 
 // Fill rules[R] (and geom[R] when non-null) for R reflections, using up to
 // n_threads host threads.  max_dim is the largest volume extent, used to set

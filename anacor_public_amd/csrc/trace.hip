@@ -2422,9 +2422,9 @@ static int report_ray_errors(Context* c, const double* out, int64_t R) {
   return ANACOR_OK;
 }
 
-// Bisection traversal of R reflections given their angles and directions.
-static int ib_run(Context* c, const ReflGeom* geom, int64_t R, int num_cls, int flags,
-                  double* out, double* lengths) {
+// Bisection traversal of R reflections whose rules the caller has written
+// into c->h_ib (pinned).
+static int ib_run(Context* c, int64_t R, int num_cls, int flags, double* out, double* lengths) {
   const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
   c->last_oob = 0;
   c->last_steps = 0;
@@ -2449,9 +2449,8 @@ static int ib_run(Context* c, const ReflGeom* geom, int64_t R, int num_cls, int 
     c->ib_cap = (size_t)R;
   }
   if (!c->d_oob) HIP_TRY(hipMalloc(&c->d_oob, sizeof(unsigned long long)));
-  HIP_TRY(c->h_ib.reserve((size_t)R));
-  ib_rules(geom, R, c->h_ib.data());
-  order_by_direction(c, R, [geom](int64_t i, int k) { return geom[i].dir[k]; });
+  const IbRule* rules = c->h_ib.data();
+  order_by_direction(c, R, [rules](int64_t i, int k) { return rules[i].ray[k].dir; });
   HIP_TRY(c->h_order_pin.reserve((size_t)R));
   for (int64_t i = 0; i < R; ++i) c->h_order_pin[(size_t)i] = c->h_order[(size_t)i];
   // Automatic layout: bricks whenever they fit.  The bisection's gathers are
@@ -2531,16 +2530,6 @@ static int ib_run(Context* c, const ReflGeom* geom, int64_t R, int num_cls, int 
   c->last_blocks = (int)nblk;
   if (d_len) HIP_TRY(hipFree(d_len));
   return report_ray_errors(c, out, R);
-}
-
-// Rotated vectors (rotated_s1[R][3], xray[R][3]) to angles and directions, as
-// ib_test computes them (ray_tracing.c:65-80).
-static void rotated_geometry(const double* rotated_s1, const double* xray, int64_t R,
-                             int64_t max_dim_, std::vector<ReflGeom>& geom) {
-  geom.resize((size_t)std::max<int64_t>(R, 1));
-  ReflRule scratch;
-  for (int64_t i = 0; i < R; ++i)
-    host_setup_rotated(rotated_s1 + 3 * i, xray + 3 * i, max_dim_, &scratch, &geom[(size_t)i]);
 }
 
 }  // namespace anacor
@@ -2882,12 +2871,10 @@ int anacor_ctx_bisection(anacor_ctx* ctx, const double* s1, const double* omega,
   std::lock_guard<std::mutex> lk(c->mu_run);
   HIP_TRY(hipSetDevice(c->device));
   const auto t0 = std::chrono::steady_clock::now();
-  HIP_TRY(c->h_rules.reserve((size_t)std::max<int64_t>(R, 1)));
-  std::vector<ReflGeom> geom((size_t)std::max<int64_t>(R, 1));
-  host_setup(s1, omega, R, raw_xray, omega_axis, F, max_dim(c), c->h_rules.data(), geom.data(),
-             host_threads(0));
+  HIP_TRY(c->h_ib.reserve((size_t)std::max<int64_t>(R, 1)));
+  ib_setup(s1, omega, R, raw_xray, omega_axis, F, c->h_ib.data(), host_threads(0));
   c->last_host_ms = ms_since(t0);
-  return ib_run(c, geom.data(), R, num_cls, flags, out, nullptr);
+  return ib_run(c, R, num_cls, flags, out, nullptr);
 }
 
 int anacor_ctx_bisection_rotated(anacor_ctx* ctx, const double* rotated_s1, const double* xray,
@@ -2903,10 +2890,10 @@ int anacor_ctx_bisection_rotated(anacor_ctx* ctx, const double* rotated_s1, cons
   std::lock_guard<std::mutex> lk(c->mu_run);
   HIP_TRY(hipSetDevice(c->device));
   const auto t0 = std::chrono::steady_clock::now();
-  std::vector<ReflGeom> geom;
-  rotated_geometry(rotated_s1, xray, R, max_dim(c), geom);
+  HIP_TRY(c->h_ib.reserve((size_t)std::max<int64_t>(R, 1)));
+  ib_setup_rotated(rotated_s1, xray, R, c->h_ib.data(), host_threads(0));
   c->last_host_ms = ms_since(t0);
-  return ib_run(c, geom.data(), R, num_cls, flags, out, lengths);
+  return ib_run(c, R, num_cls, flags, out, lengths);
 }
 
 double ib_test(int64_t* coord_list, int64_t len_coord_list, double* rotated_s1, double* xray,
@@ -2929,11 +2916,10 @@ double ib_test(int64_t* coord_list, int64_t len_coord_list, double* rotated_s1, 
   double out = __builtin_nan("");
   {
     std::lock_guard<std::mutex> lk(c->mu_run);
-    if (hipSetDevice(c->device) == hipSuccess) {
-      std::vector<ReflGeom> geom;
-      rotated_geometry(rotated_s1, xray, 1, max_dim(c), geom);
+    if (hipSetDevice(c->device) == hipSuccess && c->h_ib.reserve(1) == hipSuccess) {
+      ib_setup_rotated(rotated_s1, xray, 1, c->h_ib.data(), 1);
       const int rc = num_cls < 1 ? ib_check_num_cls(num_cls)
-                                 : ib_run(c, geom.data(), 1, num_cls, 0, &out, nullptr);
+                                 : ib_run(c, 1, num_cls, 0, &out, nullptr);
       if (rc != ANACOR_OK && rc != ANACOR_ERR_GEOMETRY)
         fprintf(stderr, "anacor: ib_test: %s\n", g_err);
     }
