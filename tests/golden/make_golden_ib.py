@@ -11,6 +11,11 @@ with the number of path points for every 8th voxel.  Each reflection runs in
 a subprocess first, because the reference asserts (and reads its label table
 without bounds checks) on some geometries; one that dies is recorded as such.
 
+Also tests/golden/golden_ib_fuzz.npz: ib_test (num_cls 4) on every
+reflection of the 48 cases of golden_fuzz.npz (random labels incl. 4 and 7,
+odd volume shapes, voxel sizes and goniometers), NaN with a flag where the
+reference dies (assert, or a read through its unchecked row table).
+
 Usage:  python tests/golden/make_golden_ib.py   (needs /root/reference)
 """
 from __future__ import annotations
@@ -122,6 +127,57 @@ def one_reflection(i: int, out_path: str) -> None:
     np.savez(out_path, means=np.array(means), L=L, npts=npts)
 
 
+def fuzz_rotated(g, c: int):
+    """Rotated s1 and xray of every reflection of fuzz case c (the oracle's
+    rotation, bit-identical to the reference's, tests/test_oracle_golden.py)."""
+    k = lambda n: g[f"c{c}_{n}"]  # noqa: E731
+    geo = Oracle().reflection_setup(k("s1"), k("omega"), k("raw_xray"), k("omega_axis"), k("F"))
+    return np.ascontiguousarray(geo["s1"]), np.ascontiguousarray(geo["xray"])
+
+
+def fuzz_case(c: int, lo: int, hi: int, out_path: str) -> None:
+    """ib_test (num_cls 4) of reflections lo..hi-1 of fuzz case c of
+    golden_fuzz.npz (random labels incl. 4 and 7, odd shapes, voxel sizes and
+    goniometers)."""
+    g = np.load(os.path.join(HERE, "golden_fuzz.npz"))
+    k = lambda n: g[f"c{c}_{n}"]  # noqa: E731
+    rs1, rxr = fuzz_rotated(g, c)
+    lib = reference_lib()
+    vol = np.ascontiguousarray(k("volume"))
+    table, keep = label_table(vol.view(np.int8))
+    shape = np.array(vol.shape, dtype=np.int64)
+    co = np.ascontiguousarray(k("coords"), dtype=np.int64)
+    vs = np.ascontiguousarray(k("voxel_size"), dtype=np.float64)
+    mu = np.ascontiguousarray(k("coefficients"), dtype=np.float64)
+    p = lambda a: a.ctypes.data_as(ct.c_void_p)  # noqa: E731
+    out = [lib.ib_test(p(co), len(co), p(np.ascontiguousarray(rs1[i])),
+                       p(np.ascontiguousarray(rxr[i])), p(vs), p(mu), table, p(shape), 0, 0, 4)
+           for i in range(lo, hi)]
+    del keep
+    np.save(out_path, np.array(out))
+
+
+def fuzz_main(tmp: str) -> dict:
+    g = np.load(os.path.join(HERE, "golden_fuzz.npz"))
+    res = {}
+    for c in range(int(g["n_cases"])):
+        R = len(g[f"c{c}_omega"])
+        vals = np.full(R, np.nan)
+        died = np.zeros(R, dtype=bool)
+        for i in range(R):
+            f = os.path.join(tmp, f"f{c}_{i}.npy")
+            rc = subprocess.run([sys.executable, __file__, "--fuzz", str(c), str(i), str(i + 1),
+                                 f], capture_output=True).returncode
+            if rc != 0:
+                died[i] = True
+                continue
+            vals[i] = np.load(f)[0]
+        res[f"c{c}_ib"] = vals
+        res[f"c{c}_ib_died"] = died
+        res[f"c{c}_rotated_s1"], res[f"c{c}_xray"] = fuzz_rotated(g, c)
+    return res
+
+
 def main() -> None:
     build(with_reference=True)
     vol, coords, s1, omega, geo = case()
@@ -131,6 +187,11 @@ def main() -> None:
     npts = np.zeros((len(NUM_CLS), R, len(sub), 2), dtype=np.int64)
     ok = np.zeros(R, dtype=bool)
     with tempfile.TemporaryDirectory() as tmp:
+        fuzz = fuzz_main(tmp)
+        died = sum(int(v.sum()) for k, v in fuzz.items() if k.endswith("_died"))
+        total = sum(v.size for k, v in fuzz.items() if k.endswith("_died"))
+        print(f"fuzz: the reference died on {died} of {total} reflections")
+        np.savez_compressed(os.path.join(HERE, "golden_ib_fuzz.npz"), **fuzz)
         for i in range(R):
             f = os.path.join(tmp, f"r{i}.npz")
             rc = subprocess.run([sys.executable, __file__, "--one", str(i), f],
@@ -155,5 +216,8 @@ def main() -> None:
 if __name__ == "__main__":
     if len(sys.argv) > 3 and sys.argv[1] == "--one":
         one_reflection(int(sys.argv[2]), sys.argv[3])
+        sys.exit(0)
+    if len(sys.argv) > 5 and sys.argv[1] == "--fuzz":
+        fuzz_case(int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4]), sys.argv[5])
         sys.exit(0)
     main()

@@ -138,3 +138,36 @@ def test_zero_v1_direction_is_an_assert_case(oracle):
     assert oracle.ib_ratios(3, theta, phi) is None
     m, _ = oracle.ib_test(wl, wl.coords, np.array([1.0, 0.0, 0.0]), np.array([0.0, 0.0, -1.0]))
     assert np.isnan(m)
+
+
+def _fuzz_wl(g, c):
+    from anacor_public_amd import workload as W
+    k = lambda n: g[f"c{c}_{n}"]  # noqa: E731
+    wl = W.Workload(f"fuzz{c}", k("volume"), k("coords"), k("s1"), k("omega"))
+    wl.voxel_size = k("voxel_size")
+    wl.coefficients = k("coefficients")
+    return wl
+
+
+def test_fuzz_matches_reference(oracle):
+    """Random labels (incl. 4 and 7), odd shapes, voxel sizes and goniometers
+    (the 48 cases of golden_fuzz.npz): wherever the reference's reads stay in
+    its tables with vacuum outside the volume, ib_test equals it bit for bit;
+    where the reference died, the oracle reports the assert (NaN) or a read
+    whose value in the reference is undefined."""
+    from conftest import load_npz
+    g = load_npz("golden_fuzz.npz")
+    ref = load_npz("golden_ib_fuzz.npz")
+    compared = 0
+    for c in range(int(g["n_cases"])):
+        wl = _fuzz_wl(g, c)
+        for i in range(len(wl.omega)):
+            m, diag = oracle.ib_test(wl, wl.coords, ref[f"c{c}_rotated_s1"][i],
+                                     ref[f"c{c}_xray"][i], 4)
+            if ref[f"c{c}_ib_died"][i]:
+                assert np.isnan(m) or diag["undefined"] > 0, (c, i)
+                continue
+            if diag["undefined"] == 0:
+                assert m == ref[f"c{c}_ib"][i], (c, i, m, ref[f"c{c}_ib"][i])
+                compared += 1
+    assert compared >= 200, compared
