@@ -209,14 +209,15 @@ __device__ __forceinline__ int clamp_int(int v, int hi) {
 }
 
 // BIAS = false drops the add of a zero bias (floor rounding): w + 0.0
-// converts to the same integer as w, -0.0 included.
-template <bool GUARD, bool BIAS = true>
+// converts to the same integer as w, -0.0 included.  CLAMP = false drops the
+// clamp where the caller has shown that the index stays in [0, hi].
+template <bool GUARD, bool BIAS = true, bool CLAMP = true>
 __device__ __forceinline__ int axis_index(double base, double incd, double d, double bias,
                                           int hi) {
   const double w = base + incd * d;
   int v = cvt_trunc_sat(BIAS ? w + bias : w);
   if (GUARD && !(w < 0x1p63)) v = 0;
-  return clamp_int(v, hi);
+  return CLAMP ? clamp_int(v, hi) : v;
 }
 
 // A ray of one lane, with its axes permuted so that `p` is the exit face's
@@ -239,11 +240,11 @@ struct RayLane {
   int ca, cb, cp, sp;    // bricked layouts: brick constants of a, b, p and stride of p
 };
 
-template <bool GUARD, bool WIDE, bool BIAS = true>
+template <bool GUARD, bool WIDE, bool BIAS = true, bool CLAMP = true>
 __device__ __forceinline__ int lane_offset(const RayLane& L, int inc) {
   const double incd = (double)inc;
-  const int ia = axis_index<GUARD, BIAS>(L.ba, incd, L.da, L.bias, L.ha);
-  const int ib = axis_index<GUARD, BIAS>(L.bb, incd, L.db, L.bias, L.hb);
+  const int ia = axis_index<GUARD, BIAS, CLAMP>(L.ba, incd, L.da, L.bias, L.ha);
+  const int ib = axis_index<GUARD, BIAS, CLAMP>(L.bb, incd, L.db, L.bias, L.hb);
   const int ip = min(inc, L.incmax);
   if (WIDE) return ia * L.sa + ib * L.sb + ip * L.pstep + L.pbase;
   // every factor fits 24 bits (checked on the host): full-rate v_mad_i32_i24
@@ -297,13 +298,13 @@ __device__ __forceinline__ int load_packed(const Volume& V, int f) {
 }
 
 // Label at increment inc of the lane's ray in the given layout.
-template <bool GUARD, int LAYOUT, bool BIAS = true>
+template <bool GUARD, int LAYOUT, bool BIAS = true, bool CLAMP = true>
 __device__ __forceinline__ int label_at(const RayLane& L, const Volume& V, int inc) {
   if (LAYOUT == kLayoutRaw || LAYOUT == kLayoutWide)
-    return load_label(V, lane_offset<GUARD, LAYOUT == kLayoutWide, BIAS>(L, inc));
+    return load_label(V, lane_offset<GUARD, LAYOUT == kLayoutWide, BIAS, CLAMP>(L, inc));
   const double incd = (double)inc;
-  const int ia = axis_index<GUARD, BIAS>(L.ba, incd, L.da, L.bias, L.ha);
-  const int ib = axis_index<GUARD, BIAS>(L.bb, incd, L.db, L.bias, L.hb);
+  const int ia = axis_index<GUARD, BIAS, CLAMP>(L.ba, incd, L.da, L.bias, L.ha);
+  const int ib = axis_index<GUARD, BIAS, CLAMP>(L.bb, incd, L.db, L.bias, L.hb);
   const int ip = mad24(L.pdir, min(inc, L.incmax), L.p0);
   // every factor fits 24 bits and every partial sum 31 (checked on the host)
   const int f = mad24(ia, L.sa, mad24(ia >> 3, L.ca, mad24(ib, L.sb, mad24(ib >> 3, L.cb,
@@ -318,7 +319,7 @@ __device__ __forceinline__ int label_at(const RayLane& L, const Volume& V, int i
 // len_path_2; out receives [li, lo, cr, bu].
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
-template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool BIAS>
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool BIAS, bool CLAMP>
 __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
                                               const double vs[3], lds_u32* seg, int& len_out,
                                               int& nseg_out, int last_zyx[3], double out[4]) {
@@ -344,7 +345,7 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
     int lab[U];
 #pragma unroll
     for (int u = 0; u < U; ++u)  // offsets stay inside the volume (clamped)
-      lab[u] = label_at<GUARD, LAYOUT, BIAS>(L, V, inc0 + u);
+      lab[u] = label_at<GUARD, LAYOUT, BIAS, CLAMP>(L, V, inc0 + u);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int inc = inc0 + u;
@@ -426,19 +427,57 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
   out[0] = li; out[1] = lo; out[2] = cr; out[3] = bu;
 }
 
-// The stepping loop in two forms: floor rounding (zero bias, one FP64 add per
-// axis and step fewer) and round-half-away.  The rounding is fixed per ray
-// and almost always the same across a wave, so the branch rarely diverges.
-template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT>
+// True when neither stepped coordinate of the lane's ray needs the clamp to
+// [0, extent-1] on increments 0 .. limit-1: each is monotone in the
+// increment, so its unclamped values at the two ends bound all the others.
+// Increments past the limit (the unrolled tail) are never taken, and their
+// loads stay range-checked.
+template <bool BIAS>
+__device__ __forceinline__ bool lane_in_range(const RayLane& L) {
+  const double i0 = 0.0, i1 = (double)(L.limit - 1);
+  const int a0 = axis_index<false, BIAS, false>(L.ba, i0, L.da, L.bias, L.ha);
+  const int a1 = axis_index<false, BIAS, false>(L.ba, i1, L.da, L.bias, L.ha);
+  const int b0 = axis_index<false, BIAS, false>(L.bb, i0, L.db, L.bias, L.hb);
+  const int b1 = axis_index<false, BIAS, false>(L.bb, i1, L.db, L.bias, L.hb);
+  return (unsigned)a0 <= (unsigned)L.ha && (unsigned)a1 <= (unsigned)L.ha &&
+         (unsigned)b0 <= (unsigned)L.hb && (unsigned)b1 <= (unsigned)L.hb;
+}
+
+// (trace_kernel and grid_map_kernel keep five waves per SIMD, at most 96
+// VGPRs, which the four inlined forms would otherwise exceed; the persistent
+// batch kernel, whose state does not fit that budget, uses the clamped forms
+// only: VOTE = false.)
+// The stepping loop in four forms: floor rounding (zero bias, one FP64 add
+// per axis and step fewer) or round-half-away, with or without the clamp of
+// the stepped coordinates.  The rounding is fixed per ray and almost always
+// the same across a wave; the clamp is dropped only when no lane of the
+// wave needs it (a wave vote), so that branch never diverges.
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool VOTE = true>
 __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, const double vs[3],
                                          lds_u32* seg, int& len_out, int& nseg_out,
                                          int last_zyx[3], double out[4]) {
-  if (L.bias != 0.0)
-    walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, true>(L, V, vs, seg, len_out, nseg_out, last_zyx,
-                                                     out);
-  else
-    walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, false>(L, V, vs, seg, len_out, nseg_out,
-                                                      last_zyx, out);
+  const bool round = L.bias != 0.0;
+  bool unclamped = false;
+  if (!GUARD && VOTE) {
+    const bool in_range = L.limit <= 0 || (round ? lane_in_range<true>(L)
+                                                 : lane_in_range<false>(L));
+    unclamped = __all(in_range) != 0;
+  }
+  if (unclamped) {
+    if (round)
+      walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, true, false>(L, V, vs, seg, len_out, nseg_out,
+                                                              last_zyx, out);
+    else
+      walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, false, false>(L, V, vs, seg, len_out,
+                                                               nseg_out, last_zyx, out);
+  } else {
+    if (round)
+      walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, true, true>(L, V, vs, seg, len_out, nseg_out,
+                                                             last_zyx, out);
+    else
+      walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, false, true>(L, V, vs, seg, len_out, nseg_out,
+                                                              last_zyx, out);
+  }
 }
 
 // Set up the lane's ray for exit face `face` (1..6) of rule `rr`; false when
@@ -530,7 +569,8 @@ struct TileAcc {
 
 // Trace the voxels [v_begin, v_end) of reflection `rule` with this lane's
 // stride; accumulate exp(-sum mu*L) (or write path records).
-template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS>
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS,
+          bool VOTE = true>
 __device__ __forceinline__ void trace_tile(const SampleView& S, const ReflRule* __restrict__ rule,
                                            int v_begin, int v_end, int is_exp, lds_u32* seg,
                                            TileAcc& acc, PathRecord* __restrict__ paths) {
@@ -558,7 +598,7 @@ __device__ __forceinline__ void trace_tile(const SampleView& S, const ReflRule* 
         verr = ANACOR_RAY_ERR_ASSERT;
         continue;
       }
-      walk_ray<CAP, U, GUARD, FULL, LAYOUT>(L, S.V, S.vs, seg, lens[k], nsegs[k], lasts[k],
+      walk_ray<CAP, U, GUARD, FULL, LAYOUT, VOTE>(L, S.V, S.vs, seg, lens[k], nsegs[k], lasts[k],
                                             Lr[k]);
       acc.steps += (unsigned long long)lens[k];
     }
@@ -655,7 +695,8 @@ __device__ __forceinline__ Volume make_volume(const uint8_t* vol, int Z, int Y, 
 }
 
 template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS>
-__global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void trace_kernel(TraceArgs a) {
   __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
   __shared__ ReduceLds red;
 
@@ -778,7 +819,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(BatchArgs a) {
     TileAcc acc = {0.f, 0.f, 0.0, 0ull, 0u};
     const int v_begin = tile * B.vpt;
     const int v_end = min(B.N, v_begin + B.vpt);
-    trace_tile<CAP, U, GUARD, FULL, WIDE ? kLayoutWide : kLayoutRaw, F64, false>(
+    trace_tile<CAP, U, GUARD, FULL, WIDE ? kLayoutWide : kLayoutRaw, F64, false, false>(
         S, B.rules + refl, v_begin, v_end, a.is_exp,
                                                       seg, acc, nullptr);
     reduce_tile<F64, false>(acc, red, &B.err[refl], a.steps, B.p32 + local, B.p64 + local);
@@ -849,7 +890,8 @@ struct GridArgs {
 };
 
 template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT>
-__global__ __launch_bounds__(kBlock) void grid_map_kernel(GridArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void grid_map_kernel(GridArgs a) {
   __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
   const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
   if (lb >= a.n_blocks) return;
