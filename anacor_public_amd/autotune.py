@@ -26,7 +26,7 @@ import numpy as np
 
 BLOCKS: Tuple[Tuple[int, int, int], ...] = ((1, 2, 64), (1, 1, 128), (2, 2, 32), (1, 4, 32),
                                              (2, 4, 16), (0, 0, 0))
-TILES: Tuple[int, ...] = (0, 4096, 8192)      # 0 = the library's automatic choice
+TILES: Tuple[int, ...] = (0, 512, 2048)       # 0 = the library's automatic choice (1,024 max)
 LAYOUTS: Tuple[int, ...] = (0, 1)             # raw, bricks (_lib.LAYOUT_RAW, _lib.LAYOUT_BRICK)
 
 _lock = threading.Lock()

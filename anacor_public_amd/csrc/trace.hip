@@ -1192,13 +1192,15 @@ static int ensure_run_buffers(Context* c, int64_t R, int64_t n_partials, bool f6
 
 // Voxel tile per workgroup: enough workgroups to fill 256 CUs several times
 // over, at least one voxel per lane, at most 64 voxels per lane.
+// Automatic: at most 4 voxels per lane (1,024 per tile; measured on the 256^3
+// workload: 80 ms at 1,024, 84 ms at 6,144, 108 ms from 8,192 up).
 static int choose_tile(const Context* c, int64_t R) {
   if (c->tile_voxels > 0) return c->tile_voxels;
   const int64_t target_blocks = 256 * 16;
   int64_t tiles = (target_blocks + R - 1) / std::max<int64_t>(R, 1);
   tiles = std::max<int64_t>(1, std::min<int64_t>(tiles, (c->N + kBlock - 1) / kBlock));
   int64_t vpt = (c->N + tiles - 1) / tiles;
-  vpt = std::min<int64_t>(std::max<int64_t>(vpt, kBlock), (int64_t)kBlock * 64);
+  vpt = std::min<int64_t>(std::max<int64_t>(vpt, kBlock), (int64_t)kBlock * 4);
   return (int)vpt;
 }
 
@@ -1795,6 +1797,8 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
   bool guard = false, wide = false;
   long long items = 0, total_refl = 0;
   b->h_desc.resize((size_t)S);
+  // staging buffers first (allocation stays on this thread), then the host
+  // setup and direction sort of the samples side by side, one thread each
   for (int i = 0; i < S; ++i) {
     Context* c = b->samples[(size_t)i];
     const int64_t Ri = R[i];
@@ -1803,14 +1807,33 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
       return ANACOR_ERR_ARG;
     }
     HIP_TRY(c->h_rules.reserve((size_t)std::max<int64_t>(Ri, 1)));
-    host_setup(s1[i], omega[i], Ri, raw_xray, omega_axis, F, max_dim(c), c->h_rules.data(),
-               nullptr, host_threads(0));
-    order_reflections(c, c->h_rules.data(), Ri);
     HIP_TRY(c->h_order_pin.reserve((size_t)std::max<int64_t>(Ri, 1)));
-    for (int64_t r = 0; r < Ri; ++r) {
-      c->h_order_pin[(size_t)r] = c->h_order[(size_t)r];
-      guard = guard || c->h_rules[(size_t)r].ray[0].guard || c->h_rules[(size_t)r].ray[1].guard;
-    }
+  }
+  std::vector<char> sample_guard((size_t)S, 0);
+  {
+    const int per_sample = std::max(1, host_threads(0) / std::max(S, 1));
+    auto prepare = [&](int i) {
+      Context* c = b->samples[(size_t)i];
+      const int64_t Ri = R[i];
+      host_setup(s1[i], omega[i], Ri, raw_xray, omega_axis, F, max_dim(c), c->h_rules.data(),
+                 nullptr, per_sample);
+      order_reflections(c, c->h_rules.data(), Ri);
+      bool g = false;
+      for (int64_t r = 0; r < Ri; ++r) {
+        c->h_order_pin[(size_t)r] = c->h_order[(size_t)r];
+        g = g || c->h_rules[(size_t)r].ray[0].guard || c->h_rules[(size_t)r].ray[1].guard;
+      }
+      sample_guard[(size_t)i] = g ? 1 : 0;
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < S; ++i) pool.emplace_back(prepare, i);
+    if (S > 0) prepare(0);
+    for (auto& th : pool) th.join();
+  }
+  for (int i = 0; i < S; ++i) {
+    Context* c = b->samples[(size_t)i];
+    const int64_t Ri = R[i];
+    guard = guard || sample_guard[(size_t)i] != 0;
     wide = wide || (int64_t)c->Y * c->X >= (1 << 23) || c->Z >= (1 << 23);
     const int vpt = c->N > 0 ? std::min(c->tile_voxels > 0 ? c->tile_voxels : kBlock * 64,
                                         std::max(c->N, 1))
