@@ -443,10 +443,10 @@ __device__ __forceinline__ bool lane_in_range(const RayLane& L) {
          (unsigned)b0 <= (unsigned)L.hb && (unsigned)b1 <= (unsigned)L.hb;
 }
 
-// (trace_kernel and grid_map_kernel keep five waves per SIMD, at most 96
-// VGPRs, which the four inlined forms would otherwise exceed; the persistent
-// batch kernel, whose state does not fit that budget, uses the clamped forms
-// only: VOTE = false.)
+// (trace_kernel and grid_map_kernel keep five waves per SIMD in the raw
+// layouts, at most 96 VGPRs, which the four inlined forms would otherwise
+// exceed; the persistent batch kernel, whose state does not fit that budget,
+// uses the clamped forms only: VOTE = false.)
 // The stepping loop in four forms: floor rounding (zero bias, one FP64 add
 // per axis and step fewer) or round-half-away, with or without the clamp of
 // the stepped coordinates.  The rounding is fixed per ray and almost always
@@ -458,7 +458,9 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
                                          int last_zyx[3], double out[4]) {
   const bool round = L.bias != 0.0;
   bool unclamped = false;
-  if (!GUARD && VOTE) {
+  // raw layouts only: the bricked walk waits on memory, not on VALU issue,
+  // and measured no faster without the clamp (1000x1000x800 volume)
+  if (!GUARD && VOTE && LAYOUT < kLayoutBrick2) {
     const bool in_range = L.limit <= 0 || (round ? lane_in_range<true>(L)
                                                  : lane_in_range<false>(L));
     unclamped = __all(in_range) != 0;
@@ -694,8 +696,13 @@ __device__ __forceinline__ Volume make_volume(const uint8_t* vol, int Z, int Y, 
   return V;
 }
 
+// Raw layouts: at least five waves per SIMD (the walk is bound by VALU
+// issue).  Bricked layouts: no bound (1), so the compiler keeps the registers
+// for eight gathers in flight per lane (the walk waits on the L2 and the
+// Infinity Cache there): 110 VGPRs, four waves.
 template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(LAYOUT >= kLayoutBrick2 ? 1 : 5)))
 void trace_kernel(TraceArgs a) {
   __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
   __shared__ ReduceLds red;
@@ -890,7 +897,8 @@ struct GridArgs {
 };
 
 template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(LAYOUT >= kLayoutBrick2 ? 1 : 5)))
 void grid_map_kernel(GridArgs a) {
   __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
   const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
@@ -1192,6 +1200,21 @@ static int ensure_run_buffers(Context* c, int64_t R, int64_t n_partials, bool f6
 
 // Voxel tile per workgroup: enough workgroups to fill 256 CUs several times
 // over, at least one voxel per lane, at most 64 voxels per lane.
+// A dispatch holds at most 2^32 - 1 work-items: 2^24 - 8 workgroups of 256
+// (a multiple of 8, for the XCD mapping of xcd_logical_block).
+constexpr int64_t kMaxBlocks = (1ll << 24) - 8;
+
+// The tile (voxels per workgroup) for R rows of work over n voxels: vpt,
+// enlarged until R * tiles fits one dispatch; 0 when even one tile per row
+// does not fit (more than kMaxBlocks rows).
+static int fit_tile(int64_t n, int64_t R, int vpt) {
+  if (n <= 0 || R <= 0) return vpt;
+  if (R * ((n + vpt - 1) / vpt) <= kMaxBlocks) return vpt;
+  const int64_t max_tiles = kMaxBlocks / R;
+  if (max_tiles < 1) return 0;
+  return (int)((n + max_tiles - 1) / max_tiles);
+}
+
 // Automatic: at most 4 voxels per lane (1,024 per tile; measured on the 256^3
 // workload: 80 ms at 1,024, 84 ms at 6,144, 108 ms from 8,192 up).
 static int choose_tile(const Context* c, int64_t R) {
@@ -1370,13 +1393,14 @@ static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flag
     for (int64_t i = 0; i < R; ++i) out ? (void)(out[i] = __builtin_nan("")) : (void)0;
     return ANACOR_OK;
   }
-  const int vpt = paths ? kBlock : choose_tile(c, R);
-  const int64_t tiles = (c->N + vpt - 1) / vpt;
-  const int64_t nblk = R * tiles;
-  if (nblk > (int64_t)0x7fffffff - 8) {
-    set_error(ANACOR_ERR_ARG, "too many workgroups (%lld)", (long long)nblk);
+  const int vpt = fit_tile(c->N, R, paths ? kBlock : choose_tile(c, R));
+  if (vpt == 0) {
+    set_error(ANACOR_ERR_ARG, "%lld reflections in one call exceed one dispatch (at most %lld)",
+              (long long)R, (long long)kMaxBlocks);
     return ANACOR_ERR_ARG;
   }
+  const int64_t tiles = (c->N + vpt - 1) / vpt;
+  const int64_t nblk = R * tiles;
   int rc = ensure_run_buffers(c, R, nblk, f64);
   if (rc) return rc;
   order_reflections(c, h_rules_in, R);
@@ -2038,16 +2062,16 @@ static int grid_build(Context* c, int64_t n_theta, int64_t n_phi, const double* 
     return fail(e, "hipMemsetAsync");
   std::vector<unsigned> h_err((size_t)D, 0u);
   if (N > 0) {
-    const int vpt = choose_tile(c, D);
-    const int64_t tiles = (N + vpt - 1) / vpt;
-    const int64_t nblk = (int64_t)D * tiles;
-    if (nblk > (int64_t)0x7fffffff - 8) {
+    const int vpt = fit_tile(N, D, choose_tile(c, D));
+    if (vpt == 0) {
       (void)hipFree(d_rays);
       (void)hipFree(d_err);
       grid_free(g);
-      set_error(ANACOR_ERR_ARG, "gridding: too many workgroups");
+      set_error(ANACOR_ERR_ARG, "gridding: %lld directions exceed one dispatch", (long long)D);
       return ANACOR_ERR_ARG;
     }
+    const int64_t tiles = (N + vpt - 1) / vpt;
+    const int64_t nblk = (int64_t)D * tiles;
     GridArgs a;
     const bool bricked = c->layout == kLayoutBrick2 || c->layout == kLayoutBrick4;
     a.vol = bricked ? c->d_pack : c->d_vol;
@@ -2130,6 +2154,11 @@ static int grid_lookup(hipStream_t stream, const float* map, const double* map64
   }
   if (L.nx < 2 || L.ny < 2 || !L.theta_list || !L.phi_list) {
     set_error(ANACOR_ERR_ARG, "bad grid lists");
+    return ANACOR_ERR_ARG;
+  }
+  if (R > kMaxBlocks) {   // one workgroup per reflection
+    set_error(ANACOR_ERR_ARG, "%lld reflections in one call exceed one dispatch (at most %lld)",
+              (long long)R, (long long)kMaxBlocks);
     return ANACOR_ERR_ARG;
   }
   const size_t need = (size_t)R * (4 * sizeof(double) + sizeof(double) + sizeof(unsigned)) + 64;
@@ -2498,13 +2527,14 @@ static int ib_run(Context* c, int64_t R, int num_cls, int flags, double* out, do
     for (int64_t i = 0; i < R; ++i) out[i] = __builtin_nan("");
     return ANACOR_OK;
   }
-  const int vpt = choose_tile(c, R);
-  const int64_t tiles = (c->N + vpt - 1) / vpt;
-  const int64_t nblk = R * tiles;
-  if (nblk > (int64_t)0x7fffffff - 8) {
-    set_error(ANACOR_ERR_ARG, "too many workgroups (%lld)", (long long)nblk);
+  const int vpt = fit_tile(c->N, R, choose_tile(c, R));
+  if (vpt == 0) {
+    set_error(ANACOR_ERR_ARG, "%lld reflections in one call exceed one dispatch (at most %lld)",
+              (long long)R, (long long)kMaxBlocks);
     return ANACOR_ERR_ARG;
   }
+  const int64_t tiles = (c->N + vpt - 1) / vpt;
+  const int64_t nblk = R * tiles;
   int rc = ensure_run_buffers(c, R, nblk, f64);
   if (rc) return rc;
   if ((size_t)R > c->ib_cap) {
