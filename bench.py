@@ -47,7 +47,10 @@ MLP_FLOPS_PER_ROW = 2 * (12 * 256 + 3 * 256 * 256 + 256)   # 2 x 199,936 MACs
 # L2<->fabric bytes per voxel step of trace_kernel on the 256^3 workload:
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a 1 GiB copy) over
 # 1.017e11 voxel steps, profiles/trace_kernel_plain-256cube.md.
-TRAFFIC_BYTES_PER_STEP = {"plain-256^3-stride2": 0.883}
+TRAFFIC_BYTES_PER_STEP = {"plain-256^3-stride2": 0.130}
+# The same per label read of ib_kernel (2-bit bricks), over 2.207e10 reads,
+# profiles/ib_kernel_bisection-256cube.md.
+IB_TRAFFIC_BYTES_PER_READ = {"plain-256^3-stride2": 0.00158}
 METRIC = "absorption factors/sec (rays/s) at 1 GPU, plain vs surrogate at stated rel. error"
 
 
@@ -293,7 +296,10 @@ def run_bisection(steps: int, warmup: int, cpu_budget: float, walk_value: float)
            "label_reads_per_s": reads / (k_ms * 1e-3), "volume_layout": layout,
            "speedup_vs_plain_walk": value / walk_value,
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": (IB_TRAFFIC_BYTES_PER_READ[wl.name] * reads
+                                    if wl.name in IB_TRAFFIC_BYTES_PER_READ and layout == "brick2"
+                                    else None),
                         "kernel": "ib_kernel", "kernel_ms": k_ms, "bytes_per_launch": int(reads)}}
     log(f"# bisection: {value:.3e} A_h/s ({rec['speedup_vs_plain_walk']:.2f}x the walk), "
         f"kernel {k_ms:.2f} ms, {reads} label reads, layout {layout}")
