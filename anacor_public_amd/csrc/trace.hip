@@ -1228,13 +1228,15 @@ static int choose_tile(const Context* c, int64_t R) {
 }
 
 constexpr int kCap = 16;     // boundaries kept per lane before the re-walk fallback
-constexpr int kUnroll = 4;   // label gathers in flight per lane, raw layouts
+constexpr int kUnroll = 4;   // label gathers in flight per lane, batch kernel
+constexpr int kUnrollRaw = 6;  // the same, trace and map kernels in the raw layouts
 
-// Label gathers in flight per lane: the raw walk is bound by VALU issue (2,
-// 6 and 8 measured slower than 4), the bricked walk waits on L2/Infinity
-// Cache latency with VALU to spare (1000x1000x800, 1,000 reflections: 164 ms
-// with 4, 140 ms with 8).
-constexpr int unroll_for(int layout) { return layout >= kLayoutBrick2 ? 8 : kUnroll; }
+// Label gathers in flight per lane.  Raw layouts (256^3 workload, clamp-free
+// forms, 1,024-voxel tiles): 2: 91.9 ms, 3: 83.8, 4: 80.1, 5: 78.8, 6: 78.4,
+// 8: 78.4.  The bricked walk waits on L2/Infinity Cache latency with VALU to
+// spare (1000x1000x800, 1,000 reflections: 164 ms with 4, 140 ms with 8).
+// The persistent batch kernel keeps 4 (its registers are at the limit).
+constexpr int unroll_for(int layout) { return layout >= kLayoutBrick2 ? 8 : kUnrollRaw; }
 
 template <bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS>
 static void launch_trace(const TraceArgs& a, int grid, hipStream_t s) {
