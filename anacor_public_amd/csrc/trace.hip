@@ -445,14 +445,13 @@ __device__ __forceinline__ bool lane_in_range(const RayLane& L) {
 
 // (trace_kernel and grid_map_kernel keep five waves per SIMD in the raw
 // layouts, at most 96 VGPRs, which the four inlined forms would otherwise
-// exceed; the persistent batch kernel, whose state does not fit that budget,
-// uses the clamped forms only: VOTE = false.)
+// exceed.)
 // The stepping loop in four forms: floor rounding (zero bias, one FP64 add
 // per axis and step fewer) or round-half-away, with or without the clamp of
 // the stepped coordinates.  The rounding is fixed per ray and almost always
 // the same across a wave; the clamp is dropped only when no lane of the
 // wave needs it (a wave vote), so that branch never diverges.
-template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool VOTE = true>
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT>
 __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, const double vs[3],
                                          lds_u32* seg, int& len_out, int& nseg_out,
                                          int last_zyx[3], double out[4]) {
@@ -460,7 +459,7 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
   bool unclamped = false;
   // raw layouts only: the bricked walk waits on memory, not on VALU issue,
   // and measured no faster without the clamp (1000x1000x800 volume)
-  if (!GUARD && VOTE && LAYOUT < kLayoutBrick2) {
+  if (!GUARD && LAYOUT < kLayoutBrick2) {
     const bool in_range = L.limit <= 0 || (round ? lane_in_range<true>(L)
                                                  : lane_in_range<false>(L));
     unclamped = __all(in_range) != 0;
@@ -571,8 +570,7 @@ struct TileAcc {
 
 // Trace the voxels [v_begin, v_end) of reflection `rule` with this lane's
 // stride; accumulate exp(-sum mu*L) (or write path records).
-template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS,
-          bool VOTE = true>
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS>
 __device__ __forceinline__ void trace_tile(const SampleView& S, const ReflRule* __restrict__ rule,
                                            int v_begin, int v_end, int is_exp, lds_u32* seg,
                                            TileAcc& acc, PathRecord* __restrict__ paths) {
@@ -600,7 +598,7 @@ __device__ __forceinline__ void trace_tile(const SampleView& S, const ReflRule* 
         verr = ANACOR_RAY_ERR_ASSERT;
         continue;
       }
-      walk_ray<CAP, U, GUARD, FULL, LAYOUT, VOTE>(L, S.V, S.vs, seg, lens[k], nsegs[k], lasts[k],
+      walk_ray<CAP, U, GUARD, FULL, LAYOUT>(L, S.V, S.vs, seg, lens[k], nsegs[k], lasts[k],
                                             Lr[k]);
       acc.steps += (unsigned long long)lens[k];
     }
@@ -753,12 +751,14 @@ __global__ void finish_kernel(const float2* __restrict__ p32, const double* __re
 
 // ------------------------------------------------------------ batched path
 //
-// Several independent samples (own volume, voxel list, reflections) traced by
-// one persistent kernel.  Work items (sample, reflection, tile) are numbered
-// sample by sample, reflections in processing (direction) order, and split
-// into 8 contiguous chunks, one queue per XCD: workgroup b serves the queue of
-// the XCD it was dispatched to (b mod 8) and steals from the next queues once
-// its own is drained, so every workgroup ends once all queues are empty.
+// Several independent samples (own volume, voxel list, reflections) traced in
+// one HIP graph: a clear kernel, one trace_kernel launch per sample into its
+// context's buffers (each launch fills the GPU: thousands of workgroups), and
+// one finish kernel over every sample's reflections.  The graph is replayed
+// while the shapes and buffers stay the same.  (A persistent work-queue
+// kernel over all samples measured 96 ms against 77 ms for these launches on
+// the eight-sample workload: its state held 124 VGPRs and kept it from the
+// walk's faster forms.)
 
 struct BatchSample {
   const uint8_t* vol;
@@ -776,70 +776,12 @@ struct BatchSample {
   double* out;
 };
 
-struct BatchArgs {
-  const BatchSample* samples;
-  int S;
-  int is_exp;
-  long long total;             // work items
-  long long chunk;             // items per XCD queue
-  unsigned long long* cursor;  // [8] per-queue cursors
-  unsigned long long* steps;
-};
-
-template <int CAP, int U, bool GUARD, bool FULL, bool WIDE, bool F64>
-__global__ __launch_bounds__(kBlock) void batch_kernel(BatchArgs a) {
-  __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
-  __shared__ ReduceLds red;
-  __shared__ long long item_sh;
-  lds_u32* seg = (lds_u32*)(seg_lds + threadIdx.x);
-  int q = (int)(blockIdx.x & 7u);
-  int drained = 0;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      long long it = -1;
-      while (drained < 8) {
-        const unsigned long long k = atomicAdd(&a.cursor[q], 1ull);
-        const long long idx = (long long)q * a.chunk + (long long)k;
-        if ((long long)k < a.chunk && idx < a.total) { it = idx; break; }
-        q = (q + 1) & 7;      // this queue is empty: move on to the next one
-        ++drained;
-      }
-      item_sh = it;
-    }
-    __syncthreads();
-    const long long it = item_sh;
-    if (it < 0) break;        // uniform: every queue is drained
-    int s = 0;
-    while (s + 1 < a.S && a.samples[s + 1].item_base <= it) ++s;
-    const BatchSample& B = a.samples[s];
-    const int local = (int)(it - B.item_base);
-    const int pos = local / B.tiles;
-    const int tile = local - pos * B.tiles;
-    const int refl = B.order[pos];
-    SampleView S;
-    S.V = make_volume(B.vol, B.Z, B.Y, B.X, B.Z * B.Y * B.X, 0, 0);
-    S.coords = B.coords;
-    S.N = B.N;
-    S.rules = B.rules;
-    for (int i = 0; i < 3; ++i) S.vs[i] = B.vs[i];
-    for (int i = 0; i < 4; ++i) S.mu[i] = B.mu[i];
-    TileAcc acc = {0.f, 0.f, 0.0, 0ull, 0u};
-    const int v_begin = tile * B.vpt;
-    const int v_end = min(B.N, v_begin + B.vpt);
-    trace_tile<CAP, U, GUARD, FULL, WIDE ? kLayoutWide : kLayoutRaw, F64, false, false>(
-        S, B.rules + refl, v_begin, v_end, a.is_exp,
-                                                      seg, acc, nullptr);
-    reduce_tile<F64, false>(acc, red, &B.err[refl], a.steps, B.p32 + local, B.p64 + local);
-  }
-}
-
-// Clears the queue cursors, the step counter and every sample's error words
-// (a kernel rather than hipMemsetAsync, so the graph holds kernels only).
+// Clears the step counter and every sample's error words (a kernel rather
+// than hipMemsetAsync, so the graph holds kernels only).
 __global__ void batch_clear_kernel(const BatchSample* __restrict__ samples, int S,
-                                   unsigned long long* cursor, unsigned long long* steps) {
+                                   unsigned long long* steps) {
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (t < 8) cursor[t] = 0ull;
-  if (t == 8) *steps = 0ull;
+  if (t == 0) *steps = 0ull;
   for (int s = 0; s < S; ++s) {
     const BatchSample& B = samples[s];
     for (int r = t; r < B.R; r += (int)(gridDim.x * blockDim.x)) B.err[r] = 0u;
@@ -1228,14 +1170,12 @@ static int choose_tile(const Context* c, int64_t R) {
 }
 
 constexpr int kCap = 16;     // boundaries kept per lane before the re-walk fallback
-constexpr int kUnroll = 4;   // label gathers in flight per lane, batch kernel
-constexpr int kUnrollRaw = 6;  // the same, trace and map kernels in the raw layouts
+constexpr int kUnrollRaw = 6;  // label gathers in flight per lane, raw layouts
 
 // Label gathers in flight per lane.  Raw layouts (256^3 workload, clamp-free
 // forms, 1,024-voxel tiles): 2: 91.9 ms, 3: 83.8, 4: 80.1, 5: 78.8, 6: 78.4,
 // 8: 78.4.  The bricked walk waits on L2/Infinity Cache latency with VALU to
 // spare (1000x1000x800, 1,000 reflections: 164 ms with 4, 140 ms with 8).
-// The persistent batch kernel keeps 4 (its registers are at the limit).
 constexpr int unroll_for(int layout) { return layout >= kLayoutBrick2 ? 8 : kUnrollRaw; }
 
 template <bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS>
@@ -1245,7 +1185,6 @@ static void launch_trace(const TraceArgs& a, int grid, hipStream_t s) {
 }
 
 using LaunchFn = void (*)(const TraceArgs&, int, hipStream_t);
-using LaunchBatchFn = void (*)(const BatchArgs&, int, hipStream_t);
 
 // Index bits: guard 1, full 2, f64 4, paths 8, layout 16 * (0..3).  Path
 // records are written from the raw layouts only.
@@ -1710,26 +1649,6 @@ static const uint8_t* flatten_table(int8_t*** table, const int64_t shape[3],
 
 // ------------------------------------------------------------ batch (host)
 
-template <bool GUARD, bool FULL, bool WIDE, bool F64>
-static void launch_batch(const BatchArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((batch_kernel<kCap, kUnroll, GUARD, FULL, WIDE, F64>), dim3(grid),
-                     dim3(kBlock), 0, s, a);
-}
-
-template <int I>
-static constexpr LaunchBatchFn batch_launcher() {
-  return &launch_batch<(I & 1) != 0, (I & 2) != 0, (I & 4) != 0, (I & 8) != 0>;
-}
-
-template <int... I>
-static constexpr std::array<LaunchBatchFn, sizeof...(I)> batch_table(
-    std::integer_sequence<int, I...>) {
-  return {batch_launcher<I>()...};
-}
-
-static const std::array<LaunchBatchFn, 16> kLaunchBatch =
-    batch_table(std::make_integer_sequence<int, 16>{});
-
 struct Batch {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -1738,7 +1657,7 @@ struct Batch {
   int full = 0;
   BatchSample* d_desc = nullptr;
   size_t desc_cap = 0;
-  unsigned long long* d_counters = nullptr;   // [8] cursors, [8] steps
+  unsigned long long* d_counters = nullptr;   // the step counter
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<long long> graph_key;
@@ -1746,7 +1665,7 @@ struct Batch {
   float last_ms = 0.f;
   unsigned long long last_steps = 0;
   double last_host_ms = 0.0;
-  int grid = 0;
+  long long blocks = 0;                       // workgroups of the last run's launches
   std::vector<BatchSample> h_desc;
   std::mutex mu;
 };
@@ -1800,14 +1719,8 @@ static int batch_create(const anacor_sample_t* in, int n, int full, int device, 
   BATCH_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   BATCH_TRY(hipEventCreate(&b->ev0));
   BATCH_TRY(hipEventCreate(&b->ev1));
-  BATCH_TRY(hipMalloc(&b->d_counters, 16 * sizeof(unsigned long long)));
-  // persistent grid: every workgroup the chip holds at once, a multiple of 8
-  int cus = 0, per_cu = 0;
-  BATCH_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
-  BATCH_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, (const void*)batch_kernel<kCap, kUnroll, false, false, false, false>, kBlock, 0));
+  BATCH_TRY(hipMalloc(&b->d_counters, sizeof(unsigned long long)));
 #undef BATCH_TRY
-  b->grid = std::max(8, (cus * std::max(per_cu, 1)) / 8 * 8);
   *out = b;
   return ANACOR_OK;
 }
@@ -1861,9 +1774,15 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
     const int64_t Ri = R[i];
     guard = guard || sample_guard[(size_t)i] != 0;
     wide = wide || (int64_t)c->Y * c->X >= (1 << 23) || c->Z >= (1 << 23);
-    const int vpt = c->N > 0 ? std::min(c->tile_voxels > 0 ? c->tile_voxels : kBlock * 64,
-                                        std::max(c->N, 1))
-                             : 1;
+    int vpt = 1;
+    if (c->N > 0) {
+      vpt = fit_tile(c->N, Ri, choose_tile(c, std::max<int64_t>(Ri, 1)));
+      if (vpt == 0) {
+        set_error(ANACOR_ERR_ARG, "sample %d: %lld reflections exceed one dispatch", i,
+                  (long long)Ri);
+        return ANACOR_ERR_ARG;
+      }
+    }
     const int tiles = c->N > 0 ? (c->N + vpt - 1) / vpt : 0;
     const int rc = ensure_run_buffers(c, std::max<int64_t>(Ri, 1),
                                       std::max<int64_t>(Ri * tiles, 1), f64);
@@ -1904,6 +1823,7 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
                          hipMemcpyHostToDevice, b->stream));
 
   // the captured graph depends on the launch shape and every buffer address
+  b->blocks = items;
   std::vector<long long> key = {S, items, total_refl, f64, guard, wide, b->full,
                                 (long long)(uintptr_t)b->d_desc};
   for (int i = 0; i < S; ++i) {
@@ -1916,20 +1836,41 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
     if (b->graph) HIP_TRY(hipGraphDestroy(b->graph));
     b->exec = nullptr;
     b->graph = nullptr;
-    BatchArgs a;
-    a.samples = b->d_desc;
-    a.S = S;
-    a.is_exp = (flags & ANACOR_FLAG_NO_EXP) ? 0 : 1;
-    a.total = items;
-    a.chunk = (items + 7) / 8;
-    a.cursor = b->d_counters;
-    a.steps = b->d_counters + 8;
+    const int is_exp = (flags & ANACOR_FLAG_NO_EXP) ? 0 : 1;
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_clear_kernel, dim3(64), dim3(256), 0, b->stream, b->d_desc, S,
-                       a.cursor, a.steps);
+                       b->d_counters);
     if (items > 0) {
-      const int i = (guard ? 1 : 0) | (b->full ? 2 : 0) | (wide ? 4 : 0) | (f64 ? 8 : 0);
-      kLaunchBatch[i](a, b->grid, b->stream);
+      // one trace_kernel launch per sample, in order, each into its
+      // context's buffers; the finish kernel then serves every sample
+      for (int i = 0; i < S; ++i) {
+        Context* c = b->samples[(size_t)i];
+        const BatchSample& d = b->h_desc[(size_t)i];
+        if (d.R == 0 || c->N == 0) continue;
+        TraceArgs t;
+        t.vol = c->d_vol;
+        t.vol_bytes = c->Z * c->Y * c->X;
+        t.cz = t.cy = 0;
+        t.Z = c->Z; t.Y = c->Y; t.X = c->X;
+        t.coords = c->d_coords;
+        t.N = c->N;
+        t.rules = c->d_rules;
+        t.R = d.R;
+        t.tiles = d.tiles;
+        t.vox_per_tile = d.vpt;
+        t.n_blocks = d.R * d.tiles;
+        for (int k = 0; k < 3; ++k) t.vs[k] = c->vs[k];
+        for (int k = 0; k < 4; ++k) t.mu[k] = c->mu[k];
+        t.is_exp = is_exp;
+        t.partial32 = d.p32;
+        t.partial64 = d.p64;
+        t.err = d.err;
+        t.steps = b->d_counters;
+        t.paths = nullptr;
+        t.order = d.order;
+        dispatch_trace(t, (t.n_blocks + 7) / 8 * 8, sample_guard[(size_t)i] != 0, b->full != 0,
+                       raw_needs_wide(c) ? kLayoutWide : kLayoutRaw, f64, false, b->stream);
+      }
     }
     if (total_refl > 0)
       hipLaunchKernelGGL(batch_finish_kernel, dim3((unsigned)((total_refl + 255) / 256)),
@@ -1948,7 +1889,7 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
       HIP_TRY(hipMemcpyAsync(out[i], c->d_out, (size_t)R[i] * sizeof(double),
                              hipMemcpyDeviceToHost, b->stream));
   }
-  HIP_TRY(hipMemcpyAsync(&b->last_steps, b->d_counters + 8, sizeof(unsigned long long),
+  HIP_TRY(hipMemcpyAsync(&b->last_steps, b->d_counters, sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   HIP_TRY(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
@@ -2773,7 +2714,7 @@ int anacor_batch_stats(const anacor_batch* batch, anacor_stats_t* st) {
   if (!batch || !st) return ANACOR_ERR_ARG;
   st->kernel_ms = batch->b->last_ms;
   st->voxel_steps = batch->b->last_steps;
-  st->workgroups = batch->b->grid;
+  st->workgroups = batch->b->blocks;
   int64_t n = 0;
   for (Context* c : batch->b->samples) n += c->N;
   st->n_voxels = n;

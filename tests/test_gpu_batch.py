@@ -1,5 +1,5 @@
-"""GPU tests of the batched path (several samples, one persistent kernel in a
-replayed HIP graph) and of the autotuner.
+"""GPU tests of the batched path (several samples, per-sample trace launches
+in one replayed HIP graph) and of the autotuner.
 
 Each sample's result must equal the unbatched path on the same sample
 bit for bit (same voxel order and tiles, so the same partial sums), across
