@@ -751,14 +751,15 @@ __global__ void finish_kernel(const float2* __restrict__ p32, const double* __re
 
 // ------------------------------------------------------------ batched path
 //
-// Several independent samples (own volume, voxel list, reflections) traced in
-// one HIP graph: a clear kernel, one trace_kernel launch per sample into its
-// context's buffers (each launch fills the GPU: thousands of workgroups), and
-// one finish kernel over every sample's reflections.  The graph is replayed
-// while the shapes and buffers stay the same.  (A persistent work-queue
-// kernel over all samples measured 96 ms against 77 ms for these launches on
-// the eight-sample workload: its state held 124 VGPRs and kept it from the
-// walk's faster forms.)
+// Several independent samples (own volume, voxel list, reflections) traced
+// as a clear kernel, one trace_kernel launch per sample into its context's
+// buffers (each launch fills the GPU: thousands of workgroups), and one
+// finish kernel over every sample's reflections.  The host prepares sample
+// i + 1 while the GPU traces sample i; uploads run on a second stream ahead
+// of the launches.  (A persistent work-queue kernel over all samples
+// measured 96 ms against 77 ms for these launches on the eight-sample
+// workload: its state held 124 VGPRs and kept it from the walk's faster
+// forms.)
 
 struct BatchSample {
   const uint8_t* vol;
@@ -1658,9 +1659,8 @@ struct Batch {
   BatchSample* d_desc = nullptr;
   size_t desc_cap = 0;
   unsigned long long* d_counters = nullptr;   // the step counter
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  std::vector<long long> graph_key;
+  hipStream_t copy_stream = nullptr;          // rule uploads, ahead of the kernels
+  std::vector<hipEvent_t> copied;             // per sample: its upload is done
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
   unsigned long long last_steps = 0;
@@ -1674,8 +1674,9 @@ static void batch_destroy(Batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->copy_stream) (void)hipStreamSynchronize(b->copy_stream);
+  for (hipEvent_t e : b->copied) (void)hipEventDestroy(e);
+  if (b->copy_stream) (void)hipStreamDestroy(b->copy_stream);
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_counters) (void)hipFree(b->d_counters);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -1717,6 +1718,10 @@ static int batch_create(const anacor_sample_t* in, int n, int full, int device, 
   } while (0)
   BATCH_TRY(hipSetDevice(b->device));
   BATCH_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  BATCH_TRY(hipStreamCreateWithFlags(&b->copy_stream, hipStreamNonBlocking));
+  b->copied.assign((size_t)n, nullptr);
+  for (int i = 0; i < n; ++i)
+    BATCH_TRY(hipEventCreateWithFlags(&b->copied[(size_t)i], hipEventDisableTiming));
   BATCH_TRY(hipEventCreate(&b->ev0));
   BATCH_TRY(hipEventCreate(&b->ev1));
   BATCH_TRY(hipMalloc(&b->d_counters, sizeof(unsigned long long)));
@@ -1730,14 +1735,13 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
                      const double* F, int flags, double* const* out) {
   std::lock_guard<std::mutex> lk(b->mu);
   HIP_TRY(hipSetDevice(b->device));
-  const auto t_host = std::chrono::steady_clock::now();
   const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
+  const int is_exp = (flags & ANACOR_FLAG_NO_EXP) ? 0 : 1;
   const int S = (int)b->samples.size();
-  bool guard = false, wide = false;
   long long items = 0, total_refl = 0;
   b->h_desc.resize((size_t)S);
-  // staging buffers first (allocation stays on this thread), then the host
-  // setup and direction sort of the samples side by side, one thread each
+  // Staging buffers, tiles and device buffers of every sample first: the
+  // allocations stay on this thread and ahead of every launch.
   for (int i = 0; i < S; ++i) {
     Context* c = b->samples[(size_t)i];
     const int64_t Ri = R[i];
@@ -1747,33 +1751,6 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
     }
     HIP_TRY(c->h_rules.reserve((size_t)std::max<int64_t>(Ri, 1)));
     HIP_TRY(c->h_order_pin.reserve((size_t)std::max<int64_t>(Ri, 1)));
-  }
-  std::vector<char> sample_guard((size_t)S, 0);
-  {
-    const int per_sample = std::max(1, host_threads(0) / std::max(S, 1));
-    auto prepare = [&](int i) {
-      Context* c = b->samples[(size_t)i];
-      const int64_t Ri = R[i];
-      host_setup(s1[i], omega[i], Ri, raw_xray, omega_axis, F, max_dim(c), c->h_rules.data(),
-                 nullptr, per_sample);
-      order_reflections(c, c->h_rules.data(), Ri);
-      bool g = false;
-      for (int64_t r = 0; r < Ri; ++r) {
-        c->h_order_pin[(size_t)r] = c->h_order[(size_t)r];
-        g = g || c->h_rules[(size_t)r].ray[0].guard || c->h_rules[(size_t)r].ray[1].guard;
-      }
-      sample_guard[(size_t)i] = g ? 1 : 0;
-    };
-    std::vector<std::thread> pool;
-    for (int i = 1; i < S; ++i) pool.emplace_back(prepare, i);
-    if (S > 0) prepare(0);
-    for (auto& th : pool) th.join();
-  }
-  for (int i = 0; i < S; ++i) {
-    Context* c = b->samples[(size_t)i];
-    const int64_t Ri = R[i];
-    guard = guard || sample_guard[(size_t)i] != 0;
-    wide = wide || (int64_t)c->Y * c->X >= (1 << 23) || c->Z >= (1 << 23);
     int vpt = 1;
     if (c->N > 0) {
       vpt = fit_tile(c->N, Ri, choose_tile(c, std::max<int64_t>(Ri, 1)));
@@ -1787,12 +1764,6 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
     const int rc = ensure_run_buffers(c, std::max<int64_t>(Ri, 1),
                                       std::max<int64_t>(Ri * tiles, 1), f64);
     if (rc) return rc;
-    if (Ri > 0) {
-      HIP_TRY(hipMemcpyAsync(c->d_rules, c->h_rules.data(), (size_t)Ri * sizeof(ReflRule),
-                             hipMemcpyHostToDevice, b->stream));
-      HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order_pin.data(), (size_t)Ri * sizeof(int),
-                             hipMemcpyHostToDevice, b->stream));
-    }
     BatchSample& d = b->h_desc[(size_t)i];
     d.vol = c->d_vol;
     d.Z = c->Z; d.Y = c->Y; d.X = c->X; d.N = c->N;
@@ -1819,69 +1790,75 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
     HIP_TRY(hipMalloc(&b->d_desc, (size_t)S * sizeof(BatchSample)));
     b->desc_cap = (size_t)S;
   }
+  b->blocks = items;
   HIP_TRY(hipMemcpyAsync(b->d_desc, b->h_desc.data(), (size_t)S * sizeof(BatchSample),
                          hipMemcpyHostToDevice, b->stream));
-
-  // the captured graph depends on the launch shape and every buffer address
-  b->blocks = items;
-  std::vector<long long> key = {S, items, total_refl, f64, guard, wide, b->full,
-                                (long long)(uintptr_t)b->d_desc};
+  hipLaunchKernelGGL(batch_clear_kernel, dim3(64), dim3(256), 0, b->stream, b->d_desc, S,
+                     b->d_counters);
+  HIP_TRY(hipGetLastError());
+  bool timing = false;   // ev0 goes in behind the first sample's upload
+  // Sample by sample: host setup and direction sort on every host thread,
+  // the upload on the copy stream, then the sample's trace launch behind
+  // that upload.  The host prepares sample i + 1 while the GPU traces i.
+  double host_ms = 0.0;
   for (int i = 0; i < S; ++i) {
-    const BatchSample& d = b->h_desc[(size_t)i];
-    key.push_back(d.R);
-    key.push_back(d.tiles);
-  }
-  if (key != b->graph_key || !b->exec) {
-    if (b->exec) HIP_TRY(hipGraphExecDestroy(b->exec));
-    if (b->graph) HIP_TRY(hipGraphDestroy(b->graph));
-    b->exec = nullptr;
-    b->graph = nullptr;
-    const int is_exp = (flags & ANACOR_FLAG_NO_EXP) ? 0 : 1;
-    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(batch_clear_kernel, dim3(64), dim3(256), 0, b->stream, b->d_desc, S,
-                       b->d_counters);
-    if (items > 0) {
-      // one trace_kernel launch per sample, in order, each into its
-      // context's buffers; the finish kernel then serves every sample
-      for (int i = 0; i < S; ++i) {
-        Context* c = b->samples[(size_t)i];
-        const BatchSample& d = b->h_desc[(size_t)i];
-        if (d.R == 0 || c->N == 0) continue;
-        TraceArgs t;
-        t.vol = c->d_vol;
-        t.vol_bytes = c->Z * c->Y * c->X;
-        t.cz = t.cy = 0;
-        t.Z = c->Z; t.Y = c->Y; t.X = c->X;
-        t.coords = c->d_coords;
-        t.N = c->N;
-        t.rules = c->d_rules;
-        t.R = d.R;
-        t.tiles = d.tiles;
-        t.vox_per_tile = d.vpt;
-        t.n_blocks = d.R * d.tiles;
-        for (int k = 0; k < 3; ++k) t.vs[k] = c->vs[k];
-        for (int k = 0; k < 4; ++k) t.mu[k] = c->mu[k];
-        t.is_exp = is_exp;
-        t.partial32 = d.p32;
-        t.partial64 = d.p64;
-        t.err = d.err;
-        t.steps = b->d_counters;
-        t.paths = nullptr;
-        t.order = d.order;
-        dispatch_trace(t, (t.n_blocks + 7) / 8 * 8, sample_guard[(size_t)i] != 0, b->full != 0,
-                       raw_needs_wide(c) ? kLayoutWide : kLayoutRaw, f64, false, b->stream);
-      }
+    Context* c = b->samples[(size_t)i];
+    const int64_t Ri = R[i];
+    const auto t0 = std::chrono::steady_clock::now();
+    host_setup(s1[i], omega[i], Ri, raw_xray, omega_axis, F, max_dim(c), c->h_rules.data(),
+               nullptr, host_threads(0));
+    order_reflections(c, c->h_rules.data(), Ri);
+    bool guard = false;
+    for (int64_t r = 0; r < Ri; ++r) {
+      c->h_order_pin[(size_t)r] = c->h_order[(size_t)r];
+      guard = guard || c->h_rules[(size_t)r].ray[0].guard || c->h_rules[(size_t)r].ray[1].guard;
     }
-    if (total_refl > 0)
-      hipLaunchKernelGGL(batch_finish_kernel, dim3((unsigned)((total_refl + 255) / 256)),
-                         dim3(256), 0, b->stream, b->d_desc, S, f64 ? 1 : 0, total_refl);
-    HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
-    HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    b->graph_key = key;
+    host_ms += ms_since(t0);
+    if (Ri == 0) continue;
+    HIP_TRY(hipMemcpyAsync(c->d_rules, c->h_rules.data(), (size_t)Ri * sizeof(ReflRule),
+                           hipMemcpyHostToDevice, b->copy_stream));
+    HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order_pin.data(), (size_t)Ri * sizeof(int),
+                           hipMemcpyHostToDevice, b->copy_stream));
+    HIP_TRY(hipEventRecord(b->copied[(size_t)i], b->copy_stream));
+    HIP_TRY(hipStreamWaitEvent(b->stream, b->copied[(size_t)i], 0));
+    if (c->N == 0) continue;
+    if (!timing) {
+      HIP_TRY(hipEventRecord(b->ev0, b->stream));
+      timing = true;
+    }
+    const BatchSample& d = b->h_desc[(size_t)i];
+    TraceArgs t;
+    t.vol = c->d_vol;
+    t.vol_bytes = c->Z * c->Y * c->X;
+    t.cz = t.cy = 0;
+    t.Z = c->Z; t.Y = c->Y; t.X = c->X;
+    t.coords = c->d_coords;
+    t.N = c->N;
+    t.rules = c->d_rules;
+    t.R = d.R;
+    t.tiles = d.tiles;
+    t.vox_per_tile = d.vpt;
+    t.n_blocks = d.R * d.tiles;
+    for (int k = 0; k < 3; ++k) t.vs[k] = c->vs[k];
+    for (int k = 0; k < 4; ++k) t.mu[k] = c->mu[k];
+    t.is_exp = is_exp;
+    t.partial32 = d.p32;
+    t.partial64 = d.p64;
+    t.err = d.err;
+    t.steps = b->d_counters;
+    t.paths = nullptr;
+    t.order = d.order;
+    dispatch_trace(t, (t.n_blocks + 7) / 8 * 8, guard, b->full != 0,
+                   raw_needs_wide(c) ? kLayoutWide : kLayoutRaw, f64, false, b->stream);
+    HIP_TRY(hipGetLastError());
   }
-  b->last_host_ms = ms_since(t_host);
-  HIP_TRY(hipEventRecord(b->ev0, b->stream));
-  HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  if (!timing) HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  if (total_refl > 0) {
+    hipLaunchKernelGGL(batch_finish_kernel, dim3((unsigned)((total_refl + 255) / 256)),
+                       dim3(256), 0, b->stream, b->d_desc, S, f64 ? 1 : 0, total_refl);
+    HIP_TRY(hipGetLastError());
+  }
+  b->last_host_ms = host_ms;
   HIP_TRY(hipEventRecord(b->ev1, b->stream));
   for (int i = 0; i < S; ++i) {
     Context* c = b->samples[(size_t)i];

@@ -347,8 +347,8 @@ def reflection_geometry(s1, omega, raw_xray, omega_axis, F, max_dim: int):
 
 
 class Batch:
-    """Several independent samples traced by one replayed HIP graph of
-    per-sample trace launches (``anacor_batch_*``).  ``samples`` are Workload-like objects
+    """Several independent samples traced by per-sample trace launches
+    pipelined behind the host setup (``anacor_batch_*``).  ``samples`` are Workload-like objects
     (volume, coords, voxel_size, coefficients); all share full_iteration."""
 
     def __init__(self, samples, full_iteration: int = 0, device: int = 0):

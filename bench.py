@@ -24,8 +24,8 @@ Workloads (anacor_public_amd/workload.py):
   single   256^3 volume, 25,323 crystal voxels (stride 2), 20,000 reflections (default)
   small    64^3 volume, all 3,169 crystal voxels, 256 reflections
   large    1000x1000x800 volume, 100,000 voxels, 100,000 reflections
-  batched  8 uneven samples (100^3 .. 383x383x135), 194,281 reflections, one
-           HIP graph of per-sample trace launches, autotuned work shapes
+  batched  8 uneven samples (100^3 .. 383x383x135), 194,281 reflections,
+           per-sample trace launches behind the host setup, autotuned work shapes
   surrogate  MLP surrogate of batched sample 2, 1M inferences with tracer fallback
 """
 from __future__ import annotations
@@ -473,7 +473,7 @@ def main():
         "voxel_steps_per_s": vsteps / (ms_per_step * 1e-3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "trace_kernel per sample (one graph)" if args.config == "batched"
+                     "kernel": "trace_kernel per sample" if args.config == "batched"
                      else "trace_kernel",
                      "kernel_ms": k_ms, "bytes_per_launch": vsteps},
         "cpu_baseline": cpu,

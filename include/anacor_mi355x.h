@@ -175,9 +175,9 @@ typedef struct {
     double         coefficients[4];
 } anacor_sample_t;
 
-/* Several samples traced by one captured HIP graph (one trace launch per
- * sample, then one finish kernel over all of them) that is replayed while the
- * launch shapes are unchanged.
+/* Several samples traced by one trace launch per sample, each behind its
+ * own upload and overlapping the host setup of the next, then one finish
+ * kernel over all of them.
  * No counterpart in the reference, whose driver runs samples one after the
  * other (main_lite.py:276-341).  All samples share full_iteration. */
 typedef struct anacor_batch anacor_batch;

@@ -1,9 +1,9 @@
 """GPU tests of the batched path (several samples, per-sample trace launches
-in one replayed HIP graph) and of the autotuner.
+pipelined behind the host setup) and of the autotuner.
 
 Each sample's result must equal the unbatched path on the same sample
 bit for bit (same voxel order and tiles, so the same partial sums), across
-graph replays with other GPU work in between, and match the FP64 oracle to
+repeated runs with other GPU work in between, and match the FP64 oracle to
 1e-6."""
 import dataclasses
 import sys
@@ -56,9 +56,9 @@ def test_batch_matches_unbatched_and_oracle(gpu, oracle, samples):
     assert st["voxel_steps"] == total_steps
 
 
-def test_graph_replays_with_eager_work_between(gpu, samples):
+def test_repeated_runs_with_eager_work_between(gpu, samples):
     from anacor_public_amd.ray_tracing import Batch, Tracer
-    say("case: graph replays")
+    say("case: repeated batch runs")
     wl0 = samples[0]
     args = ([w.s1 for w in samples], [w.omega for w in samples], wl0.raw_xray, wl0.omega_axis,
             wl0.F)
@@ -69,13 +69,14 @@ def test_graph_replays_with_eager_work_between(gpu, samples):
             again = b.run(*args)
             for i in range(len(samples)):
                 np.testing.assert_array_equal(bits(again[i]), bits(first[i]),
-                                              err_msg=f"replay {rep} sample {i}")
-        # a new launch shape rebuilds the graph
+                                              err_msg=f"run {rep} sample {i}")
+        # another launch shape (5 reflections: smaller automatic tiles, so
+        # other FP32 partial sums)
         short = b.run([w.s1[:5] for w in samples], [w.omega[:5] for w in samples],
                       *args[2:])
         for i in range(len(samples)):
-            np.testing.assert_array_equal(bits(short[i]), bits(first[i][:5]))
-        # FP64 mean through the same graph machinery
+            np.testing.assert_allclose(short[i], first[i][:5], rtol=1e-7, atol=0)
+        # FP64 mean through the same launches
         f64 = b.run(*args, f64_mean=True)
         for i in range(len(samples)):
             np.testing.assert_allclose(f64[i], first[i], rtol=1e-6, atol=0)
