@@ -761,20 +761,14 @@ __global__ void finish_kernel(const float2* __restrict__ p32, const double* __re
 // workload: its state held 124 VGPRs and kept it from the walk's faster
 // forms.)
 
+// What the clear and finish kernels need of each sample.
 struct BatchSample {
-  const uint8_t* vol;
-  int Z, Y, X, N;
-  const int4* coords;
-  const ReflRule* rules;
-  const int* order;
-  int R, tiles, vpt, pad;
-  long long item_base;
-  double vs[3];
-  double mu[4];
-  float2* p32;
+  const int* order;            // processing position -> reflection
+  int N, R, tiles, vpt;
+  float2* p32;                 // tile partials [R][tiles]
   double* p64;
-  unsigned* err;
-  double* out;
+  unsigned* err;               // [R]
+  double* out;                 // [R]
 };
 
 // Clears the step counter and every sample's error words (a kernel rather
@@ -1765,18 +1759,11 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
                                       std::max<int64_t>(Ri * tiles, 1), f64);
     if (rc) return rc;
     BatchSample& d = b->h_desc[(size_t)i];
-    d.vol = c->d_vol;
-    d.Z = c->Z; d.Y = c->Y; d.X = c->X; d.N = c->N;
-    d.coords = c->d_coords;
-    d.rules = c->d_rules;
     d.order = c->d_order;
+    d.N = c->N;
     d.R = (int)Ri;
     d.tiles = std::max(tiles, 1);
     d.vpt = vpt;
-    d.pad = 0;
-    d.item_base = items;
-    for (int k = 0; k < 3; ++k) d.vs[k] = c->vs[k];
-    for (int k = 0; k < 4; ++k) d.mu[k] = c->mu[k];
     d.p32 = (float2*)c->d_partial;
     d.p64 = (double*)c->d_partial;
     d.err = c->d_err;
