@@ -29,6 +29,8 @@ LAYOUT_BRICK2 = 2
 LAYOUT_BRICK4 = 4
 RAY_ERR_FACE = 1
 RAY_ERR_ASSERT = 2
+GATHER_LANES = 0
+GATHER_ROW = 1
 
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTED = (
@@ -44,7 +46,7 @@ EXPORTED = (
     "anacor_mlp_gradients", "anacor_mlp_train", "anacor_mlp_predict", "anacor_mlp_stats",
     "ray_tracing_single_gridding", "nearest_neighbor_interpolate_overall", "anacor_grid_create",
     "anacor_grid_destroy", "anacor_grid_map", "anacor_grid_run", "anacor_grid_stats",
-    "ib_test", "anacor_ctx_bisection", "anacor_ctx_bisection_rotated",
+    "ib_test", "anacor_ctx_bisection", "anacor_ctx_bisection_rotated", "anacor_probe_gather",
 )
 
 MLP_INPUTS = 12
@@ -194,6 +196,8 @@ def _prototypes(lib):
     lib.anacor_ctx_bisection.argtypes = [vp, vp, vp, i64, vp, vp, vp, i32, i32, vp]
     lib.anacor_ctx_bisection_rotated.restype = ct.c_int
     lib.anacor_ctx_bisection_rotated.argtypes = [vp, vp, vp, i64, i32, i32, vp, vp]
+    lib.anacor_probe_gather.restype = ct.c_int
+    lib.anacor_probe_gather.argtypes = [i32, i32, i32, i64, ct.POINTER(ct.c_double)]
     lib.free.restype = None
     lib.free.argtypes = [vp]
     return lib

@@ -36,7 +36,7 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contr
              "-Wall", "-Wno-unused-result"]
 
 
-HIP_SOURCES = ("trace.hip", "mlp.hip")   # tracer + C-ABI; learned surrogate
+HIP_SOURCES = ("trace.hip", "mlp.hip", "probe.hip")   # tracer + C-ABI; surrogate; gather probe
 
 
 def _run(cmd):

@@ -47,10 +47,10 @@ MLP_FLOPS_PER_ROW = 2 * (12 * 256 + 3 * 256 * 256 + 256)   # 2 x 199,936 MACs
 # L2<->fabric bytes per voxel step of trace_kernel on the 256^3 workload:
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a 1 GiB copy) over
 # 1.017e11 voxel steps, profiles/trace_kernel_plain-256cube.md.
-TRAFFIC_BYTES_PER_STEP = {"plain-256^3-stride2": 0.130}
+TRAFFIC_BYTES_PER_STEP = {"plain-256^3-stride2": 0.135}
 # The same per label read of ib_kernel (2-bit bricks), over 2.207e10 reads,
 # profiles/ib_kernel_bisection-256cube.md.
-IB_TRAFFIC_BYTES_PER_READ = {"plain-256^3-stride2": 0.00158}
+IB_TRAFFIC_BYTES_PER_READ = {"plain-256^3-stride2": 0.0024}
 METRIC = "absorption factors/sec (rays/s) at 1 GPU, plain vs surrogate at stated rel. error"
 
 
@@ -122,6 +122,37 @@ def calibrate_copy(nbytes: int = 1 << 30) -> None:
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def volume_bytes(shape, layout: int) -> int:
+    """Bytes of the label volume the kernel reads: one per voxel in the raw
+    layout, 128 (2-bit) or 256 (4-bit) per 8x8x8 brick otherwise."""
+    Z, Y, X = (int(v) for v in shape)
+    if layout == 0:
+        return Z * Y * X
+    bricks = -(-Z // 8) * -(-Y // 8) * -(-X // 8)
+    return bricks * (128 if layout == 2 else 256)
+
+
+def gather_ceiling(nbytes: int, achieved_gbs: float) -> dict:
+    """The device's one-byte gather rate with no arithmetic between loads
+    (anacor_probe_gather), on a working set of the next power of two above
+    the volume: every lane its own pseudo-random byte ("lanes"), and a wave's
+    64 lanes one 64-byte row ("row", the shape of the walk's gathers).  At one
+    byte per load, in GB/s, with the kernel's algorithmic rate as a fraction
+    of each."""
+    import ctypes as ct
+    from anacor_public_amd import _lib
+    lib = _lib.load()
+    k = max(6, (int(nbytes) - 1).bit_length())
+    out = {"working_set_bytes": 1 << k}
+    for name, pattern in (("lanes", _lib.GATHER_LANES), ("row", _lib.GATHER_ROW)):
+        rate = ct.c_double()
+        _lib.check(lib.anacor_probe_gather(0, k, pattern, 1 << 34, ct.byref(rate)), lib)
+        gbs = rate.value / 1e9
+        out[f"{name}_GBps"] = gbs
+        out[f"frac_of_{name}"] = achieved_gbs / gbs if gbs > 0 else None
+    return out
 
 
 def rel_stats(a, ref):
@@ -284,7 +315,8 @@ def run_bisection(steps: int, warmup: int, cpu_budget: float, walk_value: float)
             kms.append(st["kernel_ms"])
             reads = st["voxel_steps"]
         elapsed = time.perf_counter() - t0
-        layout = {0: "raw", 2: "brick2", 4: "brick4"}.get(t.layout, str(t.layout))
+        t_layout = t.layout
+        layout = {0: "raw", 2: "brick2", 4: "brick4"}.get(t_layout, str(t_layout))
         outside = st["outside_reads"]
     assert np.all(np.isfinite(A)) and np.all((A > 0) & (A <= 1))
     value = R * steps / elapsed
@@ -300,7 +332,9 @@ def run_bisection(steps: int, warmup: int, cpu_budget: float, walk_value: float)
                         "traffic": (IB_TRAFFIC_BYTES_PER_READ[wl.name] * reads
                                     if wl.name in IB_TRAFFIC_BYTES_PER_READ and layout == "brick2"
                                     else None),
-                        "kernel": "ib_kernel", "kernel_ms": k_ms, "bytes_per_launch": int(reads)}}
+                        "kernel": "ib_kernel", "kernel_ms": k_ms, "bytes_per_launch": int(reads),
+                        "gather_ceiling": gather_ceiling(
+                            volume_bytes(wl.volume.shape, t_layout), achieved)}}
     log(f"# bisection: {value:.3e} A_h/s ({rec['speedup_vs_plain_walk']:.2f}x the walk), "
         f"kernel {k_ms:.2f} ms, {reads} label reads, layout {layout}")
     if cpu_budget > 0:
@@ -479,6 +513,12 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
     }
+    if args.config != "batched":
+        rec["roofline"]["gather_ceiling"] = gather_ceiling(
+            volume_bytes(wls[0].volume.shape, runner.layout), achieved)
+        gc = rec["roofline"]["gather_ceiling"]
+        log(f"# gather ceiling at {gc['working_set_bytes']} B: lanes {gc['lanes_GBps']:.0f} GB/s, "
+            f"row {gc['row_GBps']:.0f} GB/s; trace kernel at {achieved:.0f} GB/s")
     rec.update(extra)
     runner.close()
     if args.config == "single" and not args.no_bisection and not args.reflections:

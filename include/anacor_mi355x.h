@@ -375,6 +375,19 @@ typedef struct {
 } anacor_mlp_stats_t;
 int anacor_mlp_stats(const anacor_mlp *mlp, anacor_mlp_stats_t *stats);
 
+/* ------------------------------------------------------------------------
+ * Measurement helper (no counterpart in the reference): the device's
+ * one-byte gather rate against a working set of 2^log2_bytes bytes
+ * (6..30), with no arithmetic between the loads.  It bounds the ray walk,
+ * which reads one uint8 label per voxel step, from above at the volume's own
+ * size.  About `loads` loads (capped at 2^36) per timed launch, five timed
+ * launches after one untimed one; allocates and frees its own buffers.
+ * ---------------------------------------------------------------------- */
+#define ANACOR_GATHER_LANES 0   /* every lane its own pseudo-random byte */
+#define ANACOR_GATHER_ROW   1   /* a wave's 64 lanes one 64-byte row */
+int anacor_probe_gather(int device, int log2_bytes, int pattern, int64_t loads,
+                        double *loads_per_s);
+
 #ifdef __cplusplus
 }
 #endif

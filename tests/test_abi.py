@@ -43,6 +43,15 @@ def test_device_count_without_gpu_is_nonnegative():
     assert _lib.load().anacor_device_count() >= 0
 
 
+def test_gather_probe_rejects_bad_arguments():
+    from anacor_public_amd import _lib
+    lib = _lib.load()
+    rate = ct.c_double(-1.0)
+    for k, pattern, loads in ((31, 0, 1 << 20), (5, 0, 1 << 20), (24, 2, 1 << 20), (24, 1, 0)):
+        assert lib.anacor_probe_gather(0, k, pattern, loads, ct.byref(rate)) == _lib.ANACOR_ERR_ARG
+    assert lib.anacor_probe_gather(0, 24, 0, 1 << 20, None) == _lib.ANACOR_ERR_ARG
+
+
 def _bits(a):
     return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
 
