@@ -314,10 +314,15 @@ __device__ __forceinline__ int label_at(const RayLane& L, const Volume& V, int i
 
 // Walk one ray (cal_coord + cal_path2_plus).  Boundaries beyond the implicit
 // first one (class 3 at increment 0) are kept in this lane's column of `seg`
-// (CAP entries, lane-interleaved, plus one scratch slot: every step stores
-// at the write slot, which advances only when a boundary opens); len is
-// len_path_2; out receives [li, lo, cr, bu].
+// (lane-interleaved, CAP + U slots: every step stores at the write slot,
+// which advances only when a boundary opens).  A group of U steps opens at
+// most U boundaries, so the bound is tested once per group: a group starts
+// with the write slot at CAP or below, or the list has overflowed and the
+// group writes into the spare slots above CAP again.  len is len_path_2; out
+// receives [li, lo, cr, bu].
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+constexpr int seg_slots(int cap, int u) { return cap + u; }
 
 template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool BIAS, bool CLAMP>
 __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
@@ -328,7 +333,7 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
   bool done = L.limit <= 0;
   bool overflow = false;
   lds_u32* wp = seg;                          // write slot (32-bit LDS address)
-  lds_u32* const trash = seg + CAP * kBlock;  // the scratch slot
+  lds_u32* const cap_slot = seg + CAP * kBlock;
   if (!done) {  // increment 0: its label opens no boundary
     const int lab = label_at<GUARD, LAYOUT>(L, V, 0);
     bool stop;
@@ -342,6 +347,10 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
     else { prev = lab; cnt = 1; }
   }
   for (int inc0 = 1; !done; inc0 += U) {
+    if (wp > cap_slot) {  // more than CAP boundaries so far: re-walk at the end
+      overflow = true;
+      wp = cap_slot;
+    }
     int lab[U];
 #pragma unroll
     for (int u = 0; u < U; ++u)  // offsets stay inside the volume (clamped)
@@ -362,16 +371,14 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
       // int8 labels: only values 0..4 open a boundary (appending, :343-394)
       const bool chg = take && lb != prev && (unsigned)lb <= 4u;
       *wp = ((uint32_t)inc << 3) | (uint32_t)lb;
-      const bool full = wp == trash;
-      overflow = overflow || (chg && full);
-      wp = (chg && !full) ? wp + kBlock : wp;
+      wp = chg ? wp + kBlock : wp;
       cnt += take ? 1 : 0;
       prev = lb;
       done = done || !take;
     }
   }
 
-  int nseg = (int)(wp - seg) / kBlock;   // exact unless the list overflowed
+  int nseg = (int)(wp - seg) / kBlock;   // exact unless the list overflowed (<= CAP + U)
   const int last = cnt - 1;
   const int len = cnt > 0 ? cnt : 1;
   len_out = len;
@@ -702,7 +709,7 @@ template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATH
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(LAYOUT >= kLayoutBrick2 ? 1 : 5)))
 void trace_kernel(TraceArgs a) {
-  __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
+  __shared__ uint32_t seg_lds[seg_slots(CAP, U) * kBlock];
   __shared__ ReduceLds red;
 
   const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
@@ -837,7 +844,7 @@ template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(LAYOUT >= kLayoutBrick2 ? 1 : 5)))
 void grid_map_kernel(GridArgs a) {
-  __shared__ uint32_t seg_lds[(CAP + 1) * kBlock];
+  __shared__ uint32_t seg_lds[seg_slots(CAP, U) * kBlock];
   const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
   if (lb >= a.n_blocks) return;
   const int d = lb / a.tiles;

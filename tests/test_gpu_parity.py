@@ -219,3 +219,39 @@ def test_single_gpu_config_parity_and_properties(gpu, oracle):
     with tracer(wl_p) as t:
         fp = t.run_workload(wl_p)
     np.testing.assert_allclose(fp, full1, rtol=REL_FP32, atol=0)
+
+
+@pytest.mark.parametrize("labels", [(1, 2), (1, 7, 2)])
+@pytest.mark.parametrize("full_iteration", [0, 1])
+def test_many_boundaries_vs_oracle(gpu, oracle, labels, full_iteration):
+    """Rays through one-voxel layers cross up to ~50 boundaries, more than a
+    lane's list holds (16 plus the spare slots of one unrolled group), so most
+    of them are traced again from their known length; others end between 16
+    and 22 boundaries, inside the spare slots.  Per-ray records stay bit-exact
+    and A_h matches the oracle, for labels within 0..4 and with a label 7 that
+    opens no boundary."""
+    from anacor_public_amd import workload as W
+    say(f"case: layered 96^3 volume, labels {labels}, full_iteration={full_iteration}")
+    n = 96
+    z, y, x = np.indices((n, n, n))
+    inner = (np.minimum(np.minimum(z, y), x) > 0) & (np.maximum(np.maximum(z, y), x) < n - 1)
+    vol = np.zeros((n, n, n), np.uint8)
+    layer = (x + 2 * y + 3 * z) % len(labels)
+    vol[inner] = np.array(labels, np.uint8)[layer][inner]
+    c = slice(n // 2 - 4, n // 2 + 4)
+    vol[c, c, c] = 3
+    s1, om = W.make_reflections(12, 5)
+    wl = W.Workload("layers-96^3", vol, W.crystal_voxels(vol)[::7], s1, om,
+                    full_iteration=full_iteration)
+    ref = oracle.trace(wl, rays=True)
+    nseg = ref["rays"]["nseg"]
+    assert nseg.max() > 30 and np.any((nseg > 16) & (nseg <= 22))
+    with tracer(wl) as t:
+        gp = t.paths(wl.s1, wl.omega, wl.raw_xray, wl.omega_axis, wl.F)
+        a64 = t.run(wl.s1, wl.omega, wl.raw_xray, wl.omega_axis, wl.F, f64_mean=True)
+        a32 = t.run_workload(wl)
+        st = t.stats()
+    compare_paths(gp, ref["rays"], f"layers {labels}")
+    assert st["voxel_steps"] == ref["steps"]
+    np.testing.assert_allclose(a64, ref["result"], rtol=REL_FP64, atol=0)
+    np.testing.assert_allclose(a32, ref["result"], rtol=REL_FP32, atol=0)
