@@ -324,7 +324,9 @@ typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 constexpr int seg_slots(int cap, int u) { return cap + u; }
 
-template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool BIAS, bool CLAMP>
+// L04: every label of the volume is in 0..4 (scanned on the host side), so
+// any change of label opens a boundary and the walk skips that test.
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool BIAS, bool CLAMP, bool L04>
 __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
                                               const double vs[3], lds_u32* seg, int& len_out,
                                               int& nseg_out, int last_zyx[3], double out[4]) {
@@ -368,8 +370,11 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
       }
       // once a lane stops it never takes another step: prev/va need no select
       const bool take = !done && inc < L.limit && !stop;
-      // int8 labels: only values 0..4 open a boundary (appending, :343-394)
-      const bool chg = take && lb != prev && (unsigned)lb <= 4u;
+      // int8 labels: only values 0..4 open a boundary (appending, :343-394);
+      // 2-bit bricks hold 0..3 only
+      constexpr bool kAll04 = L04 || LAYOUT == kLayoutBrick2;
+      bool chg = take && lb != prev;
+      if (!kAll04) chg = chg && (unsigned)lb <= 4u;
       *wp = ((uint32_t)inc << 3) | (uint32_t)lb;
       wp = chg ? wp + kBlock : wp;
       cnt += take ? 1 : 0;
@@ -458,7 +463,7 @@ __device__ __forceinline__ bool lane_in_range(const RayLane& L) {
 // the stepped coordinates.  The rounding is fixed per ray and almost always
 // the same across a wave; the clamp is dropped only when no lane of the
 // wave needs it (a wave vote), so that branch never diverges.
-template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT>
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool L04>
 __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, const double vs[3],
                                          lds_u32* seg, int& len_out, int& nseg_out,
                                          int last_zyx[3], double out[4]) {
@@ -473,17 +478,17 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
   }
   if (unclamped) {
     if (round)
-      walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, true, false>(L, V, vs, seg, len_out, nseg_out,
+      walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, true, false, L04>(L, V, vs, seg, len_out, nseg_out,
                                                               last_zyx, out);
     else
-      walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, false, false>(L, V, vs, seg, len_out,
+      walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, false, false, L04>(L, V, vs, seg, len_out,
                                                                nseg_out, last_zyx, out);
   } else {
     if (round)
-      walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, true, true>(L, V, vs, seg, len_out, nseg_out,
+      walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, true, true, L04>(L, V, vs, seg, len_out, nseg_out,
                                                              last_zyx, out);
     else
-      walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, false, true>(L, V, vs, seg, len_out, nseg_out,
+      walk_ray_impl<CAP, U, GUARD, FULL, LAYOUT, false, true, L04>(L, V, vs, seg, len_out, nseg_out,
                                                               last_zyx, out);
   }
 }
@@ -577,7 +582,7 @@ struct TileAcc {
 
 // Trace the voxels [v_begin, v_end) of reflection `rule` with this lane's
 // stride; accumulate exp(-sum mu*L) (or write path records).
-template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS>
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS, bool L04>
 __device__ __forceinline__ void trace_tile(const SampleView& S, const ReflRule* __restrict__ rule,
                                            int v_begin, int v_end, int is_exp, lds_u32* seg,
                                            TileAcc& acc, PathRecord* __restrict__ paths) {
@@ -605,7 +610,7 @@ __device__ __forceinline__ void trace_tile(const SampleView& S, const ReflRule* 
         verr = ANACOR_RAY_ERR_ASSERT;
         continue;
       }
-      walk_ray<CAP, U, GUARD, FULL, LAYOUT>(L, S.V, S.vs, seg, lens[k], nsegs[k], lasts[k],
+      walk_ray<CAP, U, GUARD, FULL, LAYOUT, L04>(L, S.V, S.vs, seg, lens[k], nsegs[k], lasts[k],
                                             Lr[k]);
       acc.steps += (unsigned long long)lens[k];
     }
@@ -705,7 +710,7 @@ __device__ __forceinline__ Volume make_volume(const uint8_t* vol, int Z, int Y, 
 // issue).  Bricked layouts: no bound (1), so the compiler keeps the registers
 // for eight gathers in flight per lane (the walk waits on the L2 and the
 // Infinity Cache there): 110 VGPRs, four waves.
-template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS>
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS, bool L04>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(LAYOUT >= kLayoutBrick2 ? 1 : 5)))
 void trace_kernel(TraceArgs a) {
@@ -729,7 +734,7 @@ void trace_kernel(TraceArgs a) {
   TileAcc acc = {0.f, 0.f, 0.0, 0ull, 0u};
   const int v_begin = tile * a.vox_per_tile;
   const int v_end = min(a.N, v_begin + a.vox_per_tile);
-  trace_tile<CAP, U, GUARD, FULL, LAYOUT, F64, PATHS>(
+  trace_tile<CAP, U, GUARD, FULL, LAYOUT, F64, PATHS, L04>(
       S, a.rules + refl, v_begin, v_end, a.is_exp, seg, acc,
       PATHS ? a.paths + (size_t)refl * a.N : nullptr);
   reduce_tile<F64, PATHS>(acc, red, &a.err[refl], a.steps, a.partial32 + lb, a.partial64 + lb);
@@ -867,7 +872,7 @@ void grid_map_kernel(GridArgs a) {
     } else {
       double Lr[4];
       int len, nseg, last[3];
-      walk_ray<CAP, U, GUARD, FULL, LAYOUT>(L, V, a.vs, seg, len, nseg, last, Lr);
+      walk_ray<CAP, U, GUARD, FULL, LAYOUT, false>(L, V, a.vs, seg, len, nseg, last, Lr);
       // cal_rate with zero incident lengths, same association (:1554-1583)
       const double r = a.mu[0] * (0.0 + Lr[0]) + a.mu[1] * (0.0 + Lr[1]) +
                        a.mu[2] * (0.0 + Lr[2]) + a.mu[3] * (0.0 + Lr[3]);
@@ -997,14 +1002,19 @@ __global__ __launch_bounds__(kBlock) void grid_lookup_kernel(GridLookup g) {
 
 // ------------------------------------------------------- volume packing
 
-// flags |= 1 when some label is outside 0..3 (2-bit bricks impossible).
+// flags |= 1 when some label is outside 0..3 (2-bit bricks impossible),
+// |= 2 when some label is outside 0..4 (the walk must test which labels open
+// a boundary).
 __global__ void label_scan_kernel(const uint8_t* __restrict__ vol, long long n,
                                   unsigned* __restrict__ flags) {
-  bool big = false;
+  bool over3 = false, over4 = false;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    big = big || vol[i] > 3;
-  if (__any(big) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flags, 1u);
+       i += (long long)gridDim.x * blockDim.x) {
+    over3 = over3 || vol[i] > 3;
+    over4 = over4 || vol[i] > 4;
+  }
+  const unsigned f = (__any(over3) ? 1u : 0u) | (__any(over4) ? 2u : 0u);
+  if (f && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flags, f);
 }
 
 // One thread per dword of the bricked volume (16 2-bit or 8 4-bit labels);
@@ -1066,7 +1076,8 @@ struct Context {
   int pack_bits = 0;                     // 2 or 4 once built
   long long pack_bytes = 0;
   int NBX = 0, NBY = 0, NBZ = 0;
-  int labels_over3 = -1;                 // some label outside 0..3 (-1: not scanned)
+  bool labels_over3 = true;              // some label outside 0..3 (scanned at creation)
+  bool labels_over4 = true;              // some label outside 0..4
   int4* d_coords = nullptr;
   int N = 0;
   double vs[3] = {1, 1, 1};
@@ -1180,24 +1191,28 @@ constexpr int kUnrollRaw = 6;  // label gathers in flight per lane, raw layouts
 // spare (1000x1000x800, 1,000 reflections: 164 ms with 4, 140 ms with 8).
 constexpr int unroll_for(int layout) { return layout >= kLayoutBrick2 ? 8 : kUnrollRaw; }
 
-template <bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS>
+template <bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS, bool L04>
 static void launch_trace(const TraceArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((trace_kernel<kCap, unroll_for(LAYOUT), GUARD, FULL, LAYOUT, F64, PATHS>),
-                     dim3(grid), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(
+      (trace_kernel<kCap, unroll_for(LAYOUT), GUARD, FULL, LAYOUT, F64, PATHS, L04>),
+      dim3(grid), dim3(kBlock), 0, s, a);
 }
 
 using LaunchFn = void (*)(const TraceArgs&, int, hipStream_t);
 
-// Index bits: guard 1, full 2, f64 4, paths 8, layout 16 * (0..3).  Path
-// records are written from the raw layouts only.
+// Index bits: guard 1, full 2, f64 4, paths 8, layout 16 * (0..3), labels in
+// 0..4 64.  Path records are written from the raw layouts only; the 0..4
+// variant exists for the raw layouts' A_h kernels without the guard.
 template <int I>
 static constexpr LaunchFn launcher() {
   constexpr int layout = (I >> 4) & 3;
-  constexpr bool paths = (I & 8) != 0;
+  constexpr bool guard = (I & 1) != 0, paths = (I & 8) != 0, l04 = (I & 64) != 0;
   if constexpr (paths && layout >= kLayoutBrick2) {
     return nullptr;
+  } else if constexpr (l04 && (guard || paths || layout >= kLayoutBrick2)) {
+    return nullptr;
   } else {
-    return &launch_trace<(I & 1) != 0, (I & 2) != 0, layout, (I & 4) != 0, paths>;
+    return &launch_trace<guard, (I & 2) != 0, layout, (I & 4) != 0, paths, l04>;
   }
 }
 
@@ -1206,12 +1221,15 @@ static constexpr std::array<LaunchFn, sizeof...(I)> launch_table(std::integer_se
   return {launcher<I>()...};
 }
 
-static const std::array<LaunchFn, 64> kLaunch = launch_table(std::make_integer_sequence<int, 64>{});
+static const std::array<LaunchFn, 128> kLaunch =
+    launch_table(std::make_integer_sequence<int, 128>{});
 
+// labels04: every label of the volume is in 0..4.
 static void dispatch_trace(const TraceArgs& a, int grid, bool guard, bool full, int layout,
-                           bool f64, bool paths, hipStream_t s) {
+                           bool f64, bool paths, bool labels04, hipStream_t s) {
+  const bool l04 = labels04 && !guard && !paths && layout < kLayoutBrick2;
   const int i = (guard ? 1 : 0) | (full ? 2 : 0) | (f64 ? 4 : 0) | (paths ? 8 : 0) |
-                (layout << 4);
+                (layout << 4) | (l04 ? 64 : 0);
   kLaunch[i](a, grid, s);
 }
 
@@ -1250,20 +1268,6 @@ static int ensure_layout(Context* c) {
     set_error(ANACOR_ERR_ARG, "volume %dx%dx%d too large for the bricked layout", c->Z, c->Y,
               c->X);
     return ANACOR_ERR_ARG;
-  }
-  const long long nvox = (long long)c->Z * c->Y * c->X;
-  if (c->labels_over3 < 0) {
-    unsigned* d_flag = nullptr;
-    unsigned h_flag = 0;
-    HIP_TRY(hipMalloc(&d_flag, sizeof(unsigned)));
-    HIP_TRY(hipMemsetAsync(d_flag, 0, sizeof(unsigned), c->stream));
-    hipLaunchKernelGGL(label_scan_kernel, dim3(2048), dim3(256), 0, c->stream, c->d_vol, nvox,
-                       d_flag);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(&h_flag, d_flag, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipFree(d_flag));
-    c->labels_over3 = (h_flag & 1u) ? 1 : 0;
   }
   int bits = (want == ANACOR_LAYOUT_BRICK4 || c->labels_over3) ? 4 : 2;
   if (want == ANACOR_LAYOUT_BRICK2 && c->labels_over3) {
@@ -1399,7 +1403,7 @@ static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flag
   const int grid = (int)((nblk + 7) / 8 * 8);
 
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  dispatch_trace(a, grid, guard, c->full != 0, layout, f64, paths, c->stream);
+  dispatch_trace(a, grid, guard, c->full != 0, layout, f64, paths, !c->labels_over4, c->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   if (!paths) {
@@ -1548,6 +1552,20 @@ static int ctx_create(const uint8_t* vol, const int64_t shape[3], const int64_t*
   c->N = (int)n;
   HIP_TRY(hipMalloc(&c->d_vol, nvox));
   HIP_TRY(hipMemcpy(c->d_vol, vol, nvox, hipMemcpyHostToDevice));
+  {  // which label ranges the volume holds (layout choice, walk variant)
+    unsigned* d_flag = nullptr;
+    unsigned h_flag = 0;
+    HIP_TRY(hipMalloc(&d_flag, sizeof(unsigned)));
+    HIP_TRY(hipMemsetAsync(d_flag, 0, sizeof(unsigned), c->stream));
+    hipLaunchKernelGGL(label_scan_kernel, dim3(2048), dim3(256), 0, c->stream, c->d_vol,
+                       (long long)nvox, d_flag);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(&h_flag, d_flag, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(d_flag));
+    c->labels_over3 = (h_flag & 1u) != 0;
+    c->labels_over4 = (h_flag & 2u) != 0;
+  }
   c->h_coords.resize((size_t)n);
   for (int64_t i = 0; i < n; ++i) {
     const int64_t z = coords ? coords[3 * i] : coords32[3 * i];
@@ -1843,7 +1861,8 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
     t.paths = nullptr;
     t.order = d.order;
     dispatch_trace(t, (t.n_blocks + 7) / 8 * 8, guard, b->full != 0,
-                   raw_needs_wide(c) ? kLayoutWide : kLayoutRaw, f64, false, b->stream);
+                   raw_needs_wide(c) ? kLayoutWide : kLayoutRaw, f64, false, !c->labels_over4,
+                   b->stream);
     HIP_TRY(hipGetLastError());
   }
   if (!timing) HIP_TRY(hipEventRecord(b->ev0, b->stream));
