@@ -240,12 +240,14 @@ struct RayLane {
   int ca, cb, cp, sp;    // bricked layouts: brick constants of a, b, p and stride of p
 };
 
-template <bool GUARD, bool WIDE, bool BIAS = true, bool CLAMP = true>
+// SHELL: the face coordinate is not held on its last plane; past it the
+// offset lands in the zero shell of the raw layouts.
+template <bool GUARD, bool WIDE, bool BIAS = true, bool CLAMP = true, bool SHELL = false>
 __device__ __forceinline__ int lane_offset(const RayLane& L, int inc) {
   const double incd = (double)inc;
   const int ia = axis_index<GUARD, BIAS, CLAMP>(L.ba, incd, L.da, L.bias, L.ha);
   const int ib = axis_index<GUARD, BIAS, CLAMP>(L.bb, incd, L.db, L.bias, L.hb);
-  const int ip = min(inc, L.incmax);
+  const int ip = SHELL ? inc : min(inc, L.incmax);
   if (WIDE) return ia * L.sa + ib * L.sb + ip * L.pstep + L.pbase;
   // every factor fits 24 bits (checked on the host): full-rate v_mad_i32_i24
   return mad24(ia, L.sa, mad24(ib, L.sb, mad24(ip, L.pstep, L.pbase)));
@@ -276,9 +278,14 @@ __device__ __forceinline__ void lane_zyx(const RayLane& L, int inc, int& z, int&
 // (8, 512 NBX - 64), (1, 504) for z, y, x.
 enum { kLayoutRaw = 0, kLayoutWide = 1, kLayoutBrick2 = 2, kLayoutBrick4 = 3 };
 
+// The raw layouts hold the volume inside a one-voxel shell of label 0:
+// voxel (z, y, x) is at (z + 1) * sz + (y + 1) * sy + (x + 1), sy = X + 2,
+// sz = (Y + 2) * sy.  A ray whose face coordinate steps one plane past the
+// volume reads the shell's 0 and stops, so the walk needs no loop bound.
 struct Volume {
   __amdgpu_buffer_rsrc_t rsrc;  // range-checked: offsets past the end read 0
   int Z, Y, X;
+  int sz, sy, base;             // raw layouts: strides of z and y, offset of voxel (0, 0, 0)
   int cz, cy;                   // brick constants c of z and y (bricked layouts)
 };
 
@@ -297,11 +304,12 @@ __device__ __forceinline__ int load_packed(const Volume& V, int f) {
   return (int)__builtin_amdgcn_ubfe(dw, (unsigned)(f * BITS), BITS);
 }
 
-// Label at increment inc of the lane's ray in the given layout.
-template <bool GUARD, int LAYOUT, bool BIAS = true, bool CLAMP = true>
+// Label at increment inc of the lane's ray in the given layout (SHELL: raw
+// layouts only, see lane_offset).
+template <bool GUARD, int LAYOUT, bool BIAS = true, bool CLAMP = true, bool SHELL = false>
 __device__ __forceinline__ int label_at(const RayLane& L, const Volume& V, int inc) {
   if (LAYOUT == kLayoutRaw || LAYOUT == kLayoutWide)
-    return load_label(V, lane_offset<GUARD, LAYOUT == kLayoutWide, BIAS, CLAMP>(L, inc));
+    return load_label(V, lane_offset<GUARD, LAYOUT == kLayoutWide, BIAS, CLAMP, SHELL>(L, inc));
   const double incd = (double)inc;
   const int ia = axis_index<GUARD, BIAS, CLAMP>(L.ba, incd, L.da, L.bias, L.ha);
   const int ib = axis_index<GUARD, BIAS, CLAMP>(L.bb, incd, L.db, L.bias, L.hb);
@@ -330,6 +338,15 @@ template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool BIAS, bool CLA
 __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
                                               const double vs[3], lds_u32* seg, int& len_out,
                                               int& nseg_out, int last_zyx[3], double out[4]) {
+  // raw layouts without full_iteration: the face coordinate runs into the
+  // zero shell at increment incmax + 1, which ends the ray where the loop
+  // bound of an exclusive bound does (limit = incmax + 1); an inclusive bound
+  // (limit = incmax + 2) takes that increment once more after the loop, with
+  // the face coordinate held on its last plane
+  constexpr bool SHELL = !FULL && LAYOUT < kLayoutBrick2;
+  // int8 labels: only values 0..4 open a boundary (appending, :343-394);
+  // 2-bit bricks hold 0..3 only
+  constexpr bool kAll04 = L04 || LAYOUT == kLayoutBrick2;
   // taken steps (increments 0 .. cnt-1 passed), last label, vacuum run
   int cnt = 0, prev = 0, va = 0;
   bool done = L.limit <= 0;
@@ -355,8 +372,8 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
     }
     int lab[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u)  // offsets stay inside the volume (clamped)
-      lab[u] = label_at<GUARD, LAYOUT, BIAS, CLAMP>(L, V, inc0 + u);
+    for (int u = 0; u < U; ++u)  // range-checked loads; steps past the end are not taken
+      lab[u] = label_at<GUARD, LAYOUT, BIAS, CLAMP, SHELL>(L, V, inc0 + u);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int inc = inc0 + u;
@@ -369,10 +386,7 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
         stop = lb == 0;
       }
       // once a lane stops it never takes another step: prev/va need no select
-      const bool take = !done && inc < L.limit && !stop;
-      // int8 labels: only values 0..4 open a boundary (appending, :343-394);
-      // 2-bit bricks hold 0..3 only
-      constexpr bool kAll04 = L04 || LAYOUT == kLayoutBrick2;
+      const bool take = SHELL ? !done && !stop : !done && inc < L.limit && !stop;
       bool chg = take && lb != prev;
       if (!kAll04) chg = chg && (unsigned)lb <= 4u;
       *wp = ((uint32_t)inc << 3) | (uint32_t)lb;
@@ -380,6 +394,24 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
       cnt += take ? 1 : 0;
       prev = lb;
       done = done || !take;
+    }
+  }
+  if (SHELL && cnt == L.incmax + 1 && L.limit == L.incmax + 2) {
+    if (wp > cap_slot) {
+      overflow = true;
+      wp = cap_slot;
+    }
+    const int inc = L.incmax + 1;
+    const int lb = label_at<GUARD, LAYOUT, BIAS, CLAMP>(L, V, inc);
+    if (lb != 0) {
+      // prev holds the label of the group's last step, taken or not: read
+      // the last taken step's label again
+      const int before = label_at<GUARD, LAYOUT, BIAS, CLAMP>(L, V, L.incmax);
+      bool chg = lb != before;
+      if (!kAll04) chg = chg && (unsigned)lb <= 4u;
+      *wp = ((uint32_t)inc << 3) | (uint32_t)lb;
+      wp = chg ? wp + kBlock : wp;
+      cnt += 1;
     }
   }
 
@@ -440,13 +472,13 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
 }
 
 // True when neither stepped coordinate of the lane's ray needs the clamp to
-// [0, extent-1] on increments 0 .. limit-1: each is monotone in the
-// increment, so its unclamped values at the two ends bound all the others.
-// Increments past the limit (the unrolled tail) are never taken, and their
-// loads stay range-checked.
+// [0, extent-1] on increments 0 .. last: each is monotone in the increment,
+// so its unclamped values at the two ends bound all the others.  Increments
+// past the last one (the unrolled tail) are never taken, and their loads stay
+// range-checked.
 template <bool BIAS>
-__device__ __forceinline__ bool lane_in_range(const RayLane& L) {
-  const double i0 = 0.0, i1 = (double)(L.limit - 1);
+__device__ __forceinline__ bool lane_in_range(const RayLane& L, int last) {
+  const double i0 = 0.0, i1 = (double)last;
   const int a0 = axis_index<false, BIAS, false>(L.ba, i0, L.da, L.bias, L.ha);
   const int a1 = axis_index<false, BIAS, false>(L.ba, i1, L.da, L.bias, L.ha);
   const int b0 = axis_index<false, BIAS, false>(L.bb, i0, L.db, L.bias, L.hb);
@@ -472,8 +504,11 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
   // raw layouts only: the bricked walk waits on memory, not on VALU issue,
   // and measured no faster without the clamp (1000x1000x800 volume)
   if (!GUARD && LAYOUT < kLayoutBrick2) {
-    const bool in_range = L.limit <= 0 || (round ? lane_in_range<true>(L)
-                                                 : lane_in_range<false>(L));
+    // with the zero shell the walk reads increment incmax + 1 of every ray
+    // (limit - 1 for an inclusive bound, limit for an exclusive one)
+    const int last = FULL ? L.limit - 1 : L.incmax + 1;
+    const bool in_range = L.limit <= 0 || (round ? lane_in_range<true>(L, last)
+                                                 : lane_in_range<false>(L, last));
     unclamped = __all(in_range) != 0;
   }
   if (unclamped) {
@@ -504,7 +539,7 @@ __device__ __forceinline__ bool make_lane(const RayRule* __restrict__ rr, int fa
   const uint32_t meta = rr->meta[face - 1];
   const double* d = rr->d[face - 1];
   const int axis = (int)((meta >> kMetaAxisShift) & 3u);
-  const int sz = BRICK ? 64 : V.Y * V.X, sy = BRICK ? 8 : V.X;
+  const int sz = BRICK ? 64 : V.sz, sy = BRICK ? 8 : V.sy;
   const int kz = V.cz, ky = V.cy, kx = 504;
   L.bias = (meta & kMetaRound) ? kRoundBias : 0.0;
   L.vac = (int)((meta >> kMetaVacShift) & 0xffu);
@@ -525,7 +560,7 @@ __device__ __forceinline__ bool make_lane(const RayRule* __restrict__ rr, int fa
   }
   L.sp = sp;
   L.p0 = c;
-  L.pbase = c * sp;
+  L.pbase = c * sp + (BRICK ? 0 : V.base);
   if (meta & kMetaToMax) {
     L.pdir = 1;
     L.limit = (ext - c) + ((meta & kMetaInclusive) ? 1 : 0);
@@ -702,6 +737,9 @@ __device__ __forceinline__ Volume make_volume(const uint8_t* vol, int Z, int Y, 
   Volume V;
   V.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)vol, (short)0, nbytes, 0x00020000);
   V.Z = Z; V.Y = Y; V.X = X;
+  V.sy = X + 2;
+  V.sz = (Y + 2) * V.sy;
+  V.base = V.sz + V.sy + 1;
   V.cz = cz; V.cy = cy;
   return V;
 }
@@ -1017,6 +1055,24 @@ __global__ void label_scan_kernel(const uint8_t* __restrict__ vol, long long n,
   if (f && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flags, f);
 }
 
+// The caller's [Z][Y][X] volume into the raw layout [Z+2][Y+2][X+2] with its
+// one-voxel shell of label 0 (n = padded voxel count).
+__global__ void shell_volume_kernel(const uint8_t* __restrict__ in, int Z, int Y, int X,
+                                    long long n, uint8_t* __restrict__ out) {
+  const int sx = X + 2, sy = Y + 2;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(i % sx) - 1;
+    const long long t = i / sx;
+    const int y = (int)(t % sy) - 1;
+    const int z = (int)(t / sy) - 1;
+    uint8_t v = 0;
+    if ((unsigned)z < (unsigned)Z && (unsigned)y < (unsigned)Y && (unsigned)x < (unsigned)X)
+      v = in[((size_t)z * Y + y) * X + x];
+    out[i] = v;
+  }
+}
+
 // One thread per dword of the bricked volume (16 2-bit or 8 4-bit labels);
 // voxels of the padding beyond the volume are 0.
 __global__ void pack_volume_kernel(const uint8_t* __restrict__ vol, int Z, int Y, int X, int NBX,
@@ -1033,8 +1089,8 @@ __global__ void pack_volume_kernel(const uint8_t* __restrict__ vol, int Z, int Y
     const long long t = b / NBX;
     const int by = (int)(t % NBY), bz = (int)(t / NBY);
     const int z = bz * 8 + (w >> 6), y = by * 8 + ((w >> 3) & 7), x = bx * 8 + (w & 7);
-    uint32_t lab = 0;
-    if (z < Z && y < Y && x < X) lab = vol[((size_t)z * Y + y) * X + x];
+    uint32_t lab = 0;   // vol: the raw layout with its zero shell
+    if (z < Z && y < Y && x < X) lab = vol[((size_t)(z + 1) * (Y + 2) + (y + 1)) * (X + 2) + x + 1];
     const uint32_t code = bits == 2 ? lab : (lab <= 4u ? lab : 5u);
     word |= code << (j * bits);
   }
@@ -1068,7 +1124,8 @@ struct Pinned {
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
-  uint8_t* d_vol = nullptr;
+  uint8_t* d_vol = nullptr;              // raw layout with its zero shell
+  long long vol_bytes = 0;               // (Z + 2)(Y + 2)(X + 2)
   int Z = 0, Y = 0, X = 0;
   int layout_req = ANACOR_LAYOUT_AUTO;   // requested layout of the trace kernel
   int layout = kLayoutRaw;               // layout in use (kLayout*)
@@ -1235,7 +1292,7 @@ static void dispatch_trace(const TraceArgs& a, int grid, bool guard, bool full, 
 
 // 24-bit address arithmetic unless a z-plane or the volume is too large for it.
 static bool raw_needs_wide(const Context* c) {
-  return (int64_t)c->Y * c->X >= (1 << 23) || c->Z >= (1 << 23);
+  return (int64_t)(c->Y + 2) * (c->X + 2) >= (1 << 23) || c->Z + 2 >= (1 << 23);
 }
 
 // The bricked layouts need every factor of the address chain in 24 bits and
@@ -1380,7 +1437,7 @@ static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flag
   TraceArgs a;
   const bool bricked = layout == kLayoutBrick2 || layout == kLayoutBrick4;
   a.vol = bricked ? c->d_pack : c->d_vol;
-  a.vol_bytes = bricked ? (int)c->pack_bytes : c->Z * c->Y * c->X;
+  a.vol_bytes = bricked ? (int)c->pack_bytes : (int)c->vol_bytes;
   a.cz = 512 * c->NBX * c->NBY - 512;
   a.cy = 512 * c->NBX - 64;
   a.Z = c->Z; a.Y = c->Y; a.X = c->X;
@@ -1513,8 +1570,10 @@ static int ctx_create(const uint8_t* vol, const int64_t shape[3], const int64_t*
       return ANACOR_ERR_ARG;
     }
   const uint64_t nvox = (uint64_t)shape[0] * shape[1] * shape[2];
-  if (nvox >= (1ull << 31)) {
-    set_error(ANACOR_ERR_ARG, "volume of %llu voxels exceeds 2^31", (unsigned long long)nvox);
+  const uint64_t npad = (uint64_t)(shape[0] + 2) * (shape[1] + 2) * (shape[2] + 2);
+  if (npad >= (1ull << 31)) {
+    set_error(ANACOR_ERR_ARG, "volume of %llu voxels exceeds 2^31 with its one-voxel shell",
+              (unsigned long long)nvox);
     return ANACOR_ERR_ARG;
   }
   if (n < 0 || n >= (1ll << 31) || (n > 0 && !coords && !coords32)) {
@@ -1550,19 +1609,27 @@ static int ctx_create(const uint8_t* vol, const int64_t shape[3], const int64_t*
   for (int i = 0; i < 4; ++i) c->mu[i] = mu ? mu[i] : 0.0;
   c->full = full;
   c->N = (int)n;
-  HIP_TRY(hipMalloc(&c->d_vol, nvox));
-  HIP_TRY(hipMemcpy(c->d_vol, vol, nvox, hipMemcpyHostToDevice));
-  {  // which label ranges the volume holds (layout choice, walk variant)
+  {  // upload, scan which label ranges the volume holds (layout choice, walk
+     // variant), and lay it out with its zero shell
+    uint8_t* d_in = nullptr;
     unsigned* d_flag = nullptr;
     unsigned h_flag = 0;
+    HIP_TRY(hipMalloc(&d_in, nvox));
+    HIP_TRY(hipMemcpy(d_in, vol, nvox, hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&d_flag, sizeof(unsigned)));
     HIP_TRY(hipMemsetAsync(d_flag, 0, sizeof(unsigned), c->stream));
-    hipLaunchKernelGGL(label_scan_kernel, dim3(2048), dim3(256), 0, c->stream, c->d_vol,
+    hipLaunchKernelGGL(label_scan_kernel, dim3(2048), dim3(256), 0, c->stream, d_in,
                        (long long)nvox, d_flag);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMalloc(&c->d_vol, npad));
+    c->vol_bytes = (long long)npad;
+    hipLaunchKernelGGL(shell_volume_kernel, dim3(4096), dim3(256), 0, c->stream, d_in, c->Z,
+                       c->Y, c->X, (long long)npad, c->d_vol);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(&h_flag, d_flag, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipFree(d_flag));
+    HIP_TRY(hipFree(d_in));
     c->labels_over3 = (h_flag & 1u) != 0;
     c->labels_over4 = (h_flag & 2u) != 0;
   }
@@ -1841,7 +1908,7 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
     const BatchSample& d = b->h_desc[(size_t)i];
     TraceArgs t;
     t.vol = c->d_vol;
-    t.vol_bytes = c->Z * c->Y * c->X;
+    t.vol_bytes = (int)c->vol_bytes;
     t.cz = t.cy = 0;
     t.Z = c->Z; t.Y = c->Y; t.X = c->X;
     t.coords = c->d_coords;
@@ -2008,7 +2075,7 @@ static int grid_build(Context* c, int64_t n_theta, int64_t n_phi, const double* 
     GridArgs a;
     const bool bricked = c->layout == kLayoutBrick2 || c->layout == kLayoutBrick4;
     a.vol = bricked ? c->d_pack : c->d_vol;
-    a.vol_bytes = bricked ? (int)c->pack_bytes : c->Z * c->Y * c->X;
+    a.vol_bytes = bricked ? (int)c->pack_bytes : (int)c->vol_bytes;
     a.cz = 512 * c->NBX * c->NBY - 512;
     a.cy = 512 * c->NBX - 64;
     a.Z = c->Z; a.Y = c->Y; a.X = c->X;
@@ -2213,7 +2280,7 @@ struct IbArgs {
 // The addresses use full-rate 24-bit multiply-adds (v_mad_i32_i24) wherever
 // the host has checked that every factor fits: the raw layout below 2^23
 // voxels per z plane, the bricked layouts always (bricks_fit); the wide raw
-// layout uses 32-bit multiplies.  yx = Y * X.
+// layout uses 32-bit multiplies.  yx: the z stride of the raw layouts (V.sz).
 template <int LAYOUT>
 __device__ __forceinline__ int ib_label(const Volume& V, int yx, double z, double y, double x,
                                         unsigned& oob) {
@@ -2223,8 +2290,8 @@ __device__ __forceinline__ int ib_label(const Volume& V, int yx, double z, doubl
     ++oob;
     return 0;
   }
-  if (LAYOUT == kLayoutWide) return load_label(V, iz * yx + iy * V.X + ix);
-  if (LAYOUT == kLayoutRaw) return load_label(V, mad24(iz, yx, mad24(iy, V.X, ix)));
+  if (LAYOUT == kLayoutWide) return load_label(V, iz * yx + iy * V.sy + ix + V.base);
+  if (LAYOUT == kLayoutRaw) return load_label(V, mad24(iz, yx, mad24(iy, V.sy, ix + V.base)));
   const int f = mad24(iz, 64, mad24(iz >> 3, V.cz, mad24(iy, 8, mad24(iy >> 3, V.cy,
                 mad24(ix >> 3, 504, ix)))));
   return load_packed<LAYOUT>(V, f);
@@ -2371,7 +2438,7 @@ void ib_kernel(IbArgs a) {
         default: air = (double)(a.Z - 1 - co.x); break;
       }
       const double r[3] = {ray.r[face - 1][0], ray.r[face - 1][1], ray.r[face - 1][2]};
-      ib_ray<LAYOUT>(V, a.Y * a.X, c, r, air, a.num_cls, a.vs, L, reads, oob);
+      ib_ray<LAYOUT>(V, V.sz, c, r, air, a.num_cls, a.vs, L, reads, oob);
     }
     acc.err |= verr;
     acc.steps += reads;
@@ -2512,7 +2579,7 @@ static int ib_run(Context* c, int64_t R, int num_cls, int flags, double* out, do
   const int layout = c->layout;
   const bool bricked = layout == kLayoutBrick2 || layout == kLayoutBrick4;
   a.vol = bricked ? c->d_pack : c->d_vol;
-  a.vol_bytes = bricked ? (int)c->pack_bytes : c->Z * c->Y * c->X;
+  a.vol_bytes = bricked ? (int)c->pack_bytes : (int)c->vol_bytes;
   a.cz = 512 * c->NBX * c->NBY - 512;
   a.cy = 512 * c->NBX - 64;
   a.Z = c->Z; a.Y = c->Y; a.X = c->X;
