@@ -201,6 +201,14 @@ __device__ __forceinline__ int mad24(int a, int b, int c) {
   return r;
 }
 
+// The same for 0 <= a < 2^23, written so that the backend selects the
+// v_mad_i32_i24 itself and can take a from a scalar register (the walk's
+// increment is the same in every lane) without a copy to a vector register.
+__device__ __forceinline__ int mad24_small(int a, int b, int c) {
+  __builtin_assume(a >= 0 && a < (1 << 23));
+  return ((a << 8) >> 8) * ((b << 8) >> 8) + c;
+}
+
 // median(v, 0, hi) == clamp to [0, hi] for hi >= 0, one v_med3_i32
 __device__ __forceinline__ int clamp_int(int v, int hi) {
   int r;
@@ -250,7 +258,7 @@ __device__ __forceinline__ int lane_offset(const RayLane& L, int inc) {
   const int ip = SHELL ? inc : min(inc, L.incmax);
   if (WIDE) return ia * L.sa + ib * L.sb + ip * L.pstep + L.pbase;
   // every factor fits 24 bits (checked on the host): full-rate v_mad_i32_i24
-  return mad24(ia, L.sa, mad24(ib, L.sb, mad24(ip, L.pstep, L.pbase)));
+  return mad24(ia, L.sa, mad24(ib, L.sb, mad24_small(ip, L.pstep, L.pbase)));
 }
 
 template <bool GUARD>
