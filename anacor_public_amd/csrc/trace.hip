@@ -552,19 +552,23 @@ __device__ __forceinline__ bool make_lane(const RayRule* __restrict__ rr, int fa
   L.bias = (meta & kMetaRound) ? kRoundBias : 0.0;
   L.vac = (int)((meta >> kMetaVacShift) & 0xffu);
   L.paxis = axis;
+  // extents as values first: selecting among the fields of V inside the
+  // branches let the compiler merge them into one indexed load, which put V
+  // in scratch memory
+  const int VZ = V.Z, VY = V.Y, VX = V.X;
   int c, ext, sp;
   if (axis == 0) {          // z: a = y, b = x
-    L.ba = (double)y; L.da = d[1]; L.sa = sy; L.ha = V.Y - 1; L.ca = ky;
-    L.bb = (double)x; L.db = d[2]; L.sb = 1;  L.hb = V.X - 1; L.cb = kx;
-    sp = sz; c = z; ext = V.Z; L.cp = kz;
+    L.ba = (double)y; L.da = d[1]; L.sa = sy; L.ha = VY - 1; L.ca = ky;
+    L.bb = (double)x; L.db = d[2]; L.sb = 1;  L.hb = VX - 1; L.cb = kx;
+    sp = sz; c = z; ext = VZ; L.cp = kz;
   } else if (axis == 1) {   // y: a = z, b = x
-    L.ba = (double)z; L.da = d[0]; L.sa = sz; L.ha = V.Z - 1; L.ca = kz;
-    L.bb = (double)x; L.db = d[2]; L.sb = 1;  L.hb = V.X - 1; L.cb = kx;
-    sp = sy; c = y; ext = V.Y; L.cp = ky;
+    L.ba = (double)z; L.da = d[0]; L.sa = sz; L.ha = VZ - 1; L.ca = kz;
+    L.bb = (double)x; L.db = d[2]; L.sb = 1;  L.hb = VX - 1; L.cb = kx;
+    sp = sy; c = y; ext = VY; L.cp = ky;
   } else {                  // x: a = z, b = y
-    L.ba = (double)z; L.da = d[0]; L.sa = sz; L.ha = V.Z - 1; L.ca = kz;
-    L.bb = (double)y; L.db = d[1]; L.sb = sy; L.hb = V.Y - 1; L.cb = ky;
-    sp = 1; c = x; ext = V.X; L.cp = kx;
+    L.ba = (double)z; L.da = d[0]; L.sa = sz; L.ha = VZ - 1; L.ca = kz;
+    L.bb = (double)y; L.db = d[1]; L.sb = sy; L.hb = VY - 1; L.cb = ky;
+    sp = 1; c = x; ext = VX; L.cp = kx;
   }
   L.sp = sp;
   L.p0 = c;
