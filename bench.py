@@ -44,10 +44,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 MLP_FLOPS_PER_ROW = 2 * (12 * 256 + 3 * 256 * 256 + 256)   # 2 x 199,936 MACs
-# L2<->fabric bytes per voxel step of trace_kernel on the 256^3 workload:
-# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a 1 GiB copy) over
-# 1.017e11 voxel steps, profiles/trace_kernel_plain-256cube.md.
-TRAFFIC_BYTES_PER_STEP = {"plain-256^3-stride2": 0.135}
+# L2<->fabric bytes per voxel step of trace_kernel on the 256^3 workload,
+# by voxel tile (0: automatic, 1,024 voxels here): FETCH_SIZE x 2.000 +
+# WRITE_SIZE x 1.000 (factors from a 1 GiB copy) over 1.017e11 voxel steps;
+# 512 with (1, 2, 64) voxel blocks from profiles/trace_kernel_plain-256cube.md,
+# automatic with (2, 2, 32) blocks from a FETCH_SIZE pass of the autotuned
+# default run.  Larger tiles spread a reflection's workgroups over more XCDs.
+TRAFFIC_BYTES_PER_STEP = {("plain-256^3-stride2", 512): 0.093,
+                          ("plain-256^3-stride2", 0): 0.200}
 # The same per label read of ib_kernel (2-bit bricks), over 2.207e10 reads,
 # profiles/ib_kernel_bisection-256cube.md.
 IB_TRAFFIC_BYTES_PER_READ = {"plain-256^3-stride2": 0.0024}
@@ -126,10 +130,11 @@ def log(msg):
 
 def volume_bytes(shape, layout: int) -> int:
     """Bytes of the label volume the kernel reads: one per voxel in the raw
-    layout, 128 (2-bit) or 256 (4-bit) per 8x8x8 brick otherwise."""
+    layout (with its one-voxel shell of zeros), 128 (2-bit) or 256 (4-bit) per
+    8x8x8 brick otherwise."""
     Z, Y, X = (int(v) for v in shape)
     if layout == 0:
-        return Z * Y * X
+        return (Z + 2) * (Y + 2) * (X + 2)
     bricks = -(-Z // 8) * -(-Y // 8) * -(-X // 8)
     return bricks * (128 if layout == 2 else 256)
 
@@ -481,9 +486,12 @@ def main():
 
     traffic = None
     # the calibrated traffic per step was measured with the raw layout
-    if (name in TRAFFIC_BYTES_PER_STEP and not args.reflections and args.config != "batched"
-            and runner.layout == 0):
-        traffic = TRAFFIC_BYTES_PER_STEP[name] * vsteps
+    tile = args.tile
+    if not tile and isinstance(extra.get("autotune"), dict):
+        tile = extra["autotune"]["tile"]
+    if ((name, tile) in TRAFFIC_BYTES_PER_STEP and not args.reflections
+            and args.config != "batched" and runner.layout == 0):
+        traffic = TRAFFIC_BYTES_PER_STEP[(name, tile)] * vsteps
     pairs = sum(len(w.coords) * len(w.omega) for w in wls)
     rec = {
         "metric": METRIC,
