@@ -141,16 +141,37 @@ __device__ __forceinline__ int xcd_logical_block(int hw, int n_hw) {
   return (hw & 7) * per + (hw >> 3);
 }
 
+// The two planes of one axis, t = (0 - c) / e and (n - c) / e with the
+// direction e already negated for the incident ray (-(k / d) and k / (-d) are
+// the same double).  With 0 <= c < n only one quotient can be >= 0 unless
+// c == 0: for e with its sign bit clear (> 0 or +0) the upper plane's
+// (positive, or +inf), for e with the sign bit set the lower plane's
+// (positive, +inf, -0 when c == 0, NaN for 0 / -0); the other is negative,
+// -inf or NaN — except (0 - 0) / e = +0 for e > 0 — so it is not computed
+// (a direction component that is not finite gets both quotients).
+// Non-candidates are returned as -1, which neither passes the >= 0 test nor
+// equals a minimum.  One division per axis instead of two.
+__device__ __forceinline__ void plane_pair(int c, int n, double e, double& tlo, double& thi) {
+  const double no = -1.0;
+  if (!__builtin_isfinite(e)) {   // k / +-inf is a signed zero, k / NaN NaN: both as they are
+    tlo = (double)(0 - c) / e;
+    thi = (double)(n - c) / e;
+  } else if (__builtin_signbit(e)) {
+    tlo = (double)(0 - c) / e;
+    thi = no;
+  } else {
+    thi = (double)(n - c) / e;
+    tlo = (c == 0 && e > 0.0) ? 0.0 : no;   // 0 / +0 and 0 / NaN are NaN
+  }
+}
+
 // cube_face (ray_tracing_basic.c:69-156).  0 when no exit plane qualifies.
 __device__ __forceinline__ int exit_face(int z, int y, int x, double dz, double dy, double dx,
                                          int Z, int Y, int X, bool neg) {
-  double t0 = (double)(0 - x) / dx;   // x_min
-  double t1 = (double)(0 - y) / dy;   // y_min
-  double t2 = (double)(0 - z) / dz;   // z_min
-  double t3 = (double)(X - x) / dx;   // x_max
-  double t4 = (double)(Y - y) / dy;   // y_max
-  double t5 = (double)(Z - z) / dz;   // z_max
-  if (neg) { t0 = -t0; t1 = -t1; t2 = -t2; t3 = -t3; t4 = -t4; t5 = -t5; }
+  double t0, t1, t2, t3, t4, t5;      // x_min, y_min, z_min, x_max, y_max, z_max
+  plane_pair(x, X, neg ? -dx : dx, t0, t3);
+  plane_pair(y, Y, neg ? -dy : dy, t1, t4);
+  plane_pair(z, Z, neg ? -dz : dz, t2, t5);
   const double big = __builtin_huge_val();
   // minimum over the non-negative candidates (NaN never qualifies)
   double m = big;
