@@ -293,6 +293,15 @@ __device__ __forceinline__ void lane_zyx(const RayLane& L, int inc, int& z, int&
   else { z = ia; y = ib; x = ip; }
 }
 
+// Coordinates at increment 0: the origin itself (c + 0 * d rounds to the
+// integer c, and the origin lies inside the volume).
+__device__ __forceinline__ void lane_origin(const RayLane& L, int& z, int& y, int& x) {
+  const int ia = (int)L.ba, ib = (int)L.bb, ip = L.p0;
+  if (L.paxis == 0) { z = ip; y = ia; x = ib; }
+  else if (L.paxis == 1) { z = ia; y = ip; x = ib; }
+  else { z = ia; y = ib; x = ip; }
+}
+
 // Volume layouts (the LAYOUT template parameter of the walk):
 //   kLayoutRaw    uint8 [z][y][x], 24-bit address arithmetic
 //   kLayoutWide   uint8 [z][y][x], 32-bit address arithmetic
@@ -453,7 +462,7 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
   last_zyx[0] = last_zyx[1] = last_zyx[2] = 0;
   if (last < 0) return;
   int fz, fy, fx, lz, ly, lx;
-  lane_zyx<GUARD>(L, 0, fz, fy, fx);
+  lane_origin(L, fz, fy, fx);
   lane_zyx<GUARD>(L, last, lz, ly, lx);
   last_zyx[0] = lz; last_zyx[1] = ly; last_zyx[2] = lx;
   if (len == 1) return;
@@ -463,9 +472,17 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
   const double ex = (double)(lx - fx) * vs[2];
   const double T = __builtin_sqrt(ey * ey + ez * ez + ex * ex);
   const double D = (double)(len - 1);
+  // RN(span / D) for every segment from one division per ray: with
+  // y = RN(1 / D), q0 = RN(span * y) is within an ulp of span / D and one fused
+  // correction rounds it correctly (Markstein's theorem; also checked for
+  // every D <= 2^18 with 0 <= span <= D + 1 and on 3.2e9 random pairs below
+  // 2^24 on the host)
+  const double yD = 1.0 / D;
   double li = 0.0, lo = 0.0, cr = 0.0, bu = 0.0;
   auto add = [&](int cls, int span) {
-    const double term = T * ((double)span / D);
+    const double a = (double)span;
+    const double q0 = a * yD;
+    const double term = T * __builtin_fma(__builtin_fma(-q0, D, a), yD, q0);
     li += (cls == 1) ? term : 0.0;
     lo += (cls == 2) ? term : 0.0;
     cr += (cls == 3) ? term : 0.0;
