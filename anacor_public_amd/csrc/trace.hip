@@ -1429,6 +1429,13 @@ static void order_by_direction(Context* c, int64_t R, DirOf dir_of) {
     const uint64_t kb = (spread_bits(q(b[0])) << 2) | (spread_bits(q(b[1])) << 1) | spread_bits(q(b[2]));
     key[(size_t)i] = (ka << 21) | kb;
   }
+  if (R <= (int64_t(1) << 22)) {
+    // 42-bit key above a 22-bit index: one contiguous sort, stable by the index
+    for (int64_t i = 0; i < R; ++i) key[(size_t)i] = (key[(size_t)i] << 22) | (uint64_t)i;
+    std::sort(key.begin(), key.end());
+    for (int64_t i = 0; i < R; ++i) c->h_order[(size_t)i] = (int)(key[(size_t)i] & ((1u << 22) - 1));
+    return;
+  }
   std::stable_sort(c->h_order.begin(), c->h_order.end(),
                    [&](int x, int y) { return key[(size_t)x] < key[(size_t)y]; });
 }
