@@ -94,7 +94,9 @@ int anacor_device_count(void);
 /* A device-resident problem: the label volume [z][y][x] (uint8, contiguous)
  * and the crystal-voxel list (int64 [n][3], z y x) are uploaded once;
  * voxel_size (z, y, x) and coefficients [li, lo, cr, bu] as in
- * ray_tracing_overall.  Returns NULL on failure (see anacor_last_error). */
+ * ray_tracing_overall.  The device holds the volume inside a one-voxel shell
+ * of zeros: each extent at most 2^20 and (Z+2)(Y+2)(X+2) < 2^31.  Returns
+ * NULL on failure (see anacor_last_error). */
 typedef struct anacor_ctx anacor_ctx;
 anacor_ctx *anacor_ctx_create(const uint8_t *volume, const int64_t shape[3],
                               const int64_t *coords, int64_t n_coords,
