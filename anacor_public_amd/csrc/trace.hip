@@ -131,6 +131,8 @@ struct TraceArgs {
   unsigned long long* steps;   // sum of len_path_2
   PathRecord* paths;           // [R * N] in original voxel order (paths kernel)
   const int* order;            // [R] caller's index of the reflection at each processing position
+  const double* incd;          // (double)i for i = 0 .. incd_last
+  int incd_last;
 };
 
 // Map hardware workgroup id to a logical one so that consecutive logical
@@ -272,8 +274,7 @@ struct RayLane {
 // SHELL: the face coordinate is not held on its last plane; past it the
 // offset lands in the zero shell of the raw layouts.
 template <bool GUARD, bool WIDE, bool BIAS = true, bool CLAMP = true, bool SHELL = false>
-__device__ __forceinline__ int lane_offset(const RayLane& L, int inc) {
-  const double incd = (double)inc;
+__device__ __forceinline__ int lane_offset(const RayLane& L, int inc, double incd) {
   const int ia = axis_index<GUARD, BIAS, CLAMP>(L.ba, incd, L.da, L.bias, L.ha);
   const int ib = axis_index<GUARD, BIAS, CLAMP>(L.bb, incd, L.db, L.bias, L.hb);
   const int ip = SHELL ? inc : min(inc, L.incmax);
@@ -320,11 +321,18 @@ enum { kLayoutRaw = 0, kLayoutWide = 1, kLayoutBrick2 = 2, kLayoutBrick4 = 3 };
 // voxel (z, y, x) is at (z + 1) * sz + (y + 1) * sy + (x + 1), sy = X + 2,
 // sz = (Y + 2) * sy.  A ray whose face coordinate steps one plane past the
 // volume reads the shell's 0 and stops, so the walk needs no loop bound.
+typedef const __attribute__((address_space(4))) double* const_f64_ptr;
+
 struct Volume {
   __amdgpu_buffer_rsrc_t rsrc;  // range-checked: offsets past the end read 0
   int Z, Y, X;
   int sz, sy, base;             // raw layouts: strides of z and y, offset of voxel (0, 0, 0)
   int cz, cy;                   // brick constants c of z and y (bricked layouts)
+  // (double)i for i = 0 .. incd_last, read through the scalar cache: the
+  // walk's increment is the same in every lane, so its conversion is a
+  // scalar load instead of a vector instruction per step
+  const_f64_ptr incd;
+  int incd_last;
 };
 
 __device__ __forceinline__ int load_label(const Volume& V, int off) {
@@ -344,11 +352,12 @@ __device__ __forceinline__ int load_packed(const Volume& V, int f) {
 
 // Label at increment inc of the lane's ray in the given layout (SHELL: raw
 // layouts only, see lane_offset).
+// incd: (double)inc.
 template <bool GUARD, int LAYOUT, bool BIAS = true, bool CLAMP = true, bool SHELL = false>
-__device__ __forceinline__ int label_at(const RayLane& L, const Volume& V, int inc) {
+__device__ __forceinline__ int label_at(const RayLane& L, const Volume& V, int inc, double incd) {
   if (LAYOUT == kLayoutRaw || LAYOUT == kLayoutWide)
-    return load_label(V, lane_offset<GUARD, LAYOUT == kLayoutWide, BIAS, CLAMP, SHELL>(L, inc));
-  const double incd = (double)inc;
+    return load_label(V, lane_offset<GUARD, LAYOUT == kLayoutWide, BIAS, CLAMP, SHELL>(L, inc,
+                                                                                        incd));
   const int ia = axis_index<GUARD, BIAS, CLAMP>(L.ba, incd, L.da, L.bias, L.ha);
   const int ib = axis_index<GUARD, BIAS, CLAMP>(L.bb, incd, L.db, L.bias, L.hb);
   const int ip = mad24(L.pdir, min(inc, L.incmax), L.p0);
@@ -392,7 +401,7 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
   lds_u32* wp = seg;                          // write slot (32-bit LDS address)
   lds_u32* const cap_slot = seg + CAP * kBlock;
   if (!done) {  // increment 0: its label opens no boundary
-    const int lab = label_at<GUARD, LAYOUT>(L, V, 0);
+    const int lab = label_at<GUARD, LAYOUT>(L, V, 0, 0.0);
     bool stop;
     if (FULL) {
       va = (lab == 0) ? 1 : 0;
@@ -409,9 +418,12 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
       wp = cap_slot;
     }
     int lab[U];
+    // the table covers every increment a walk reaches (extent + 2U); the
+    // bound only keeps the group's scalar loads inside it
+    const const_f64_ptr tbl = V.incd + min(inc0, V.incd_last - (U - 1));
 #pragma unroll
     for (int u = 0; u < U; ++u)  // range-checked loads; steps past the end are not taken
-      lab[u] = label_at<GUARD, LAYOUT, BIAS, CLAMP, SHELL>(L, V, inc0 + u);
+      lab[u] = label_at<GUARD, LAYOUT, BIAS, CLAMP, SHELL>(L, V, inc0 + u, tbl[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int inc = inc0 + u;
@@ -440,11 +452,11 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
       wp = cap_slot;
     }
     const int inc = L.incmax + 1;
-    const int lb = label_at<GUARD, LAYOUT, BIAS, CLAMP>(L, V, inc);
+    const int lb = label_at<GUARD, LAYOUT, BIAS, CLAMP>(L, V, inc, (double)inc);
     if (lb != 0) {
       // prev holds the label of the group's last step, taken or not: read
       // the last taken step's label again
-      const int before = label_at<GUARD, LAYOUT, BIAS, CLAMP>(L, V, L.incmax);
+      const int before = label_at<GUARD, LAYOUT, BIAS, CLAMP>(L, V, L.incmax, (double)L.incmax);
       bool chg = lb != before;
       if (!kAll04) chg = chg && (unsigned)lb <= 4u;
       *wp = ((uint32_t)inc << 3) | (uint32_t)lb;
@@ -499,10 +511,10 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
     }
   } else {
     // the boundary list overflowed: walk the (now known) path again in order
-    int pl = label_at<GUARD, LAYOUT>(L, V, 0);
+    int pl = label_at<GUARD, LAYOUT>(L, V, 0, 0.0);
     nseg = 0;
     for (int inc = 1; inc <= last; ++inc) {
-      const int lb = label_at<GUARD, LAYOUT>(L, V, inc);
+      const int lb = label_at<GUARD, LAYOUT>(L, V, inc, (double)inc);
       if (lb != pl && (unsigned)lb <= 4u) {
         add(cls, inc - start);
         cls = lb;
@@ -783,8 +795,11 @@ __device__ __forceinline__ void reduce_tile(TileAcc& acc, ReduceLds& red, unsign
 
 // nbytes: size of the buffer behind vol (the raw volume, or the packed one).
 __device__ __forceinline__ Volume make_volume(const uint8_t* vol, int Z, int Y, int X, int nbytes,
-                                              int cz, int cy) {
+                                              int cz, int cy, const double* incd = nullptr,
+                                              int incd_last = 0) {
   Volume V;
+  V.incd = (const_f64_ptr)incd;
+  V.incd_last = incd_last;
   V.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)vol, (short)0, nbytes, 0x00020000);
   V.Z = Z; V.Y = Y; V.X = X;
   V.sy = X + 2;
@@ -811,7 +826,7 @@ void trace_kernel(TraceArgs a) {
   const int tile = lb - pos * a.tiles;
   const int refl = a.order[pos];         // the caller's reflection index
   SampleView S;
-  S.V = make_volume(a.vol, a.Z, a.Y, a.X, a.vol_bytes, a.cz, a.cy);
+  S.V = make_volume(a.vol, a.Z, a.Y, a.X, a.vol_bytes, a.cz, a.cy, a.incd, a.incd_last);
   S.coords = a.coords;
   S.N = a.N;
   S.rules = a.rules;
@@ -931,6 +946,8 @@ struct GridArgs {
   float* map;                  // [D][N], voxels in the caller's order
   double* map64;               // same, unrounded (nullable)
   unsigned* err;               // [D]
+  const double* incd;          // (double)i for i = 0 .. incd_last
+  int incd_last;
 };
 
 template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT>
@@ -942,7 +959,8 @@ void grid_map_kernel(GridArgs a) {
   if (lb >= a.n_blocks) return;
   const int d = lb / a.tiles;
   const int tile = lb - d * a.tiles;
-  const Volume V = make_volume(a.vol, a.Z, a.Y, a.X, a.vol_bytes, a.cz, a.cy);
+  const Volume V = make_volume(a.vol, a.Z, a.Y, a.X, a.vol_bytes, a.cz, a.cy, a.incd,
+                               a.incd_last);
   lds_u32* seg = (lds_u32*)(seg_lds + threadIdx.x);
   const RayRule* rr = a.rays + d;
   unsigned err = 0;
@@ -1176,6 +1194,8 @@ struct Context {
   hipStream_t stream = nullptr;
   uint8_t* d_vol = nullptr;              // raw layout with its zero shell
   long long vol_bytes = 0;               // (Z + 2)(Y + 2)(X + 2)
+  double* d_incd = nullptr;              // (double)i, i = 0 .. incd_last (walk increments)
+  int incd_last = 0;
   int Z = 0, Y = 0, X = 0;
   int layout_req = ANACOR_LAYOUT_AUTO;   // requested layout of the trace kernel
   int layout = kLayoutRaw;               // layout in use (kLayout*)
@@ -1514,6 +1534,8 @@ static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flag
   a.steps = c->d_steps;
   a.paths = d_paths;
   a.order = c->d_order;
+  a.incd = c->d_incd;
+  a.incd_last = c->incd_last;
   const int grid = (int)((nblk + 7) / 8 * 8);
 
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
@@ -1693,6 +1715,15 @@ static int ctx_create(const uint8_t* vol, const int64_t shape[3], const int64_t*
       c->labels_over3 = (h_flag & 1u) != 0;
       c->labels_over4 = (h_flag & 2u) != 0;
     }
+    {  // increments as doubles: a walk reaches at most extent + 1 plus one
+       // unrolled group; 32 spare entries
+      const int m = std::max(c->Z, std::max(c->Y, c->X)) + 32;
+      std::vector<double> h((size_t)m);
+      for (int i = 0; i < m; ++i) h[(size_t)i] = (double)i;
+      HIP_TRY(hipMalloc(&c->d_incd, (size_t)m * sizeof(double)));
+      HIP_TRY(hipMemcpy(c->d_incd, h.data(), (size_t)m * sizeof(double), hipMemcpyHostToDevice));
+      c->incd_last = m - 1;
+    }
     c->h_coords.resize((size_t)n);
     for (int64_t i = 0; i < n; ++i) {
       const int64_t z = coords ? coords[3 * i] : coords32[3 * i];
@@ -1723,6 +1754,7 @@ static void ctx_destroy(Context* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->d_vol) (void)hipFree(c->d_vol);
+  if (c->d_incd) (void)hipFree(c->d_incd);
   if (c->d_pack) (void)hipFree(c->d_pack);
   if (c->d_coords) (void)hipFree(c->d_coords);
   if (c->d_rules) (void)hipFree(c->d_rules);
@@ -2011,6 +2043,8 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
     t.steps = b->d_counters;
     t.paths = nullptr;
     t.order = d.order;
+    t.incd = c->d_incd;
+    t.incd_last = c->incd_last;
     dispatch_trace(t, (t.n_blocks + 7) / 8 * 8, guard, b->full != 0,
                    raw_needs_wide(c) ? kLayoutWide : kLayoutRaw, f64, false, !c->labels_over4,
                    b->stream);
@@ -2176,6 +2210,8 @@ static int grid_build(Context* c, int64_t n_theta, int64_t n_phi, const double* 
     a.map = g->d_map;
     a.map64 = d_map64;
     a.err = d_err;
+    a.incd = c->d_incd;
+    a.incd_last = c->incd_last;
     if ((e = hipEventRecord(c->ev0, c->stream)) != hipSuccess) return fail(e, "event");
     const int i = (guard ? 1 : 0) | (c->full ? 2 : 0) | (c->layout << 2);
     kLaunchGrid[i](a, (int)((nblk + 7) / 8 * 8), c->stream);
