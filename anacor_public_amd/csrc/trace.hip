@@ -412,6 +412,10 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
     if (stop) done = true;
     else { prev = lab; cnt = 1; }
   }
+  // With the shell every walk ends on a label 0, and that step records too
+  // (0 differs from any taken label): the last record holds the step count,
+  // so the loop keeps no count per step.
+  const bool ran = !done;
   for (int inc0 = 1; !done; inc0 += U) {
     if (wp > cap_slot) {  // more than CAP boundaries so far: re-walk at the end
       overflow = true;
@@ -437,25 +441,31 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
       }
       // once a lane stops it never takes another step: prev/va need no select
       const bool take = SHELL ? !done && !stop : !done && inc < L.limit && !stop;
-      bool chg = take && lb != prev;
+      bool chg = SHELL ? !done && lb != prev : take && lb != prev;
       if (!kAll04) chg = chg && (unsigned)lb <= 4u;
       *wp = ((uint32_t)inc << 3) | (uint32_t)lb;
       wp = chg ? wp + kBlock : wp;
-      cnt += take ? 1 : 0;
+      if (!SHELL) cnt += take ? 1 : 0;
       prev = lb;
       done = done || !take;
     }
   }
+  bool stop_record = false;
+  if (SHELL && ran) {
+    // the last record is the stop, written in the lane's last group (after
+    // any overflow reset): its increment is the number of steps taken
+    cnt = (int)(*(wp - kBlock) >> 3);
+    stop_record = true;
+  }
   if (SHELL && cnt == L.incmax + 1 && L.limit == L.incmax + 2) {
-    if (wp > cap_slot) {
-      overflow = true;
-      wp = cap_slot;
-    }
     const int inc = L.incmax + 1;
     const int lb = label_at<GUARD, LAYOUT, BIAS, CLAMP>(L, V, inc, (double)inc);
     if (lb != 0) {
-      // prev holds the label of the group's last step, taken or not: read
-      // the last taken step's label again
+      // the ray takes this step: drop the shell's stop record; prev holds
+      // the label of the group's last step, taken or not, so read the last
+      // taken step's label again
+      wp -= kBlock;
+      stop_record = false;
       const int before = label_at<GUARD, LAYOUT, BIAS, CLAMP>(L, V, L.incmax, (double)L.incmax);
       bool chg = lb != before;
       if (!kAll04) chg = chg && (unsigned)lb <= 4u;
@@ -465,7 +475,8 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
     }
   }
 
-  int nseg = (int)(wp - seg) / kBlock;   // exact unless the list overflowed (<= CAP + U)
+  // exact unless the list overflowed (<= CAP + U records, the stop included)
+  int nseg = (int)(wp - seg) / kBlock - (stop_record ? 1 : 0);
   const int last = cnt - 1;
   const int len = cnt > 0 ? cnt : 1;
   len_out = len;
