@@ -502,14 +502,26 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
   // 2^24 on the host)
   const double yD = 1.0 / D;
   double li = 0.0, lo = 0.0, cr = 0.0, bu = 0.0;
+  // With a finite term, acc + (cls == k ? term : 0) equals fma(term, m, acc)
+  // for m = 1.0 or 0.0: the product is exact, and term * 0 = +0 leaves acc
+  // (which starts at +0 and only grows) unchanged.  A non-finite T (absurd
+  // voxel sizes) keeps the selects.
+  const bool finite_T = __builtin_isfinite(T);
   auto add = [&](int cls, int span) {
     const double a = (double)span;
     const double q0 = a * yD;
     const double term = T * __builtin_fma(__builtin_fma(-q0, D, a), yD, q0);
-    li += (cls == 1) ? term : 0.0;
-    lo += (cls == 2) ? term : 0.0;
-    cr += (cls == 3) ? term : 0.0;
-    bu += (cls == 4) ? term : 0.0;
+    if (finite_T) {
+      li = __builtin_fma(term, cls == 1 ? 1.0 : 0.0, li);
+      lo = __builtin_fma(term, cls == 2 ? 1.0 : 0.0, lo);
+      cr = __builtin_fma(term, cls == 3 ? 1.0 : 0.0, cr);
+      bu = __builtin_fma(term, cls == 4 ? 1.0 : 0.0, bu);
+    } else {
+      li += (cls == 1) ? term : 0.0;
+      lo += (cls == 2) ? term : 0.0;
+      cr += (cls == 3) ? term : 0.0;
+      bu += (cls == 4) ? term : 0.0;
+    }
   };
   int cls = 3, start = 0;
   if (!overflow) {
