@@ -178,6 +178,12 @@ bool needs_guard(const RayRule& r, int64_t max_dim) {
   return false;
 }
 
+// RN(1 / dir) per axis into the spare fourth delta slots of faces 1-3: the
+// GPU's cube_face divides by the direction through these reciprocals.
+static void set_reciprocals(RayRule& r) {
+  for (int a = 0; a < 3; ++a) r.d[a][3] = 1.0 / r.dir[a];
+}
+
 void fill_ray(const double* v, bool incident, int64_t max_dim, RayRule& r, double* theta,
               double* phi, double* dir) {
   std::memset(&r, 0, sizeof(r));
@@ -190,6 +196,7 @@ void fill_ray(const double* v, bool incident, int64_t max_dim, RayRule& r, doubl
   r.dir[2] = d3[2];
   r.dir[3] = 0.0;
   face_rules(t, p, r);
+  set_reciprocals(r);
   r.neg = incident ? 1u : 0u;
   r.guard = needs_guard(r, max_dim) ? 1u : 0u;
   *theta = t;
@@ -361,6 +368,7 @@ void host_setup_directions(const double* rotated_s1, const double* theta_phi, in
     r.dir[1] = d3[1];
     r.dir[2] = d3[2];
     face_rules(theta_phi[2 * i], theta_phi[2 * i + 1], r);
+    set_reciprocals(r);
     r.neg = 0u;
     r.guard = needs_guard(r, max_dim) ? 1u : 0u;
   }

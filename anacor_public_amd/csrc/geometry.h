@@ -21,7 +21,8 @@ enum : uint32_t {
 // 256 bytes, so a reflection's two rays are one 512-byte record.
 struct alignas(16) RayRule {
   double dir[4];        // myframe (z, y, x) direction, [3] unused
-  double d[6][4];       // per face: per-increment delta (z, y, x), [3] unused
+  double d[6][4];       // per face: per-increment delta (z, y, x); d[a][3] for
+                        // a = 0, 1, 2: RN(1 / dir[a]), the rest unused
   uint32_t meta[6];     // per face flags, see above
   uint32_t neg;         // 1 for the incident ray (cube_face's L1)
   uint32_t guard;       // 1 when a coordinate can leave the int64 range
