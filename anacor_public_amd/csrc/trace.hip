@@ -8,7 +8,8 @@
 // from neighbouring voxels: their label gathers hit the same cache lines.
 //
 // Numerics (bit-exact path lengths against AnACor/src/ray_tracing_basic.c):
-//   * cube_face: six FP64 divisions, the reference's candidate order and ties.
+//   * cube_face: correctly rounded FP64 quotients (one per axis, two for a
+//     non-finite component), the reference's candidate order and ties.
 //   * stepping: coord + (double)inc * d as a separately rounded multiply and
 //     add (this file is built with -ffp-contract=off), then floor or
 //     round-half-away, the clamp to [0, extent-1] and the double->int64 quirk
