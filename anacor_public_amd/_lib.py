@@ -13,7 +13,9 @@ import threading
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "lib", "libanacor_mi355x.so")
+# ANACOR_LIB selects another build of the same library (e.g. the host-sanitizer
+# build of `python -m anacor_public_amd.build --sanitize`)
+LIB_PATH = os.environ.get("ANACOR_LIB", os.path.join(PKG, "lib", "libanacor_mi355x.so"))
 
 ANACOR_OK = 0
 ANACOR_ERR_ARG = 1
@@ -36,13 +38,16 @@ GATHER_ROW = 1
 EXPORTED = (
     "ray_tracing_overall", "ray_tracing_gpu_overall", "ray_tracing_single",
     "ray_tracing_single_mp", "anacor_last_error", "anacor_device_count", "anacor_ctx_create",
-    "anacor_ctx_destroy", "anacor_ctx_set_tile", "anacor_ctx_set_voxel_block",
+    "anacor_ctx_destroy", "anacor_ctx_set_tile", "anacor_ctx_set_order",
+    "anacor_ctx_get_work_shape",
+    "anacor_ctx_set_voxel_block",
     "anacor_ctx_set_layout", "anacor_ctx_get_layout", "anacor_ctx_run", "anacor_ctx_paths",
     "anacor_ctx_stats", "anacor_reflection_geometry", "anacor_batch_create",
     "anacor_batch_destroy", "anacor_batch_sample", "anacor_batch_run", "anacor_batch_stats",
     "anacor_mlp_create", "anacor_mlp_destroy", "anacor_mlp_num_params", "anacor_mlp_get_params",
     "anacor_mlp_set_params", "anacor_mlp_set_scale", "anacor_mlp_set_features",
-    "anacor_mlp_set_envelope", "anacor_mlp_features", "anacor_mlp_forward",
+    "anacor_mlp_set_envelope", "anacor_mlp_set_bands", "anacor_mlp_features",
+    "anacor_mlp_forward",
     "anacor_mlp_gradients", "anacor_mlp_train", "anacor_mlp_predict", "anacor_mlp_stats",
     "ray_tracing_single_gridding", "nearest_neighbor_interpolate_overall", "anacor_grid_create",
     "anacor_grid_destroy", "anacor_grid_map", "anacor_grid_run", "anacor_grid_stats",
@@ -79,6 +84,14 @@ class Envelope(ct.Structure):
                 ("cells", ct.c_int32 * 3), ("pad", ct.c_int32)]
 
 
+class Band(ct.Structure):
+    _fields_ = [("ray", ct.c_int32), ("i", ct.c_int32), ("j", ct.c_int32), ("pad", ct.c_int32),
+                ("ratio", ct.c_double), ("width", ct.c_double)]
+
+
+MLP_MAX_BANDS = 32
+
+
 class MlpStats(ct.Structure):
     _fields_ = [("mlp_ms", ct.c_double), ("total_ms", ct.c_double), ("rows", ct.c_int64),
                 ("in_envelope", ct.c_int64)]
@@ -106,6 +119,11 @@ def _prototypes(lib):
     lib.anacor_ctx_destroy.argtypes = [vp]
     lib.anacor_ctx_set_tile.restype = ct.c_int
     lib.anacor_ctx_set_tile.argtypes = [vp, i32]
+    lib.anacor_ctx_get_work_shape.restype = ct.c_int
+    lib.anacor_ctx_get_work_shape.argtypes = [vp, vp, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
+                                              ct.POINTER(ct.c_int)]
+    lib.anacor_ctx_set_order.restype = ct.c_int
+    lib.anacor_ctx_set_order.argtypes = [vp, i32, i32]
     lib.anacor_ctx_set_layout.restype = ct.c_int
     lib.anacor_ctx_set_layout.argtypes = [vp, i32]
     lib.anacor_ctx_get_layout.restype = ct.c_int
@@ -147,6 +165,8 @@ def _prototypes(lib):
     lib.anacor_mlp_set_features.argtypes = [vp, i32]
     lib.anacor_mlp_set_envelope.restype = ct.c_int
     lib.anacor_mlp_set_envelope.argtypes = [vp, ct.POINTER(Envelope), vp]
+    lib.anacor_mlp_set_bands.restype = ct.c_int
+    lib.anacor_mlp_set_bands.argtypes = [vp, ct.POINTER(Band), i32]
     lib.anacor_mlp_features.restype = ct.c_int
     lib.anacor_mlp_features.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, vp]
     lib.anacor_mlp_forward.restype = ct.c_int

@@ -44,27 +44,53 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose: bool = True) -> str:
-    os.makedirs(BUILD, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
-    geo_o = os.path.join(BUILD, "geometry.o")
-    _run([CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, "geometry.cpp"), "-o", geo_o])
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer build for the CPU
+# tests: the host code of every source (geometry.cpp with the ROCm clang, the
+# host half of the .hip files through -Xarch_host) is instrumented, the GPU code
+# is not.  Run with the clang runtime preloaded, e.g.
+#   LD_PRELOAD=$(/opt/rocm/llvm/bin/clang -print-file-name=libclang_rt.asan-x86_64.so)
+#   ASAN_OPTIONS=detect_leaks=0 ANACOR_LIB=anacor_public_amd/_sanitize/libanacor_mi355x.so
+#   python -m pytest tests -m "not gpu"
+SAN_BUILD = os.path.join(PKG, "_sanitize")
+SAN_CXX = os.environ.get("SAN_CXX", "/opt/rocm/llvm/bin/clang++")
+SAN = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+       "-fno-omit-frame-pointer", "-g"]
+
+
+def build(verbose: bool = True, sanitize: bool = False) -> str:
+    out_dir = SAN_BUILD if sanitize else BUILD
+    lib_dir = SAN_BUILD if sanitize else LIBDIR
+    os.makedirs(out_dir, exist_ok=True)
+    os.makedirs(lib_dir, exist_ok=True)
+    lib = os.path.join(lib_dir, "libanacor_mi355x.so")
+    geo_o = os.path.join(out_dir, "geometry.o")
+    if sanitize:
+        _run([SAN_CXX, *HOST_FLAGS, *SAN, "-c", os.path.join(CSRC, "geometry.cpp"), "-o", geo_o])
+    else:
+        _run([CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, "geometry.cpp"), "-o", geo_o])
+    host_san = [a for f in SAN for a in ("-Xarch_host", f)] if sanitize else []
     objs = [geo_o]
     for src in HIP_SOURCES:
-        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c",
+        obj = os.path.join(out_dir, os.path.splitext(src)[0] + ".o")
+        _run([HIPCC, *HIP_FLAGS, *host_san, "-I", os.path.join(ROOT, "include"), "-c",
               os.path.join(CSRC, src), "-o", obj])
         objs.insert(0, obj)
-    tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lpthread"])
-    os.replace(tmp, LIB)
-    for alias in ALIASES:
-        dst = os.path.join(LIBDIR, alias)
-        shutil.copyfile(LIB, dst + ".tmp")
-        os.replace(dst + ".tmp", dst)
-    return LIB
+    tmp = lib + ".tmp"
+    # -shared-libsan: the sanitizer runtime becomes a dependency of the
+    # library, so a caller that resolves free through the library (as
+    # utils/utils_mp.py:212 does) gets the runtime's free, matching its malloc
+    link_san = [*host_san, "-shared-libsan"] if sanitize else []
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *link_san, "-o", tmp, *objs,
+          "-lpthread"])
+    os.replace(tmp, lib)
+    if not sanitize:
+        for alias in ALIASES:
+            dst = os.path.join(LIBDIR, alias)
+            shutil.copyfile(LIB, dst + ".tmp")
+            os.replace(dst + ".tmp", dst)
+    return lib
 
 
 if __name__ == "__main__":
-    build()
+    build(sanitize="--sanitize" in sys.argv[1:])
     sys.exit(0)

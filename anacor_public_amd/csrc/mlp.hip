@@ -601,14 +601,34 @@ __global__ void zero_floats(float* __restrict__ p, int n) {
 
 enum { kFeaturesHarmonic = 0, kFeaturesChamber = 1 };
 
+// Resonance bands (anacor_band_t): near a small rational ratio of two
+// components of a ray's direction the voxel steps of all crystal voxels fall
+// into step together, and the traced absorption factor oscillates (moire)
+// faster than the training set samples it.  Reflections inside a band go to
+// the tracer.
 struct FeatureGeo {
   double raw[3], axis[3], F[9], u[3], e1[3], e2[3];
   double axial_lo, axial_hi, tth_lo, tth_hi;
   int cells[3];
   int feature_set;
   int use_env;
+  int n_bands;
   const uint8_t* trust;   // device, cells[0] * cells[1] * cells[2]; null: all trusted
+  anacor_band_t bands[ANACOR_MLP_MAX_BANDS];
 };
+
+// |min(|v_i|, |v_j|) / max(|v_i|, |v_j|) - ratio| < width for some band.
+__device__ __forceinline__ bool in_band(const FeatureGeo& g, const double* x, const double* s) {
+  for (int k = 0; k < g.n_bands; ++k) {
+    const anacor_band_t& b = g.bands[k];
+    const double* v = b.ray ? s : x;
+    const double p = fabs(v[b.i]), q = fabs(v[b.j]);
+    const double hi = fmax(p, q);
+    const double t = hi > 0.0 ? fmin(p, q) / hi : 0.0;
+    if (fabs(t - b.ratio) < b.width) return true;
+  }
+  return false;
+}
 
 __device__ void rodrigues_dev(const double* a, double t, double* r) {
   const double x = a[0], y = a[1], z = a[2];
@@ -706,7 +726,8 @@ __global__ void features_kernel(const double* __restrict__ s1, const double* __r
     if (cell) cell[i] = c;
     if (flag) {
       const bool in = axial >= g.axial_lo && axial <= g.axial_hi && tth >= g.tth_lo &&
-                      tth <= g.tth_hi && (!g.trust || g.trust[c] != 0);   // NaN compares false
+                      tth <= g.tth_hi && (!g.trust || g.trust[c] != 0) &&   // NaN compares false
+                      !in_band(g, x, s);
       flag[i] = in ? 1 : 0;
     }
   }
@@ -754,6 +775,8 @@ struct Model {
   int feature_set = kFeaturesHarmonic;
   bool has_env = false, trust_set = false;
   anacor_envelope_t env{};
+  int n_bands = 0;
+  anacor_band_t bands[ANACOR_MLP_MAX_BANDS] = {};
   double y_mean = 0.0, y_std = 1.0;
   long long adam_t = 0;
   double last_mlp_ms = 0.0, last_total_ms = 0.0;
@@ -918,6 +941,8 @@ static int geo_from(const Model* M, const double* raw, const double* axis, const
     for (int k = 0; k < 3; ++k) g->cells[k] = e.cells[k];
     g->trust = M->trust_set ? M->trust.p : nullptr;
     g->use_env = 1;
+    g->n_bands = M->n_bands;
+    std::memcpy(g->bands, M->bands, sizeof(g->bands));
   } else {
     double u[3], nn = 0.0;
     for (int b = 0; b < 3; ++b) {
@@ -1153,6 +1178,28 @@ int anacor_mlp_set_envelope(anacor_mlp* mlp, const anacor_envelope_t* env, const
   M->env = *env;
   M->has_env = true;
   M->trust_set = trust != nullptr;
+  return ANACOR_OK;
+}
+
+int anacor_mlp_set_bands(anacor_mlp* mlp, const anacor_band_t* bands, int n) {
+  Model* M = model_of(mlp);
+  if (!M || n < 0 || n > ANACOR_MLP_MAX_BANDS || (n > 0 && !bands)) {
+    set_error(ANACOR_ERR_ARG, "mlp set_bands: null model or %d bands (at most %d)", n,
+              ANACOR_MLP_MAX_BANDS);
+    return ANACOR_ERR_ARG;
+  }
+  for (int k = 0; k < n; ++k) {
+    const anacor_band_t& b = bands[k];
+    if ((b.ray != 0 && b.ray != 1) || b.i < 0 || b.i > 2 || b.j < 0 || b.j > 2 || b.i == b.j ||
+        !(b.ratio >= 0.0 && b.ratio <= 1.0) || !(b.width >= 0.0)) {
+      set_error(ANACOR_ERR_ARG, "mlp set_bands: band %d is malformed", k);
+      return ANACOR_ERR_ARG;
+    }
+  }
+  std::lock_guard<std::mutex> lk(M->mu);
+  M->n_bands = n;
+  std::memset(M->bands, 0, sizeof(M->bands));
+  if (n > 0) std::memcpy(M->bands, bands, (size_t)n * sizeof(anacor_band_t));
   return ANACOR_OK;
 }
 

@@ -134,14 +134,37 @@ struct TraceArgs {
   const int* order;            // [R] caller's index of the reflection at each processing position
   const double* incd;          // (double)i for i = 0 .. incd_last
   int incd_last;
+  int group;                   // reflections per group of the launch order (<= 1: none)
 };
 
 // Map hardware workgroup id to a logical one so that consecutive logical
-// blocks (tiles of the same reflection) land on the same XCD: the dispatcher
-// deals workgroups round-robin over the 8 XCDs, each with its own L2.
+// blocks land on the same XCD: the dispatcher deals workgroups round-robin
+// over the 8 XCDs, each with its own L2.
 __device__ __forceinline__ int xcd_logical_block(int hw, int n_hw) {
   const int per = n_hw >> 3;  // n_hw is a multiple of 8
   return (hw & 7) * per + (hw >> 3);
+}
+
+// Logical block lb -> (processing position, voxel tile).  Without groups the
+// tiles of one reflection are consecutive.  With groups of G consecutive
+// processing positions (reflections of similar directions), a group runs
+// tile by tile, each tile for all G reflections, so the workgroups resident
+// on an XCD trace nearly parallel rays from one compact set of voxels and
+// share the bricks they cross in that XCD's L2.  Every group but the last
+// has G reflections.
+__device__ __forceinline__ void block_work(int lb, int tiles, int R, int group, int& pos,
+                                           int& tile) {
+  if (group <= 1) {
+    pos = lb / tiles;
+    tile = lb - pos * tiles;
+    return;
+  }
+  const int span = group * tiles;
+  const int g = lb / span;
+  const int w = lb - g * span;
+  const int gsize = min(group, R - g * group);
+  tile = w / gsize;
+  pos = g * group + (w - tile * gsize);
 }
 
 // The two planes of one axis, t = (0 - c) / e and (n - c) / e with the
@@ -155,19 +178,32 @@ __device__ __forceinline__ int xcd_logical_block(int hw, int n_hw) {
 // Non-candidates are returned as -1, which neither passes the >= 0 test nor
 // equals a minimum.  One division per axis instead of two.
 //
-// k / e for an integer k: with y = RN(1 / e) from the host (the direction is
-// the reflection's, so this is a per-ray value), q0 = RN(k * y) is within an
-// ulp of k / e and one fused correction rounds it correctly (Markstein's
-// theorem; checked on the host for 1.9e9 (k, e) pairs with |k| < 2^21,
-// signed zeros included).  Components below 2^-900 in magnitude, where k * y
-// could overflow, divide.
+// RN(a / b) from y = RN(1 / b) by two fused corrections of q0 = RN(a * y).
+// q0 can be up to about 1.5 ulp from a / b (the error of y, scaled by a, plus
+// the rounding of the product).  The first correction computes
+// q0 + RN(a - b q0) y exactly before rounding it; that value lies within
+// about 4 u^2 |a / b| (u = 2^-53) of a / b, so q1 is faithful (one of the two
+// doubles around a / b).  Markstein's theorem (P. Markstein, IBM J. Res.
+// Dev. 34(1), 1990; Muller et al., Handbook of Floating-Point Arithmetic,
+// 2nd ed., section 4.8) then makes the second correction's
+// RN(q1 + (a - b q1) y), with the exact remainder a - b q1, the correctly
+// rounded quotient.  Callers keep a, b and the results away from overflow
+// and underflow.
+__device__ __forceinline__ double quot_rn(double a, double b, double y) {
+  const double q0 = a * y;
+  const double q1 = __builtin_fma(__builtin_fma(-q0, b, a), y, q0);
+  return __builtin_fma(__builtin_fma(-q1, b, a), y, q1);
+}
+
+// k / e for an integer k (|k| < 2^21): with y = RN(1 / e) from the host (the
+// direction is the reflection's, so this is a per-ray value), quot_rn.
+// Components outside [2^-900, 2^900] in magnitude, where k * y could overflow
+// or y lose precision, divide.
 template <bool INV>
 __device__ __forceinline__ double plane_quot(int k, double e, double y) {
   const double a = (double)k;
-  if (INV && __builtin_fabs(e) >= 0x1p-900) {
-    const double q0 = a * y;
-    return __builtin_fma(__builtin_fma(-q0, e, a), y, q0);
-  }
+  const double m = __builtin_fabs(e);
+  if (INV && m >= 0x1p-900 && m <= 0x1p900) return quot_rn(a, e, y);
   return a / e;
 }
 
@@ -519,11 +555,8 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
   const double ex = (double)(lx - fx) * vs[2];
   const double T = __builtin_sqrt(ey * ey + ez * ez + ex * ex);
   const double D = (double)(len - 1);
-  // RN(span / D) for every segment from one division per ray: with
-  // y = RN(1 / D), q0 = RN(span * y) is within an ulp of span / D and one fused
-  // correction rounds it correctly (Markstein's theorem; also checked for
-  // every D <= 2^18 with 0 <= span <= D + 1 and on 3.2e9 random pairs below
-  // 2^24 on the host)
+  // RN(span / D) for every segment from one division per ray (quot_rn with
+  // y = RN(1 / D); 1 <= D < 2^21, 0 <= span <= D + 1)
   const double yD = 1.0 / D;
   double li = 0.0, lo = 0.0, cr = 0.0, bu = 0.0;
   // With a finite term, acc + (cls == k ? term : 0) equals fma(term, m, acc)
@@ -532,9 +565,7 @@ __device__ __forceinline__ void walk_ray_impl(const RayLane& L, const Volume& V,
   // voxel sizes) keeps the selects.
   const bool finite_T = __builtin_isfinite(T);
   auto add = [&](int cls, int span) {
-    const double a = (double)span;
-    const double q0 = a * yD;
-    const double term = T * __builtin_fma(__builtin_fma(-q0, D, a), yD, q0);
+    const double term = T * quot_rn((double)span, D, yD);
     if (finite_T) {
       li = __builtin_fma(term, cls == 1 ? 1.0 : 0.0, li);
       lo = __builtin_fma(term, cls == 2 ? 1.0 : 0.0, lo);
@@ -870,8 +901,8 @@ void trace_kernel(TraceArgs a) {
 
   const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
   if (lb >= a.n_blocks) return;
-  const int pos = lb / a.tiles;          // processing position
-  const int tile = lb - pos * a.tiles;
+  int pos, tile;                         // processing position, voxel tile
+  block_work(lb, a.tiles, a.R, a.group, pos, tile);
   const int refl = a.order[pos];         // the caller's reflection index
   SampleView S;
   S.V = make_volume(a.vol, a.Z, a.Y, a.X, a.vol_bytes, a.cz, a.cy, a.incd, a.incd_last);
@@ -888,7 +919,9 @@ void trace_kernel(TraceArgs a) {
   trace_tile<CAP, U, GUARD, FULL, LAYOUT, F64, PATHS, L04>(
       S, a.rules + refl, v_begin, v_end, a.is_exp, seg, acc,
       PATHS ? a.paths + (size_t)refl * a.N : nullptr);
-  reduce_tile<F64, PATHS>(acc, red, &a.err[refl], a.steps, a.partial32 + lb, a.partial64 + lb);
+  const int slot = pos * a.tiles + tile;   // finish_kernel reads [position][tile]
+  reduce_tile<F64, PATHS>(acc, red, &a.err[refl], a.steps, a.partial32 + slot,
+                          a.partial64 + slot);
 }
 
 // Per reflection: combine the tile partials in tile order and divide by N.
@@ -1238,6 +1271,17 @@ struct Pinned {
   T* data() { return p; }
 };
 
+// Device allocation owned by one scope: freed on every way out of it.
+struct DeviceTemp {
+  void* p = nullptr;
+  DeviceTemp() = default;
+  DeviceTemp(const DeviceTemp&) = delete;
+  DeviceTemp& operator=(const DeviceTemp&) = delete;
+  ~DeviceTemp() {
+    if (p) (void)hipFree(p);
+  }
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -1260,6 +1304,8 @@ struct Context {
   double mu[4] = {0, 0, 0, 0};
   int full = 0;
   int tile_voxels = 0;          // 0: automatic
+  int refl_group = 0;           // reflections per launch group (0: automatic, 1: none)
+  int sort_major = 0;           // direction sort key led by ray 0 (incident) or 1 (diffracted)
   std::vector<int4> h_coords;   // (z, y, x, index) in the caller's order
   int block[3] = {0, 0, 0};     // voxel grouping in use (0,0,0: Morton)
   // reusable per-run buffers
@@ -1426,7 +1472,9 @@ static bool bricks_fit(const Context* c) {
 // Automatic: bricks for volumes above 64 MB, which exceed the L2s and most of
 // the Infinity Cache in the raw layout; raw otherwise, where the kernel is
 // bound by VALU issue and the brick address arithmetic would only add to it.
-static int ensure_layout(Context* c) {
+// may_build false: an error instead of allocating a bricked copy that is not
+// there yet.
+static int ensure_layout(Context* c, bool may_build = true) {
   int want = c->layout_req;
   if (want == ANACOR_LAYOUT_AUTO) {
     const int64_t nvox = (int64_t)c->Z * c->Y * c->X;
@@ -1448,6 +1496,11 @@ static int ensure_layout(Context* c) {
   int bits = (want == ANACOR_LAYOUT_BRICK4 || c->labels_over3) ? 4 : 2;
   if (want == ANACOR_LAYOUT_BRICK2 && c->labels_over3) {
     set_error(ANACOR_ERR_ARG, "2-bit bricks need labels in 0..3");
+    return ANACOR_ERR_ARG;
+  }
+  if (c->pack_bits != bits && !may_build) {
+    set_error(ANACOR_ERR_ARG, "the %d-bit bricked copy of the volume is not built yet "
+              "(anacor_ctx_set_layout builds it)", bits);
     return ANACOR_ERR_ARG;
   }
   if (c->pack_bits != bits) {
@@ -1478,10 +1531,10 @@ static int ensure_layout(Context* c) {
 // workgroups resident on an XCD at one time trace nearly parallel rays whose
 // label gathers share that XCD's L2.  Results are written back through the
 // inverse permutation, so the order is invisible to callers.
+// order[R] receives the processing order; key[R] is work space.
 template <class DirOf>   // DirOf(i, k): direction (z, y, x) of ray k of reflection i
-static void order_by_direction(Context* c, int64_t R, DirOf dir_of) {
-  c->h_order.resize((size_t)R);
-  for (int64_t i = 0; i < R; ++i) c->h_order[(size_t)i] = (int)i;
+static void direction_order(int64_t R, DirOf dir_of, int* order, uint64_t* key, int major = 0) {
+  for (int64_t i = 0; i < R; ++i) order[i] = (int)i;
   const char* env = std::getenv("ANACOR_SORT_REFLECTIONS");
   if (env && env[0] == '0') return;
   auto q = [](double v) {
@@ -1490,30 +1543,162 @@ static void order_by_direction(Context* c, int64_t R, DirOf dir_of) {
     if (t > 127.0) t = 127.0;
     return (uint64_t)t;
   };
-  std::vector<uint64_t> key((size_t)R);
   for (int64_t i = 0; i < R; ++i) {
-    const double* a = dir_of(i, 0);
-    const double* b = dir_of(i, 1);
+    const double* a = dir_of(i, major);
+    const double* b = dir_of(i, 1 - major);
     const uint64_t ka = (spread_bits(q(a[0])) << 2) | (spread_bits(q(a[1])) << 1) | spread_bits(q(a[2]));
     const uint64_t kb = (spread_bits(q(b[0])) << 2) | (spread_bits(q(b[1])) << 1) | spread_bits(q(b[2]));
-    key[(size_t)i] = (ka << 21) | kb;
+    key[i] = (ka << 21) | kb;
   }
   if (R <= (int64_t(1) << 22)) {
-    // 42-bit key above a 22-bit index: one contiguous sort, stable by the index
-    for (int64_t i = 0; i < R; ++i) key[(size_t)i] = (key[(size_t)i] << 22) | (uint64_t)i;
-    std::sort(key.begin(), key.end());
-    for (int64_t i = 0; i < R; ++i) c->h_order[(size_t)i] = (int)(key[(size_t)i] & ((1u << 22) - 1));
+    // 42-bit key above a 22-bit index: one contiguous sort (in place), stable
+    // by the index
+    for (int64_t i = 0; i < R; ++i) key[i] = (key[i] << 22) | (uint64_t)i;
+    std::sort(key, key + R);
+    for (int64_t i = 0; i < R; ++i) order[i] = (int)(key[i] & ((1u << 22) - 1));
     return;
   }
-  std::stable_sort(c->h_order.begin(), c->h_order.end(),
-                   [&](int x, int y) { return key[(size_t)x] < key[(size_t)y]; });
+  std::stable_sort(order, order + R, [&](int x, int y) { return key[x] < key[y]; });
+}
+
+template <class DirOf>
+static void order_by_direction(Context* c, int64_t R, DirOf dir_of) {
+  c->h_order.resize((size_t)R);
+  std::vector<uint64_t> key((size_t)R);
+  direction_order(R, dir_of, c->h_order.data(), key.data(), c->sort_major);
 }
 
 static void order_reflections(Context* c, const ReflRule* rules, int64_t R) {
   order_by_direction(c, R, [rules](int64_t i, int k) { return rules[i].ray[k].dir; });
 }
 
-static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flags, double* out,
+// Reflections per group of the trace launch order (block_work).  Automatic:
+// none in the raw layouts, whose volumes the L2s and the Infinity Cache
+// already hold; kBrickGroup in the bricked ones.
+constexpr int kBrickGroup = 128;
+static int launch_group(const Context* c, int layout) {
+  if (c->refl_group > 0) return c->refl_group;
+  return (layout == kLayoutBrick2 || layout == kLayoutBrick4) ? kBrickGroup : 1;
+}
+
+// Device buffers of one trace call (the context's grow-only ones,
+// ensure_run_buffers).
+struct RunBuffers {
+  ReflRule* rules = nullptr;           // [R]
+  int* order = nullptr;                // [R] processing position -> caller's index
+  unsigned* err = nullptr;             // [R] per-reflection error bits
+  double* out = nullptr;               // [R] means
+  void* partial = nullptr;             // [R * tiles] float2 or double
+  unsigned long long* steps = nullptr; // voxel-step counter
+};
+
+// Clears what a trace call accumulates into (a fill kernel rather than
+// hipMemsetAsync, so the sequence holds no memset node if it is captured).
+__global__ void clear_run_kernel(unsigned* __restrict__ err, int R,
+                                 unsigned long long* __restrict__ steps) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < R) err[i] = 0u;
+  if (i == 0) *steps = 0ull;
+}
+
+// One trace of R reflections from their host rules h_rules and processing
+// order h_order (both uploaded), with vpt voxels per tile in `layout`: the
+// trace kernel, then (without path records) the finish kernel and the
+// download of the means into out.  h_err (R words, host) receives the error
+// words when some mean is NaN; nullptr: a temporary.  Synchronous.
+static int trace_call(Context* c, const RunBuffers& B, const ReflRule* h_rules,
+                      const int* h_order, int64_t R, int vpt, int layout, int flags, double* out,
+                      PathRecord* d_paths, PathRecord* paths_out, unsigned* h_err) {
+  const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
+  const bool paths = paths_out != nullptr;
+  const int64_t tiles = (c->N + vpt - 1) / vpt;
+  const int64_t nblk = R * tiles;
+  bool guard = false;
+  for (int64_t i = 0; i < R; ++i)
+    guard = guard || h_rules[i].ray[0].guard || h_rules[i].ray[1].guard;
+
+  HIP_TRY(hipMemcpyAsync(B.rules, h_rules, (size_t)R * sizeof(ReflRule), hipMemcpyHostToDevice,
+                         c->stream));
+  HIP_TRY(hipMemcpyAsync(B.order, h_order, (size_t)R * sizeof(int), hipMemcpyHostToDevice,
+                         c->stream));
+  hipLaunchKernelGGL(clear_run_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, c->stream,
+                     B.err, (int)R, B.steps);
+  HIP_TRY(hipGetLastError());
+
+  TraceArgs a;
+  const bool bricked = layout == kLayoutBrick2 || layout == kLayoutBrick4;
+  a.vol = bricked ? c->d_pack : c->d_vol;
+  a.vol_bytes = bricked ? (int)c->pack_bytes : (int)c->vol_bytes;
+  a.cz = 512 * c->NBX * c->NBY - 512;
+  a.cy = 512 * c->NBX - 64;
+  a.Z = c->Z; a.Y = c->Y; a.X = c->X;
+  a.coords = c->d_coords;
+  a.N = c->N;
+  a.rules = B.rules;
+  a.R = (int)R;
+  a.tiles = (int)tiles;
+  a.vox_per_tile = vpt;
+  a.n_blocks = (int)nblk;
+  for (int i = 0; i < 3; ++i) a.vs[i] = c->vs[i];
+  for (int i = 0; i < 4; ++i) a.mu[i] = c->mu[i];
+  a.is_exp = (flags & ANACOR_FLAG_NO_EXP) ? 0 : 1;
+  a.partial32 = (float2*)B.partial;
+  a.partial64 = (double*)B.partial;
+  a.err = B.err;
+  a.steps = B.steps;
+  a.paths = d_paths;
+  a.order = B.order;
+  a.incd = c->d_incd;
+  a.incd_last = c->incd_last;
+  a.group = launch_group(c, layout);
+  const int grid = (int)((nblk + 7) / 8 * 8);
+
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  dispatch_trace(a, grid, guard, c->full != 0, layout, f64, paths, !c->labels_over4, c->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  if (!paths) {
+    hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, c->stream,
+                       (const float2*)B.partial, (const double*)B.partial, B.err, B.order,
+                       (int)R, (int)tiles, c->N, f64 ? 1 : 0, B.out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, B.out, (size_t)R * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  } else {
+    HIP_TRY(hipMemcpyAsync(paths_out, d_paths, (size_t)R * c->N * sizeof(PathRecord),
+                           hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(&c->last_steps, B.steps, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipEventElapsedTime(&c->last_kernel_ms, c->ev0, c->ev1));
+  c->last_blocks = (int)nblk;
+  if (paths) return ANACOR_OK;
+  // report the first failing reflection (its result is already NaN); the
+  // error words are read back only when some result is NaN
+  int64_t bad = -1;
+  for (int64_t i = 0; i < R && bad < 0; ++i)
+    if (out[i] != out[i]) bad = i;
+  if (bad < 0) return ANACOR_OK;
+  std::vector<unsigned> tmp;
+  if (!h_err) {
+    tmp.resize((size_t)R);
+    h_err = tmp.data();
+  }
+  HIP_TRY(hipMemcpy(h_err, B.err, (size_t)R * sizeof(unsigned), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < R; ++i)
+    if (h_err[i]) {
+      set_error(ANACOR_ERR_GEOMETRY,
+                "reflection %lld: %s (the reference aborts here); result set to NaN",
+                (long long)i,
+                (h_err[i] & ANACOR_RAY_ERR_FACE) ? "no exit face (cube_face)"
+                                                 : "exit face inconsistent with beam angles");
+      return ANACOR_ERR_GEOMETRY;
+    }
+  return ANACOR_OK;
+}
+
+static int run_rules(Context* c, const ReflRule* h_rules, int64_t R, int flags, double* out,
                      PathRecord* paths_out) {
   const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
   const bool paths = paths_out != nullptr;
@@ -1530,107 +1715,29 @@ static int run_rules(Context* c, const ReflRule* h_rules_in, int64_t R, int flag
     return ANACOR_ERR_ARG;
   }
   const int64_t tiles = (c->N + vpt - 1) / vpt;
-  const int64_t nblk = R * tiles;
-  int rc = ensure_run_buffers(c, R, nblk, f64);
+  int rc = ensure_run_buffers(c, R, R * tiles, f64);
   if (rc) return rc;
-  order_reflections(c, h_rules_in, R);
+  order_reflections(c, h_rules, R);
   HIP_TRY(c->h_order_pin.reserve((size_t)R));
   for (int64_t i = 0; i < R; ++i) c->h_order_pin[(size_t)i] = c->h_order[(size_t)i];
-  const ReflRule* h_rules = h_rules_in;
-  bool guard = false;
-  for (int64_t i = 0; i < R; ++i)
-    guard = guard || h_rules[i].ray[0].guard || h_rules[i].ray[1].guard;
-
-  PathRecord* d_paths = nullptr;
-  if (paths) HIP_TRY(hipMalloc(&d_paths, (size_t)R * c->N * sizeof(PathRecord)));
-
-  HIP_TRY(hipMemcpyAsync(c->d_rules, h_rules, (size_t)R * sizeof(ReflRule),
-                         hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order_pin.data(), (size_t)R * sizeof(int),
-                         hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipMemsetAsync(c->d_err, 0, (size_t)R * sizeof(unsigned), c->stream));
-  HIP_TRY(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
 
   int layout = raw_needs_wide(c) ? kLayoutWide : kLayoutRaw;   // path records: raw layout
   if (!paths) {
     rc = ensure_layout(c);
-    if (rc) {
-      if (d_paths) (void)hipFree(d_paths);
-      return rc;
-    }
+    if (rc) return rc;
     layout = c->layout;
   }
-  TraceArgs a;
-  const bool bricked = layout == kLayoutBrick2 || layout == kLayoutBrick4;
-  a.vol = bricked ? c->d_pack : c->d_vol;
-  a.vol_bytes = bricked ? (int)c->pack_bytes : (int)c->vol_bytes;
-  a.cz = 512 * c->NBX * c->NBY - 512;
-  a.cy = 512 * c->NBX - 64;
-  a.Z = c->Z; a.Y = c->Y; a.X = c->X;
-  a.coords = c->d_coords;
-  a.N = c->N;
-  a.rules = c->d_rules;
-  a.R = (int)R;
-  a.tiles = (int)tiles;
-  a.vox_per_tile = vpt;
-  a.n_blocks = (int)nblk;
-  for (int i = 0; i < 3; ++i) a.vs[i] = c->vs[i];
-  for (int i = 0; i < 4; ++i) a.mu[i] = c->mu[i];
-  a.is_exp = (flags & ANACOR_FLAG_NO_EXP) ? 0 : 1;
-  a.partial32 = (float2*)c->d_partial;
-  a.partial64 = (double*)c->d_partial;
-  a.err = c->d_err;
-  a.steps = c->d_steps;
-  a.paths = d_paths;
-  a.order = c->d_order;
-  a.incd = c->d_incd;
-  a.incd_last = c->incd_last;
-  const int grid = (int)((nblk + 7) / 8 * 8);
-
-  HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  dispatch_trace(a, grid, guard, c->full != 0, layout, f64, paths, !c->labels_over4, c->stream);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(c->ev1, c->stream));
-  if (!paths) {
-    hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, c->stream,
-                       (const float2*)c->d_partial, (const double*)c->d_partial, c->d_err,
-                       c->d_order, (int)R, (int)tiles, c->N, f64 ? 1 : 0, c->d_out);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(out, c->d_out, (size_t)R * sizeof(double), hipMemcpyDeviceToHost,
-                           c->stream));
-  } else {
-    HIP_TRY(hipMemcpyAsync(paths_out, d_paths, (size_t)R * c->N * sizeof(PathRecord),
-                           hipMemcpyDeviceToHost, c->stream));
-  }
-  HIP_TRY(hipMemcpyAsync(&c->last_steps, c->d_steps, sizeof(unsigned long long),
-                         hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipEventElapsedTime(&c->last_kernel_ms, c->ev0, c->ev1));
-  c->last_blocks = (int)nblk;
-  if (d_paths) HIP_TRY(hipFree(d_paths));
-  if (!paths) {
-    // report the first failing reflection (its result is already NaN); the
-    // error words are read back only when some result is NaN
-    int64_t bad = -1;
-    for (int64_t i = 0; i < R && bad < 0; ++i)
-      if (out[i] != out[i]) bad = i;
-    if (bad >= 0) {
-      std::vector<unsigned> herr((size_t)R);
-      HIP_TRY(hipMemcpy(herr.data(), c->d_err, (size_t)R * sizeof(unsigned),
-                        hipMemcpyDeviceToHost));
-      for (int64_t i = 0; i < R; ++i)
-        if (herr[(size_t)i]) {
-          set_error(ANACOR_ERR_GEOMETRY,
-                    "reflection %lld: %s (the reference aborts here); result set to NaN",
-                    (long long)i,
-                    (herr[(size_t)i] & ANACOR_RAY_ERR_FACE)
-                        ? "no exit face (cube_face)"
-                        : "exit face inconsistent with beam angles");
-          return ANACOR_ERR_GEOMETRY;
-        }
-    }
-  }
-  return ANACOR_OK;
+  DeviceTemp paths_buf;   // path records, freed on every return below
+  if (paths) HIP_TRY(hipMalloc(&paths_buf.p, (size_t)R * c->N * sizeof(PathRecord)));
+  RunBuffers B;
+  B.rules = c->d_rules;
+  B.order = c->d_order;
+  B.err = c->d_err;
+  B.out = c->d_out;
+  B.partial = c->d_partial;
+  B.steps = c->d_steps;
+  return trace_call(c, B, h_rules, c->h_order_pin.data(), R, vpt, layout, flags, out,
+                    (PathRecord*)paths_buf.p, paths_out, nullptr);
 }
 
 static void ctx_destroy(Context* c);
@@ -1742,10 +1849,7 @@ static int ctx_create(const uint8_t* vol, const int64_t shape[3], const int64_t*
     c->N = (int)n;
     {  // upload, scan which label ranges the volume holds (layout choice, walk
        // variant), and lay it out with its zero shell
-      struct Temp {   // freed on every way out of this block
-        void* p = nullptr;
-        ~Temp() { if (p) (void)hipFree(p); }
-      } in, flag;
+      DeviceTemp in, flag;   // freed on every way out of this block
       unsigned h_flag = 0;
       HIP_TRY(hipMalloc(&in.p, nvox));
       HIP_TRY(hipMemcpy(in.p, vol, nvox, hipMemcpyHostToDevice));
@@ -2094,6 +2198,7 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
     t.order = d.order;
     t.incd = c->d_incd;
     t.incd_last = c->incd_last;
+    t.group = launch_group(c, kLayoutRaw);
     dispatch_trace(t, (t.n_blocks + 7) / 8 * 8, guard, b->full != 0,
                    raw_needs_wide(c) ? kLayoutWide : kLayoutRaw, f64, false, !c->labels_over4,
                    b->stream);
@@ -2375,9 +2480,16 @@ static int grid_lookup(hipStream_t stream, const float* map, const double* map64
   g.R = (int)R;
   g.out = d_out;
   g.err = d_err;
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
+  struct Events {   // destroyed on every way out
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Events() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } ev;
+  HIP_TRY(hipEventCreate(&ev.e0));
+  HIP_TRY(hipEventCreate(&ev.e1));
+  hipEvent_t e0 = ev.e0, e1 = ev.e1;
   HIP_TRY(hipEventRecord(e0, stream));
   const bool res = map != nullptr;
   if (method == 1 && res) hipLaunchKernelGGL((grid_lookup_kernel<1, true>), dim3((unsigned)R), dim3(kBlock), 0, stream, g);
@@ -2392,8 +2504,6 @@ static int grid_lookup(hipStream_t stream, const float* map, const double* map64
                          hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   if (ms) (void)hipEventElapsedTime(ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   for (int64_t i = 0; i < R; ++i)
     if (h_err[(size_t)i]) {
       set_error(ANACOR_ERR_GEOMETRY, "reflection %lld: direction outside the grid (result NaN)",
@@ -2733,8 +2843,9 @@ static int ib_run(Context* c, int64_t R, int num_cls, int flags, double* out, do
     rc = ensure_layout(c);
   }
   if (rc) return rc;
-  double* d_len = nullptr;
-  if (lengths) HIP_TRY(hipMalloc(&d_len, (size_t)R * c->N * 8 * sizeof(double)));
+  DeviceTemp len_buf;   // per-ray lengths, freed on every return below
+  if (lengths) HIP_TRY(hipMalloc(&len_buf.p, (size_t)R * c->N * 8 * sizeof(double)));
+  double* const d_len = (double*)len_buf.p;
 
   HIP_TRY(hipMemcpyAsync(c->d_ib, c->h_ib.data(), (size_t)R * sizeof(IbRule),
                          hipMemcpyHostToDevice, c->stream));
@@ -2792,7 +2903,6 @@ static int ib_run(Context* c, int64_t R, int num_cls, int flags, double* out, do
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipEventElapsedTime(&c->last_kernel_ms, c->ev0, c->ev1));
   c->last_blocks = (int)nblk;
-  if (d_len) HIP_TRY(hipFree(d_len));
   return report_ray_errors(c, out, R);
 }
 
@@ -2863,6 +2973,34 @@ int anacor_ctx_set_voxel_block(anacor_ctx* ctx, int bz, int by, int bx) {
   std::lock_guard<std::mutex> lk(ctx->c->mu_run);
   HIP_TRY(hipSetDevice(ctx->c->device));
   return upload_voxel_order(ctx->c, bz, by, bx);
+}
+
+int anacor_ctx_set_order(anacor_ctx* ctx, int refl_group, int direction_major) {
+  if (!ctx || !ctx->c || refl_group < 0 || refl_group > (1 << 16) ||
+      (direction_major != 0 && direction_major != 1)) {
+    set_error(ANACOR_ERR_ARG, "bad context or launch order (%d, %d)", refl_group,
+              direction_major);
+    return ANACOR_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(ctx->c->mu_run);
+  ctx->c->refl_group = refl_group;
+  ctx->c->sort_major = direction_major;
+  return ANACOR_OK;
+}
+
+int anacor_ctx_get_work_shape(const anacor_ctx* ctx, int block[3], int* tile, int* refl_group,
+                              int* direction_major) {
+  if (!ctx || !ctx->c) {
+    set_error(ANACOR_ERR_ARG, "null context");
+    return ANACOR_ERR_ARG;
+  }
+  const Context* c = ctx->c;
+  if (block)
+    for (int i = 0; i < 3; ++i) block[i] = c->block[i];
+  if (tile) *tile = c->tile_voxels;
+  if (refl_group) *refl_group = launch_group(c, c->layout);
+  if (direction_major) *direction_major = c->sort_major;
+  return ANACOR_OK;
 }
 
 int anacor_ctx_set_tile(anacor_ctx* ctx, int voxels_per_tile) {

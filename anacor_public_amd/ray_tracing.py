@@ -225,6 +225,24 @@ class Tracer:
     def set_tile(self, voxels_per_tile: int):
         self._lib.anacor_ctx_set_tile(self.handle, int(voxels_per_tile))
 
+    def set_order(self, refl_group: int = 0, direction_major: int = 0):
+        """Launch order of the trace kernel: groups of refl_group direction-sorted
+        reflections run tile by tile (0 automatic, 1 none); the sort key is led
+        by the incident (0) or the diffracted (1) ray.  Speed only."""
+        _lib.check(self._lib.anacor_ctx_set_order(self.handle, int(refl_group),
+                                                  int(direction_major)), self._lib)
+
+    def work_shape(self) -> dict:
+        """The work shape in use: voxel block, tile setting (0 automatic),
+        reflections per launch group, leading ray of the direction sort."""
+        blk = (ct.c_int * 3)()
+        tile, group, major = ct.c_int(), ct.c_int(), ct.c_int()
+        _lib.check(self._lib.anacor_ctx_get_work_shape(self.handle, blk, ct.byref(tile),
+                                                       ct.byref(group), ct.byref(major)),
+                   self._lib)
+        return {"block": list(blk), "tile": tile.value, "group": group.value,
+                "direction_major": major.value}
+
     def set_layout(self, layout: int):
         """Volume layout of the trace kernel (_lib.LAYOUT_*): raw uint8, or 8x8x8
         bricks of 2-bit / 4-bit labels.  Results are bit-identical in every

@@ -114,6 +114,17 @@ class MLP:
                                                       _p(t) if t is not None else None))
         self._trust_keep = t
 
+    def set_bands(self, bands) -> None:
+        """bands: sequence of (ray, i, j, ratio, width) (see fit_bands); empty removes them."""
+        n = len(bands)
+        if n > _lib.MLP_MAX_BANDS:
+            raise ValueError(f"{n} bands, at most {_lib.MLP_MAX_BANDS}")
+        arr = (_lib.Band * max(n, 1))()
+        for k, (ray, i, j, ratio, width) in enumerate(bands):
+            arr[k].ray, arr[k].i, arr[k].j = int(ray), int(i), int(j)
+            arr[k].ratio, arr[k].width = float(ratio), float(width)
+        self._check(self._lib.anacor_mlp_set_bands(self.handle, arr, n))
+
     def features(self, s1, omega, raw_xray, omega_axis, F, cells: bool = False,
                  flags: bool = False):
         """X float32[R][12], invariants float64[R][4] (axial, two_theta, alpha,
@@ -177,6 +188,77 @@ class MLP:
                 "in_envelope": st.in_envelope}
 
 
+BAND_PAIRS = ((0, 1), (0, 2), (1, 2))
+BAND_RATIOS = (0.0, 0.25, 1.0 / 3.0, 0.5, 2.0 / 3.0, 0.75, 1.0)
+
+
+def _component_ratio(v: np.ndarray, i: int, j: int) -> np.ndarray:
+    a, b = np.abs(v[:, i]), np.abs(v[:, j])
+    hi = np.maximum(a, b)
+    return np.where(hi > 0, np.minimum(a, b) / np.where(hi > 0, hi, 1.0), 0.0)
+
+
+def fit_bands(x: np.ndarray, s: np.ndarray, err: np.ndarray, target: float,
+              max_width: float = 0.06, grow: float = 1.1, max_bands: int = 32,
+              max_step: float = 0.05):
+    """Resonance bands that cover every training reflection whose relative
+    error exceeds target.
+
+    Near a small rational ratio t = min(|v_i|, |v_j|) / max(|v_i|, |v_j|) of
+    two components of a ray's crystal-frame direction v (t = 0, 1/4, 1/3,
+    1/2, 2/3, 3/4, 1), the voxel steps of all crystal voxels fall into step
+    together and the traced absorption factor oscillates faster than the
+    training set samples it; away from them the network's error is small.
+    Greedy: take the worst uncovered reflection, widen the (ray, i, j, ratio)
+    band that covers it at the least additional training fallback (a band
+    wider than max_width, or one step that would take more than max_step of
+    the training set, is not a candidate), repeat.  Widths are grown by
+    `grow` as a margin for reflections the training set does not hold.
+
+    x, s: [n, 3] unit vectors (incident, diffracted); err: [n] relative
+    errors.  Returns (bands as (ray, i, j, ratio, width) tuples, indices of
+    reflections no band can cover)."""
+    cands = [(r, i, j, q) for r in (0, 1) for (i, j) in BAND_PAIRS for q in BAND_RATIOS]
+    dist = np.stack([np.abs(_component_ratio(x if r == 0 else s, i, j) - q)
+                     for r, i, j, q in cands])
+    n = len(err)
+    width = np.zeros(len(cands))
+    served = np.ones(n, dtype=bool)
+    left = []
+    while True:
+        bad = served & (err > target)
+        if not bad.any():
+            break
+        k = int(np.argmax(np.where(bad, err, -1.0)))
+        need = dist[:, k] * (1.0 + 1e-6) + 1e-12
+        ok = (need <= max_width) & (dist[:, k] > 0)
+        if np.count_nonzero(width) >= max_bands:
+            ok &= width > 0
+        best, pick = None, -1
+        for c in np.flatnonzero(ok):
+            add = np.count_nonzero(served & (dist[c] < max(width[c], need[c])))
+            if add > max_step * n:
+                continue
+            if best is None or add < best:
+                best, pick = add, int(c)
+        if pick < 0:
+            served[k] = False
+            left.append(k)
+            continue
+        width[pick] = max(width[pick], need[pick])
+        served &= ~(dist[pick] < width[pick])
+    bands = [(*cands[c], float(width[c] * grow)) for c in np.flatnonzero(width)]
+    return bands, np.asarray(left, dtype=np.int64)
+
+
+def in_bands(bands, x: np.ndarray, s: np.ndarray) -> np.ndarray:
+    """bool[n]: the reflections the bands send to the tracer."""
+    out = np.zeros(len(x), dtype=bool)
+    for r, i, j, q, w in bands:
+        out |= np.abs(_component_ratio(x if r == 0 else s, i, j) - q) < w
+    return out
+
+
 def crystal_axis(omega_axis, F) -> np.ndarray:
     """The goniometer axis in the crystal frame, unit F^T omega_axis (the
     rotation about omega_axis leaves it fixed)."""
@@ -217,6 +299,8 @@ class Surrogate:
         self.mlp.set_features(self.feature_set)
         self.envelope: Optional[_lib.Envelope] = None
         self.trust: Optional[np.ndarray] = None
+        self.bands: list = []
+        self.bounded = False
         self.y_mean = 0.0
         self.y_std = 1.0
 
@@ -242,12 +326,19 @@ class Surrogate:
     def fit(self, s1, omega, epochs: int = 600, batch: int = 1024, lr: float = 1e-3,
             lr_min: float = 1e-6, seed: int = 0, targets: Optional[np.ndarray] = None,
             cells=(64, 8, 32), tau: Optional[float] = None, min_count: int = 4,
-            axial_tol: float = 1e-9) -> dict:
+            axial_tol: float = 1e-9, bound: Optional[float] = 1e-3,
+            band_target: float = 0.85, band_grow: float = 1.1) -> dict:
         """Trace the training reflections (unless targets are given), train
         with Adam, and set the envelope: the training axial and two-theta
         ranges, and trust only in the (alpha, two_theta, beta) cells holding at
         least min_count training reflections (and, with tau, whose training
-        relative errors are all at most tau)."""
+        relative errors are all at most tau).
+
+        With a bound, resonance bands are fitted to the training errors
+        (fit_bands with target band_target * bound, widths grown by
+        band_grow) and the surrogate starts in bounded mode: reflections in
+        a band go to the tracer (set_bounded(False) serves them from the
+        network again)."""
         t0 = time.perf_counter()
         A = self.trace(s1, omega) if targets is None else _f64(targets)
         t_trace = time.perf_counter() - t0
@@ -278,10 +369,19 @@ class Surrogate:
         ok = count >= min_count
         if tau is not None:
             ok &= worst <= tau
+        bands, left = [], np.zeros(0, dtype=np.int64)
+        if bound is not None:
+            x64, s64 = X[:, 0:3].astype(np.float64), X[:, 3:6].astype(np.float64)
+            bands, left = fit_bands(x64, s64, res, band_target * bound, grow=band_grow)
+            ok[cell[left]] = False   # errors no band covers: their cells go to the tracer
         trust = ok.astype(np.uint8)
         self.mlp.set_envelope(env, trust)
         self.envelope, self.trust = env, trust
+        self.bands = bands
+        self.set_bounded(bound is not None)
         served = trust[cell].astype(bool)
+        band_out = (in_bands(bands, X[:, 0:3].astype(np.float64), X[:, 3:6].astype(np.float64))
+                    if bands else np.zeros(len(y), dtype=bool))
         return {"n_train": int(len(y)), "trace_s": t_trace, "train_s": t_train,
                 "epochs": int(epochs), "batch": int(batch), "lr": lr, "lr_min": lr_min,
                 "final_loss": float(losses[-1]), "losses": losses,
@@ -290,7 +390,16 @@ class Surrogate:
                                   float(np.degrees(env.two_theta[1]))),
                 "train_rel_max": float(res.max()), "train_rel_mean": float(res.mean()),
                 "cells": list(cells), "tau": tau, "min_count": min_count,
-                "trusted_cells": float(trust.mean()), "train_served": float(served.mean())}
+                "trusted_cells": float(trust.mean()), "train_served": float(served.mean()),
+                "bound": bound, "bands": [list(b) for b in bands],
+                "band_train_fallback": float(band_out.mean()),
+                "band_uncovered": int(len(left))}
+
+    def set_bounded(self, bounded: bool) -> None:
+        """Bounded mode: the fitted resonance bands send their reflections to
+        the tracer.  Fast mode: the network answers them too."""
+        self.mlp.set_bands(self.bands if bounded else [])
+        self.bounded = bool(bounded) and bool(self.bands)
 
     def predict(self, s1, omega, fallback: bool = True):
         """(A float64[R], used bool[R]): the network's A_h where it may answer
@@ -311,7 +420,9 @@ class Surrogate:
                 "axis": np.array(e.axis[:]), "e1": np.array(e.e1[:]), "e2": np.array(e.e2[:]),
                 "axial": np.array(e.axial[:]), "two_theta": np.array(e.two_theta[:]),
                 "cells": np.array(e.cells[:]), "trust": self.trust,
-                "feature_set": self.feature_set}
+                "feature_set": self.feature_set,
+                "bands": np.array(self.bands, dtype=np.float64).reshape(-1, 5),
+                "bounded": self.bounded}
 
     def load_state(self, st: dict) -> None:
         self.set_feature_set(int(st["feature_set"]))
@@ -329,3 +440,6 @@ class Surrogate:
         self.envelope = env
         self.trust = np.ascontiguousarray(st["trust"], dtype=np.uint8)
         self.mlp.set_envelope(env, self.trust)
+        self.bands = [(int(b[0]), int(b[1]), int(b[2]), float(b[3]), float(b[4]))
+                      for b in np.asarray(st.get("bands", np.zeros((0, 5)))).reshape(-1, 5)]
+        self.set_bounded(bool(st.get("bounded", False)))

@@ -33,7 +33,10 @@ _MASK = np.uint64(0xFFFFFFFFFFFFFFFF)
 
 def splitmix64(seed: int, stream: int, n: int) -> np.ndarray:
     """n uint64 values of the SplitMix64 output function applied to the
-    counters ``seed * 2**32 + stream * 2**48 + i``."""
+    states ``(seed + i) * G + stream * D`` (mod 2**64), i = 1 .. n, with G and
+    D the constants below: the seed and the counter share the increment G,
+    so seed s + k on a stream repeats seed s's values k counters later; sets
+    meant to be independent use different streams."""
     with np.errstate(over="ignore"):
         base = np.uint64((seed * 0x9E3779B97F4A7C15 + stream * 0xD1B54A32D192ED03) & 0xFFFFFFFFFFFFFFFF)
         z = base + np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
@@ -154,11 +157,15 @@ OMEGA_AXIS = np.array([-1.0, 0.0, 0.0])
 F_IDENTITY = np.eye(3).reshape(-1).astype(np.float64)
 
 
-def make_reflections(R: int, seed: int, two_theta_deg=(5.0, 60.0)) -> Tuple[np.ndarray, np.ndarray]:
+def make_reflections(R: int, seed: int, two_theta_deg=(5.0, 60.0),
+                     stream: int = 303) -> Tuple[np.ndarray, np.ndarray]:
     """R reflections: rotation angle omega ~ U[0, 2pi); diffracted unit vector
     s1 at 2theta ~ U[lo, hi] degrees from the beam with uniform azimuth.
-    Returns (s1 float64 [R,3] DIALS frame, omega float64 [R])."""
-    u = uniform01(seed, 303, 3 * R).reshape(R, 3)
+    Returns (s1 float64 [R,3] DIALS frame, omega float64 [R]).
+
+    Seeds that differ by k on one stream give the same values shifted by k
+    counters, so sets that must be independent take different streams."""
+    u = uniform01(seed, stream, 3 * R).reshape(R, 3)
     omega = 2.0 * np.pi * u[:, 0]
     lo, hi = np.deg2rad(two_theta_deg[0]), np.deg2rad(two_theta_deg[1])
     tth = lo + (hi - lo) * u[:, 1]
@@ -286,11 +293,12 @@ def surrogate_config(seed: int = 5, n_train: int = 200_000, n_heldout: int = 10_
     vol = make_volume(VolumeSpec(sp["shape"], sp["cyl_radius"], sp["slab"], sp["semi_axes"],
                                  sp["seed"]))
     coords = sample_stride(crystal_voxels(vol), sp["stride"])
+    # one stream per set: on one stream, nearby seeds only shift the values
     s1, om = make_reflections(n_train, seed * 1000 + 1)
-    hs1, hom = make_reflections(n_heldout, seed * 1000 + 2)
+    hs1, hom = make_reflections(n_heldout, seed * 1000 + 2, stream=311)
     n_out = int(round(n_infer * outside_fraction))
-    is1, iom = make_reflections(n_infer - n_out, seed * 1000 + 3)
-    os1, oom = make_reflections(n_out, seed * 1000 + 4, two_theta_deg=(60.0, 75.0))
+    is1, iom = make_reflections(n_infer - n_out, seed * 1000 + 3, stream=313)
+    os1, oom = make_reflections(n_out, seed * 1000 + 4, two_theta_deg=(60.0, 75.0), stream=317)
     outside = np.zeros(n_infer, dtype=bool)        # n_out seeded positions
     if n_out:
         outside[np.argpartition(splitmix64(seed, 505, n_infer), n_out - 1)[:n_out]] = True

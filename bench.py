@@ -56,6 +56,8 @@ TRAFFIC_BYTES_PER_STEP = {("plain-256^3-stride2", 512): 0.093,
 # profiles/ib_kernel_bisection-256cube.md.
 IB_TRAFFIC_BYTES_PER_READ = {"plain-256^3-stride2": 0.0024}
 METRIC = "absorption factors/sec (rays/s) at 1 GPU, plain vs surrogate at stated rel. error"
+SURROGATE_BOUND = 1e-3      # the stated relative error bound of the surrogate
+LARGE_REFLECTIONS = 10000   # the default run's 1000x1000x800 record: its first 10k reflections
 
 
 def cpu_cores() -> int:
@@ -195,7 +197,11 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
                   cpu_budget: float = 0.0):
     """Fit the surrogate, judge it on the held-out set, time 1M inferences
     (features, envelope test, MLP on MFMA, tracer fallback) and the plain GPU
-    path on the same sample.  Returns the surrogate record."""
+    path on the same sample.  Two operating points: bounded (the resonance
+    bands fitted on the training errors send their reflections to the
+    tracer, so the network answers only where it met the 1e-3 bound) and
+    fast (the network answers every reflection inside the training range).
+    Returns the surrogate record, bounded mode at its top level."""
     from anacor_public_amd import workload as W
     from anacor_public_amd.ray_tracing import Tracer
     from anacor_public_amd.surrogate import Surrogate
@@ -215,42 +221,64 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
         for _ in range(3):
             tr.run(ps1, pom, *geo)
         plain_rate = 3 * len(pom) / (time.perf_counter() - t1)
-        rep = sur.fit(wl.s1, wl.omega, epochs=epochs)
+        rep = sur.fit(wl.s1, wl.omega, epochs=epochs, bound=SURROGATE_BOUND)
         log(f"# surrogate fit: trace {rep['trace_s']:.2f}s, train {rep['train_s']:.1f}s "
-            f"({epochs} epochs), final loss {rep['final_loss']:.3e}")
+            f"({epochs} epochs), final loss {rep['final_loss']:.3e}, {len(rep['bands'])} "
+            f"resonance bands ({rep['band_train_fallback'] * 100:.1f}% of training reflections)")
         A_true = tr.run(sw.heldout_s1, sw.heldout_omega, *geo)
-        A_sur, used = sur.predict(sw.heldout_s1, sw.heldout_omega)
-        err = rel_stats(A_sur[used], A_true[used])
-        log(f"# held-out ({int(used.sum())}/{len(used)} answered by the network): {err}")
-        for _ in range(warmup):
-            sur.predict(sw.infer_s1, sw.infer_omega)
-        mlp_ms, fb = [], 0
-        t2 = time.perf_counter()
-        for _ in range(steps):
-            _, u = sur.predict(sw.infer_s1, sw.infer_omega)
-            mlp_ms.append(sur.mlp.stats()["mlp_ms"])
-            fb = int((~u).sum())
-        elapsed = time.perf_counter() - t2
-        value = n_infer * steps / elapsed
-        mms = float(np.mean(mlp_ms))
+
+        def mode(bounded: bool) -> dict:
+            sur.set_bounded(bounded)
+            A_sur, used = sur.predict(sw.heldout_s1, sw.heldout_omega)
+            err = rel_stats(A_sur[used], A_true[used])
+            for _ in range(warmup):
+                sur.predict(sw.infer_s1, sw.infer_omega)
+            mlp_ms, fb = [], 0
+            t2 = time.perf_counter()
+            for _ in range(steps):
+                _, u = sur.predict(sw.infer_s1, sw.infer_omega)
+                mlp_ms.append(sur.mlp.stats()["mlp_ms"])
+                fb = int((~u).sum())
+            elapsed = time.perf_counter() - t2
+            value = n_infer * steps / elapsed
+            out = {"value": value, "ms_per_step": 1e3 * elapsed / steps,
+                   "heldout_answered": int(used.sum()), "heldout_rel_error": err,
+                   "fallback_reflections": fb, "speedup_vs_plain_gpu": value / plain_rate,
+                   "mlp_ms": float(np.mean(mlp_ms))}
+            name = "bounded" if bounded else "fast"
+            log(f"# surrogate {name}: held-out ({out['heldout_answered']}/{len(used)} answered) "
+                f"{err}; {value:.3e} A_h/s ({out['speedup_vs_plain_gpu']:.1f}x the plain GPU "
+                f"path's {plain_rate:.3e}), fallback {fb}")
+            return out
+
+        fast = mode(False)
+        bounded = mode(True)
+        mms = bounded["mlp_ms"]
         achieved = n_infer * MLP_FLOPS_PER_ROW / (mms * 1e-3) / 1e12
         rec = {
-            "value": value, "unit": "absorption factors/s", "ms_per_step": 1e3 * elapsed / steps,
-            "dtype": "bf16", "workload": wl.name, "n_voxels": int(len(wl.coords)),
-            "inferences": n_infer, "fallback_reflections": fb,
+            "value": bounded["value"], "unit": "absorption factors/s",
+            "ms_per_step": bounded["ms_per_step"], "mode": "bounded",
+            "bound": SURROGATE_BOUND, "dtype": "bf16", "workload": wl.name,
+            "n_voxels": int(len(wl.coords)), "inferences": n_infer,
+            "fallback_reflections": bounded["fallback_reflections"],
             "training_pairs": int(len(wl.omega)), "epochs": epochs,
             "train_s": rep["train_s"], "train_trace_s": rep["trace_s"],
-            "heldout": int(len(sw.heldout_omega)), "heldout_answered": int(used.sum()),
-            "heldout_rel_error": err,
-            "plain_gpu_same_sample": plain_rate, "speedup_vs_plain_gpu": value / plain_rate,
+            "resonance_bands": rep["bands"],
+            "band_training_fallback": rep["band_train_fallback"],
+            "heldout": int(len(sw.heldout_omega)),
+            "heldout_answered": bounded["heldout_answered"],
+            "heldout_rel_error": bounded["heldout_rel_error"],
+            "plain_gpu_same_sample": plain_rate,
+            "speedup_vs_plain_gpu": bounded["speedup_vs_plain_gpu"],
             "mlp_only_rate": n_infer / (mms * 1e-3),
+            "fast": {k: fast[k] for k in ("value", "ms_per_step", "heldout_answered",
+                                          "heldout_rel_error", "fallback_reflections",
+                                          "speedup_vs_plain_gpu")},
             "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS,
                          "traffic": None, "kernel": "MLP forward (HIP events)",
                          "kernel_ms": mms, "flops_per_launch": n_infer * MLP_FLOPS_PER_ROW},
         }
-        log(f"# surrogate: {value:.3e} A_h/s ({rec['speedup_vs_plain_gpu']:.1f}x the plain GPU "
-            f"path's {plain_rate:.3e} on this sample), MLP {mms:.2f} ms per 1M, fallback {fb}")
         # the reference's own approximation on the same sample: gridding maps
         rec["gridding_baseline"] = gridding_baseline(tr, sw, geo, A_true, steps)
         if cpu_budget > 0:
@@ -352,6 +380,146 @@ def run_bisection(steps: int, warmup: int, cpu_budget: float, walk_value: float)
     return rec
 
 
+def traffic_for(name: str, layout: int, block, tile: int, group: int, vsteps: int):
+    """L2<->fabric bytes of the trace launch from the calibrated per-step
+    figure measured for this exact work shape (workload, layout, voxel
+    block, tile, reflection group), or None without one."""
+    key = (name, int(layout), tuple(int(b) for b in block), int(tile), int(group))
+    per = TRAFFIC_BYTES_PER_STEP.get(key)
+    return None if per is None else per * vsteps
+
+
+def measure_plain(config: str, steps: int, warmup: int, cpu_budget: float,
+                  reflections: int = 0, tile: int = 0, autotune_on: bool = True,
+                  gather: bool = True) -> dict:
+    """Time `steps` whole plain-path steps of one workload (volumes and voxel
+    lists resident): host setup, upload, trace, reduction and download of
+    every reflection's A_h.  Returns the record (value in A_h/s, roofline of
+    the trace kernel, CPU baseline on a bounded sample of the same
+    workload)."""
+    from anacor_public_amd import autotune
+    from anacor_public_amd import workload as W
+    from anacor_public_amd.ray_tracing import Batch, Tracer
+    t0 = time.perf_counter()
+    if config == "batched":
+        wls = W.batched_config()
+    else:
+        wls = [{"single": W.single_gpu_config, "small": W.small_config,
+                "large": W.large_config}[config]()]
+    if reflections:
+        wls = [w.subset(reflections) for w in wls]
+    R = sum(len(w.omega) for w in wls)
+    N = sum(len(w.coords) for w in wls)
+    name = wls[0].name if len(wls) == 1 else f"batched-{len(wls)}-samples"
+    log(f"# workload {name}: volumes {[w.volume.shape for w in wls]}, N={N}, R={R} "
+        f"(built in {time.perf_counter() - t0:.1f}s)")
+    extra = {}
+    if config == "batched":
+        runner = Batch(wls)
+        if autotune_on:
+            t1 = time.perf_counter()
+            picks = autotune.tune_batch(runner, wls, trial=256, use_cache=False)
+            extra["autotune"] = [{"block": p["block"], "tile": p["tile"]} for p in picks]
+            log(f"# autotune ({time.perf_counter() - t1:.1f}s): {extra['autotune']}")
+
+        def call():
+            return runner.run([w.s1 for w in wls], [w.omega for w in wls], wls[0].raw_xray,
+                              wls[0].omega_axis, wls[0].F)
+    else:
+        runner = Tracer.for_workload(wls[0])
+        if tile:
+            runner.set_tile(tile)
+        elif autotune_on:
+            w = wls[0]
+            t1 = time.perf_counter()
+            pick = autotune.tune(runner, w.s1, w.omega, w.raw_xray, w.omega_axis, w.F, w.coords,
+                                 trial=512, layouts=autotune.LAYOUTS, use_cache=False)
+            extra["autotune"] = {"layout": pick["layout"], "block": pick["block"],
+                                 "tile": pick["tile"]}
+            log(f"# autotune ({time.perf_counter() - t1:.1f}s): {extra['autotune']}")
+
+        def call():
+            return [runner.run_workload(wls[0])]
+
+    for _ in range(warmup):
+        call()
+    kernel_ms, vsteps_l, host_ms = [], [], []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        outs = call()
+        st = runner.stats()
+        kernel_ms.append(st["kernel_ms"])
+        vsteps_l.append(st["voxel_steps"])
+        host_ms.append(st["host_ms"])
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1e3 * elapsed / steps
+    value = R * steps / elapsed
+    k_ms = float(np.mean(kernel_ms))
+    vsteps = int(vsteps_l[-1])
+    achieved = vsteps * 1.0 / (k_ms * 1e-3) / 1e9   # 1 algorithmic byte per voxel step
+    extra["host_setup_ms"] = float(np.mean(host_ms))
+    log(f"# {name}: {steps} steps, {ms_per_step:.2f} ms/step, trace kernel {k_ms:.2f} ms, "
+        f"host setup {extra['host_setup_ms']:.2f} ms, {vsteps} voxel steps per launch")
+    for o in outs:
+        assert np.all(np.isfinite(o)) and np.all((o > 0) & (o <= 1))
+    if config == "large":
+        # the FP32 compensated mean against an FP64 GPU accumulation
+        f64 = runner.run_workload(wls[0], f64_mean=True)
+        extra["fp32_vs_fp64_max_rel"] = float(np.max(np.abs(outs[0] - f64) / f64))
+        log(f"# FP32 compensated vs FP64 mean: max rel {extra['fp32_vs_fp64_max_rel']:.2e}")
+    cpu = None
+    if cpu_budget > 0:
+        cpu, cpu_outs = cpu_baseline(wls, cpu_budget)
+        rel = max(float(np.max(np.abs(o[:len(c)] - c) / c)) for o, c in zip(outs, cpu_outs))
+        cpu["max_rel_diff_vs_gpu"] = rel
+        log(f"# cpu baseline ({cpu['kind']}, {cpu['cores']} cores): {cpu['value']:.2f} A_h/s, "
+            f"max rel diff vs GPU {rel:.2e}")
+    layout = 0 if config == "batched" else runner.layout
+    traffic, shape = None, None
+    if config != "batched":
+        shape = runner.work_shape()
+        key_name = name if not reflections else f"{name}-first{reflections}"
+        traffic = traffic_for(key_name, layout, shape["block"], shape["tile"], shape["group"],
+                              vsteps)
+        extra["work_shape"] = shape
+    pairs = sum(len(w.coords) * len(w.omega) for w in wls)
+    rec = {
+        "value": value,
+        "unit": "absorption factors/s",
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": ms_per_step,
+        "workload": name,
+        "volumes": [list(w.volume.shape) for w in wls],
+        "n_voxels": N,
+        "reflections": R,
+        "volume_layout": {0: "raw", 2: "brick2", 4: "brick4"}[layout],
+        "rays_per_s": 2.0 * pairs / (ms_per_step * 1e-3),
+        "voxel_steps_per_s": vsteps / (ms_per_step * 1e-3),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": ("trace_kernel, one launch per sample" if config == "batched"
+                                else "trace_kernel"),
+                     "kernel_ms": k_ms, "bytes_per_launch": vsteps},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    if config == "batched":
+        rec["graph"] = False
+        rec["launch"] = ("per-sample trace launches on one stream behind each sample's "
+                         "upload, host setup of the next sample overlapped")
+        rec["roofline"]["kernel_ms_covers"] = "first sample's trace to the finish kernel"
+    if config != "batched" and gather:
+        rec["roofline"]["gather_ceiling"] = gather_ceiling(
+            volume_bytes(wls[0].volume.shape, layout), achieved)
+        gc = rec["roofline"]["gather_ceiling"]
+        log(f"# gather ceiling at {gc['working_set_bytes']} B: lanes {gc['lanes_GBps']:.0f} GB/s, "
+            f"row {gc['row_GBps']:.0f} GB/s; trace kernel at {achieved:.0f} GB/s")
+    rec.update(extra)
+    runner.close()
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -363,6 +531,12 @@ def main():
                     help="skip the surrogate record of the default run")
     ap.add_argument("--no-bisection", action="store_true",
                     help="skip the iterative-bisection record of the default run")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched (eight samples) record of the default run")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 1000x1000x800 record of the default run")
+    ap.add_argument("--large-reflections", type=int, default=LARGE_REFLECTIONS,
+                    help="reflections of the default run's 1000x1000x800 record (its first R)")
     ap.add_argument("--epochs", type=int, default=600, help="surrogate training epochs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -380,12 +554,10 @@ def main():
               "reflections across GPUs by construction but this build runs and reports N=1.",
               file=sys.stderr)
         sys.exit(2)
-
-    from anacor_public_amd import workload as W
+    budget = 0.0 if args.no_cpu_baseline else args.cpu_budget
 
     if args.config == "surrogate":
-        rec = run_surrogate(args.steps, args.warmup, epochs=args.epochs,
-                            cpu_budget=0.0 if args.no_cpu_baseline else args.cpu_budget)
+        rec = run_surrogate(args.steps, args.warmup, epochs=args.epochs, cpu_budget=budget)
         cpu = rec.pop("cpu_baseline", None)
         out = {"metric": METRIC, "value": rec["value"], "unit": rec["unit"], "n_gpus": 1,
                "steps": args.steps, "warmup": args.warmup, "ms_per_step": rec["ms_per_step"],
@@ -401,140 +573,45 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
-    t0 = time.perf_counter()
-    if args.config == "batched":
-        wls = W.batched_config()
-    else:
-        wls = [{"single": W.single_gpu_config, "small": W.small_config,
-                "large": W.large_config}[args.config]()]
-    if args.reflections:
-        wls = [w.subset(args.reflections) for w in wls]
-    R = sum(len(w.omega) for w in wls)
-    N = sum(len(w.coords) for w in wls)
-    name = wls[0].name if len(wls) == 1 else f"batched-{len(wls)}-samples"
-    log(f"# workload {name}: volumes {[w.volume.shape for w in wls]}, N={N}, R={R} "
-        f"(built in {time.perf_counter() - t0:.1f}s)")
-
-    extra = {}
-    if args.config == "batched":
-        from anacor_public_amd import autotune
-        from anacor_public_amd.ray_tracing import Batch
-        runner = Batch(wls)
-        if not args.no_autotune:
-            t1 = time.perf_counter()
-            picks = autotune.tune_batch(runner, wls, trial=256, use_cache=False)
-            extra["autotune"] = [{"block": p["block"], "tile": p["tile"]} for p in picks]
-            log(f"# autotune ({time.perf_counter() - t1:.1f}s): {extra['autotune']}")
-
-        def call():
-            return runner.run([w.s1 for w in wls], [w.omega for w in wls], wls[0].raw_xray,
-                              wls[0].omega_axis, wls[0].F)
-    else:
-        from anacor_public_amd import autotune
-        from anacor_public_amd.ray_tracing import Tracer
-        runner = Tracer.for_workload(wls[0])
-        if args.tile:
-            runner.set_tile(args.tile)
-        elif not args.no_autotune:
-            w = wls[0]
-            t1 = time.perf_counter()
-            pick = autotune.tune(runner, w.s1, w.omega, w.raw_xray, w.omega_axis, w.F, w.coords,
-                                 trial=128 if args.config == "large" else 512,
-                                 layouts=autotune.LAYOUTS, use_cache=False)
-            extra["autotune"] = {"layout": pick["layout"], "block": pick["block"],
-                                 "tile": pick["tile"]}
-            log(f"# autotune ({time.perf_counter() - t1:.1f}s): {extra['autotune']}")
-
-        def call():
-            return [runner.run_workload(wls[0])]
-
-    for _ in range(args.warmup):
-        call()
-    kernel_ms, steps, host_ms = [], [], []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        outs = call()
-        st = runner.stats()
-        kernel_ms.append(st["kernel_ms"])
-        steps.append(st["voxel_steps"])
-        host_ms.append(st["host_ms"])
-    elapsed = time.perf_counter() - t_start
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = R * args.steps / elapsed
-    k_ms = float(np.mean(kernel_ms))
-    vsteps = int(steps[-1])
-    achieved = vsteps * 1.0 / (k_ms * 1e-3) / 1e9   # 1 algorithmic byte per voxel step
-    extra["host_setup_ms"] = float(np.mean(host_ms))
-    log(f"# {args.steps} steps: {ms_per_step:.2f} ms/step, trace kernel {k_ms:.2f} ms, "
-        f"host setup {extra['host_setup_ms']:.2f} ms, {vsteps} voxel steps per launch")
-    for o in outs:
-        assert np.all(np.isfinite(o)) and np.all((o > 0) & (o <= 1))
-
-    if args.config == "large":
-        # the FP32 compensated mean against an FP64 GPU accumulation
-        f64 = runner.run_workload(wls[0], f64_mean=True)
-        extra["fp32_vs_fp64_max_rel"] = float(np.max(np.abs(outs[0] - f64) / f64))
-        log(f"# FP32 compensated vs FP64 mean: max rel {extra['fp32_vs_fp64_max_rel']:.2e}")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu, cpu_outs = cpu_baseline(wls, args.cpu_budget)
-        rel = max(float(np.max(np.abs(o[:len(c)] - c) / c)) for o, c in zip(outs, cpu_outs))
-        cpu["max_rel_diff_vs_gpu"] = rel
-        log(f"# cpu baseline ({cpu['kind']}, {cpu['cores']} cores): {cpu['value']:.2f} A_h/s, "
-            f"max rel diff vs GPU {rel:.2e}")
-
-    traffic = None
-    # the calibrated traffic per step was measured with the raw layout
-    tile = args.tile
-    if not tile and isinstance(extra.get("autotune"), dict):
-        tile = extra["autotune"]["tile"]
-    if ((name, tile) in TRAFFIC_BYTES_PER_STEP and not args.reflections
-            and args.config != "batched" and runner.layout == 0):
-        traffic = TRAFFIC_BYTES_PER_STEP[(name, tile)] * vsteps
-    pairs = sum(len(w.coords) * len(w.omega) for w in wls)
-    rec = {
+    rec = measure_plain(args.config, args.steps, args.warmup, budget,
+                        reflections=args.reflections, tile=args.tile,
+                        autotune_on=not args.no_autotune)
+    head = {
         "metric": METRIC,
-        "value": value,
-        "unit": "absorption factors/s",
+        "value": rec.pop("value"),
+        "unit": rec.pop("unit"),
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "steps": rec.pop("steps"),
+        "warmup": rec.pop("warmup"),
+        "ms_per_step": rec.pop("ms_per_step"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded four-material volumes and reflections)",
-        "config": {"workload": name, "volumes": [list(w.volume.shape) for w in wls],
-                   "n_voxels": N, "reflections": R, "path": "plain",
-                   "mean": "fp32 exp + compensated sum",
-                   "volume_layout": ("raw" if args.config == "batched" else
-                                     {0: "raw", 2: "brick2", 4: "brick4"}[runner.layout])},
-        "rays_per_s": 2.0 * pairs / (ms_per_step * 1e-3),
-        "voxel_steps_per_s": vsteps / (ms_per_step * 1e-3),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "trace_kernel per sample" if args.config == "batched"
-                     else "trace_kernel",
-                     "kernel_ms": k_ms, "bytes_per_launch": vsteps},
-        "cpu_baseline": cpu,
-        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "config": {"workload": rec.pop("workload"), "volumes": rec.pop("volumes"),
+                   "n_voxels": rec.pop("n_voxels"), "reflections": rec.pop("reflections"),
+                   "path": "plain", "mean": "fp32 exp + compensated sum",
+                   "volume_layout": rec.pop("volume_layout")},
+        "roofline": rec.pop("roofline"),
+        "cpu_baseline": rec.pop("cpu_baseline"),
     }
-    if args.config != "batched":
-        rec["roofline"]["gather_ceiling"] = gather_ceiling(
-            volume_bytes(wls[0].volume.shape, runner.layout), achieved)
-        gc = rec["roofline"]["gather_ceiling"]
-        log(f"# gather ceiling at {gc['working_set_bytes']} B: lanes {gc['lanes_GBps']:.0f} GB/s, "
-            f"row {gc['row_GBps']:.0f} GB/s; trace kernel at {achieved:.0f} GB/s")
-    rec.update(extra)
-    runner.close()
-    if args.config == "single" and not args.no_bisection and not args.reflections:
-        rec["bisection"] = run_bisection(args.steps, args.warmup,
-                                         0.0 if args.no_cpu_baseline else 8.0, value)
-    if args.config == "single" and not args.no_surrogate and not args.reflections:
-        rec["surrogate"] = run_surrogate(args.steps, args.warmup, epochs=args.epochs)
-    print(json.dumps(rec), flush=True)
+    head.update(rec)
+    default_run = args.config == "single" and not args.reflections
+    if default_run and not args.no_bisection:
+        head["bisection"] = run_bisection(args.steps, args.warmup, min(budget, 8.0),
+                                          head["value"])
+    if default_run and not args.no_surrogate:
+        head["surrogate"] = run_surrogate(args.steps, args.warmup, epochs=args.epochs)
+    if default_run and not args.no_batched:
+        head["batched"] = measure_plain("batched", min(args.steps, 10), min(args.warmup, 2),
+                                        min(budget, 10.0))
+    if default_run and not args.no_large:
+        head["large"] = measure_plain("large", min(args.steps, 3), min(args.warmup, 1),
+                                      min(budget, 6.0), reflections=args.large_reflections,
+                                      autotune_on=False, gather=False)
+        head["large"]["reflections_of_config"] = 100000
+    print(json.dumps(head), flush=True)
 
 
 if __name__ == "__main__":

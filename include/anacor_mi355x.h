@@ -107,6 +107,22 @@ void anacor_ctx_destroy(anacor_ctx *ctx);
 /* Voxels per workgroup tile (0 = automatic). */
 int anacor_ctx_set_tile(anacor_ctx *ctx, int voxels_per_tile);
 
+/* Launch order of the trace kernel (speed only, never results): reflections
+ * are sorted by the directions of their rays, led by the incident ray's
+ * (direction_major 0) or the diffracted ray's (1), and launched in groups of
+ * refl_group consecutive reflections, tile by tile within a group, so that
+ * the workgroups resident on an XCD share the bricks their rays cross
+ * (0: automatic, 128 in the bricked layouts and none in the raw one; 1: no
+ * groups, every tile of a reflection in turn). */
+int anacor_ctx_set_order(anacor_ctx *ctx, int refl_group, int direction_major);
+
+/* The work shape in use: voxel block (bz, by, bx; 0, 0, 0 = Morton), tile
+ * setting (0 = automatic), reflections per launch group for the current
+ * layout (1 = none) and the leading ray of the direction sort.  Any output
+ * may be NULL. */
+int anacor_ctx_get_work_shape(const anacor_ctx *ctx, int block[3], int *tile, int *refl_group,
+                              int *direction_major);
+
 /* Volume layout of the trace kernel.  Never changes a result: every layout
  * yields the same labels along every ray (bit-identical results).  Bricks are
  * 8x8x8 voxels; 4-bit bricks store labels 0..4 as they are and any other value
@@ -338,6 +354,23 @@ int anacor_mlp_set_features(anacor_mlp *mlp, int feature_set);
  * index (ia * cells[1] + it) * cells[2] + ib; NULL = every cell trusted).
  * env NULL removes the envelope. */
 int anacor_mlp_set_envelope(anacor_mlp *mlp, const anacor_envelope_t *env, const uint8_t *trust);
+
+/* Resonance bands: near a small rational ratio of two components of a ray's
+ * crystal-frame direction, the voxel steps of all crystal voxels fall into
+ * step together and the traced absorption factor oscillates (by up to a few
+ * 1e-3 relative, over a few 0.01 rad) faster than 200k training reflections
+ * sample it.  A reflection whose ray `ray` (0 incident, 1 diffracted) has
+ * |min(|v_i|, |v_j|) / max(|v_i|, |v_j|) - ratio| < width for some band lies
+ * outside the envelope and goes to the tracer.  n = 0 removes them. */
+typedef struct {
+    int32_t ray;
+    int32_t i, j;             /* components (0, 1, 2) of the unit vector, i != j */
+    int32_t pad;
+    double  ratio;            /* in [0, 1] */
+    double  width;
+} anacor_band_t;
+#define ANACOR_MLP_MAX_BANDS 32
+int anacor_mlp_set_bands(anacor_mlp *mlp, const anacor_band_t *bands, int n);
 
 /* Network inputs of R reflections (features[R][12]) and invariants[R][4] =
  * (axial, two_theta, alpha, beta); with an envelope set also the cell index

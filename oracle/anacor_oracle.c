@@ -11,7 +11,8 @@
  * baseline ISA, which has no FMA instruction to contract into).
  *
  * Arithmetic details that matter for bit-exactness, all checked against the
- * reference build (oracle/_ref) by tests/test_oracle_reference.py:
+ * reference build (oracle/_ref) through the golden fixtures it writes
+ * (tests/golden/make_golden.py), compared in tests/test_oracle_golden.py:
  *   - pow(v, 2) in the reference is folded to v*v by gcc at -O3; glibc pow(v, 2)
  *     differs from v*v in ~0.1% of inputs, so the restatement writes v*v.
  *   - double -> int64 conversion of an out-of-range or NaN value yields
@@ -856,4 +857,78 @@ int64_t oracle_ib_overall(const int64_t *coords, int64_t n_coords, const double 
         diag->reads = reads;
     }
     return failed;
+}
+
+/* ------------------------------------------- the device's quotient sequence */
+
+/* The GPU kernels compute RN(a / b) from a per-ray reciprocal y = RN(1 / b)
+ * by two fused corrections of q0 = RN(a * y) (quot_rn in csrc/trace.hip).
+ * This replays that sequence with C99 fma() (correctly rounded, as the
+ * device's v_fma_f64) against IEEE division on n seeded cases of each kind:
+ *   kind 0: a an integer with |a| < 2^21, b a double of any sign with
+ *           exponent in [-900, 900] (cube_face's (bound - origin) / dir);
+ *   kind 1: b an integer in [1, 2^21), a an integer in [0, b + 1]
+ *           (cal_path2_plus' span / (len - 1)).
+ * Significands are drawn near 1 and near 2 as well as uniformly, where the
+ * error of q0 is largest.  Returns the number of mismatches of the
+ * two-correction sequence; *one_step receives those of a single correction
+ * (which Markstein's theorem does not cover, since q0 need not be faithful). */
+static uint64_t qc_next(uint64_t *s)
+{
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static double qc_significand(uint64_t *s)   /* in [1, 2) */
+{
+    const uint64_t r = qc_next(s);
+    uint64_t frac = r & ((1ull << 52) - 1);
+    switch ((r >> 60) & 3) {
+    case 0: frac &= 0xFFFFull; break;                          /* just above 1 */
+    case 1: frac = ((1ull << 52) - 1) - (frac & 0xFFFFull); break; /* just below 2 */
+    default: break;
+    }
+    double m;
+    const uint64_t bits = (1023ull << 52) | frac;
+    memcpy(&m, &bits, sizeof m);
+    return m;
+}
+
+int64_t oracle_quot_rn_check(int kind, int64_t n, uint64_t seed, int64_t *one_step)
+{
+    uint64_t s = seed;
+    int64_t bad = 0, bad1 = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        double a, b;
+        if (kind == 0) {
+            const int e = (int)(qc_next(&s) % 1801) - 900;
+            b = ldexp(qc_significand(&s), e);
+            if (qc_next(&s) & 1) b = -b;
+            /* |a| < 2^21: powers of two and their neighbours often */
+            int64_t k;
+            const uint64_t r = qc_next(&s);
+            if ((r & 3) == 0) k = ((int64_t)1 << (r >> 2) % 21) - (int64_t)((r >> 8) % 3);
+            else k = (int64_t)((r >> 2) % ((1u << 21) - 1));
+            a = (double)((r >> 40) & 1 ? -k : k);
+        } else {
+            const uint64_t r = qc_next(&s);
+            int64_t D = (r & 1) ? (int64_t)((r >> 1) % ((1u << 21) - 1)) + 1
+                                : ((int64_t)1 << (r >> 1) % 21) + (int64_t)((r >> 8) % 3) - 1;
+            if (D < 1) D = 1;
+            const int64_t span = (int64_t)(qc_next(&s) % (uint64_t)(D + 2));
+            a = (double)span;
+            b = (double)D;
+        }
+        const double y = 1.0 / b;
+        const double q0 = a * y;
+        const double q1 = fma(fma(-q0, b, a), y, q0);
+        const double q2 = fma(fma(-q1, b, a), y, q1);
+        const double ref = a / b;
+        if (memcmp(&q2, &ref, sizeof ref) != 0) ++bad;
+        if (memcmp(&q1, &ref, sizeof ref) != 0) ++bad1;
+    }
+    if (one_step) *one_step = bad1;
+    return bad;
 }

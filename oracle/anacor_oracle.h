@@ -126,6 +126,12 @@ int64_t oracle_ib_overall(const int64_t *coords, int64_t n_coords, const double 
                           const int64_t shape[3], int num_cls, int is_exp, int n_threads,
                           double *result, oracle_ib_diag_t *diag);
 
+/* The GPU's quotient sequence (two fused corrections of RN(a * RN(1 / b)))
+ * against IEEE division on n seeded cases of `kind` (0: integer / double,
+ * 1: integer span / integer D); returns the mismatches, *one_step those of a
+ * single correction. */
+int64_t oracle_quot_rn_check(int kind, int64_t n, uint64_t seed, int64_t *one_step);
+
 #ifdef __cplusplus
 }
 #endif

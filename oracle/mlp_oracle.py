@@ -196,6 +196,25 @@ def invariants(s1, omega, raw_xray, omega_axis, F, axis, e1, e2):
     return np.stack([x @ u, tth, alpha, beta], axis=1)
 
 
+def band_mask(x, s, bands):
+    """True for the reflections inside some resonance band (ray, i, j, ratio,
+    width): the band's ray (0 incident x, 1 diffracted s) has
+    |min(|v_i|, |v_j|) / max(|v_i|, |v_j|) - ratio| < width (0 / 0 counts as
+    ratio 0)."""
+    n = len(x)
+    out = np.zeros(n, dtype=bool)
+    for k in range(n):
+        for ray, i, j, ratio, width in bands:
+            v = s[k] if ray else x[k]
+            a, b = abs(v[int(i)]), abs(v[int(j)])
+            hi, lo = max(a, b), min(a, b)
+            t = lo / hi if hi > 0 else 0.0
+            if abs(t - ratio) < width:
+                out[k] = True
+                break
+    return out
+
+
 def cell_index(inv, two_theta_range, cells):
     """Trust-grid cell of each reflection: alpha and beta over [-pi, pi],
     two_theta over its range, bins clamped to the grid."""

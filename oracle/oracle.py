@@ -17,7 +17,9 @@ import sys
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle_anacor.so")
+# ANACOR_ORACLE_LIB selects another build of the restatement (e.g. the
+# sanitizer build of `make -C oracle sanitize`)
+ORACLE_SO = os.environ.get("ANACOR_ORACLE_LIB", os.path.join(HERE, "liboracle_anacor.so"))
 REF_SO = os.path.join(HERE, "_ref", "ray_tracing_cpu.so")
 REF_SRC = "/root/reference/AnACor/src"
 
@@ -115,7 +117,18 @@ class Oracle:
                                           ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                           ct.c_int, ct.c_int, ct.c_int, ct.c_void_p,
                                           ct.POINTER(IbDiag)]
+        lib.oracle_quot_rn_check.restype = ct.c_int64
+        lib.oracle_quot_rn_check.argtypes = [ct.c_int, ct.c_int64, ct.c_uint64,
+                                             ct.POINTER(ct.c_int64)]
         self.lib = lib
+
+    def quot_rn_check(self, kind: int, n: int, seed: int):
+        """(mismatches of two fused corrections, of one) of the GPU's quotient
+        sequence against IEEE division on n seeded cases (kind 0: integer /
+        double, 1: integer span / integer D)."""
+        one = ct.c_int64(0)
+        bad = self.lib.oracle_quot_rn_check(int(kind), int(n), int(seed), ct.byref(one))
+        return int(bad), int(one.value)
 
     def ib_ratios(self, face, theta, phi):
         """increments (bisection.c:21-148): (rz, ry, rx) or None where the

@@ -11,7 +11,9 @@ reference build:
   the reference's verification compares with, verification/main_verification.py:92-106),
   at percent level as voxelisation allows;
 * the rounding identity the GPU kernels use (round-half-away == trunc(w + 0.5 - 2^-54)
-  after the clamp), checked over adversarial doubles.
+  after the clamp), checked over adversarial doubles;
+* the GPU's quotient sequence (RN(a / b) from RN(1 / b) by two fused corrections,
+  correctly rounded by Markstein's theorem) against IEEE division.
 """
 import math
 
@@ -225,3 +227,14 @@ def test_biased_trunc_rounding_identity():
     # the naive bias 0.5 fails on exactly one double
     bad = np.nextafter(0.5, 0)
     assert sat_trunc(np.array([bad + 0.5]))[0] == 1 and _round_half_away(bad) == 0
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_quotient_sequence_is_correctly_rounded(oracle, kind):
+    """cube_face's (bound - origin) / dir (kind 0) and cal_path2_plus'
+    span / (len - 1) (kind 1) are computed on the GPU as
+    q1 = fma(fma(-q0, b, a), y, q0), q2 = fma(fma(-q1, b, a), y, q1) with
+    q0 = RN(a * y), y = RN(1 / b); q2 must equal a / b bit for bit.  Seeded
+    cases with significands near 1 and 2, where q0's error is largest."""
+    bad, _ = oracle.quot_rn_check(kind, 4_000_000, 20240 + kind)
+    assert bad == 0

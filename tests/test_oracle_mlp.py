@@ -130,3 +130,28 @@ def test_chamber_signs():
     s = np.array([[0.6, -0.48, 0.64], [-0.36, 0.48, -0.8]])
     np.testing.assert_array_equal(MO.chamber(x, s),
                                   [[1, -1, -1, 1, 1, -1], [-1, 1, -1, -1, -1, -1]])
+
+
+def test_band_mask_closed_forms():
+    """Directions built with a known component ratio fall inside a band of
+    that ratio exactly when the width exceeds their offset; the other ray and
+    the other component pairs are left alone."""
+    def unit(v):
+        v = np.asarray(v, dtype=np.float64)
+        return v / np.linalg.norm(v)
+    # s with |s0| / |s1| = 0.5 + d for a few offsets d (s2 fixed)
+    d = np.array([-0.02, -0.005, 0.0, 0.004, 0.03])
+    s = np.array([unit([0.5 + e, -1.0, 0.3]) for e in d])
+    # x along the second axis: ratio 0 for the pairs (0, 1) and (1, 2), 0 / 0 for (0, 2)
+    x = np.tile(unit([0.0, 1.0, 0.0]), (len(d), 1))
+    got = MO.band_mask(x, s, [(1, 0, 1, 0.5, 0.01)])
+    np.testing.assert_array_equal(got, np.abs(d) < 0.01)
+    # ratio taken as min / max: |s1| / |s0| = 2 is the same band as 1/2
+    s_swapped = s[:, [1, 0, 2]]
+    np.testing.assert_array_equal(MO.band_mask(x, s_swapped, [(1, 0, 1, 0.5, 0.01)]),
+                                  np.abs(d) < 0.01)
+    # a band on the incident ray does not look at s, and vice versa
+    assert not MO.band_mask(x, s, [(0, 0, 1, 0.5, 0.1)]).any()
+    assert MO.band_mask(x, s, [(0, 0, 1, 0.0, 1e-9)]).all()      # x0 = 0: ratio 0 exactly
+    assert MO.band_mask(x, s, [(0, 0, 2, 0.0, 1e-9)]).all()      # 0 / 0 counts as ratio 0
+    assert not MO.band_mask(x, s, [(1, 0, 1, 0.5, 0.0)]).any()   # zero width: nothing
