@@ -31,6 +31,9 @@ LAYOUT_BRICK2 = 2
 LAYOUT_BRICK4 = 4
 RAY_ERR_FACE = 1
 RAY_ERR_ASSERT = 2
+NOALLOC_MAX_REFLECTIONS = 16384
+NOALLOC_MAX_PARTIALS = 1 << 20
+NOALLOC_SCRATCH_BYTES = 17039616
 GATHER_LANES = 0
 GATHER_ROW = 1
 
@@ -41,7 +44,8 @@ EXPORTED = (
     "anacor_ctx_destroy", "anacor_ctx_set_tile", "anacor_ctx_set_order",
     "anacor_ctx_get_work_shape",
     "anacor_ctx_set_voxel_block",
-    "anacor_ctx_set_layout", "anacor_ctx_get_layout", "anacor_ctx_run", "anacor_ctx_paths",
+    "anacor_ctx_set_layout", "anacor_ctx_get_layout", "anacor_ctx_run",
+    "anacor_ctx_run_noalloc", "anacor_ctx_paths",
     "anacor_ctx_stats", "anacor_reflection_geometry", "anacor_batch_create",
     "anacor_batch_destroy", "anacor_batch_sample", "anacor_batch_run", "anacor_batch_stats",
     "anacor_mlp_create", "anacor_mlp_destroy", "anacor_mlp_num_params", "anacor_mlp_get_params",
@@ -132,6 +136,8 @@ def _prototypes(lib):
     lib.anacor_ctx_set_voxel_block.argtypes = [vp, i32, i32, i32]
     lib.anacor_ctx_run.restype = ct.c_int
     lib.anacor_ctx_run.argtypes = [vp, vp, vp, i64, vp, vp, vp, i32, vp]
+    lib.anacor_ctx_run_noalloc.restype = ct.c_int
+    lib.anacor_ctx_run_noalloc.argtypes = [vp, vp, vp, i64, vp, vp, vp, i32, vp]
     lib.anacor_ctx_paths.restype = ct.c_int
     lib.anacor_ctx_paths.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp]
     lib.anacor_ctx_stats.restype = ct.c_int

@@ -1581,8 +1581,9 @@ static int launch_group(const Context* c, int layout) {
   return (layout == kLayoutBrick2 || layout == kLayoutBrick4) ? kBrickGroup : 1;
 }
 
-// Device buffers of one trace call (the context's grow-only ones,
-// ensure_run_buffers).
+// Device buffers of one trace call: the context's grow-only ones
+// (ensure_run_buffers) or, for anacor_ctx_run_noalloc, parts of the static
+// scratch array.
 struct RunBuffers {
   ReflRule* rules = nullptr;           // [R]
   int* order = nullptr;                // [R] processing position -> caller's index
@@ -1957,6 +1958,112 @@ static int ctx_run(Context* c, const double* s1, const double* omega, int64_t R,
              host_threads(n_threads));
   c->last_host_ms = ms_since(t0);
   return run_rules(c, c->h_rules.data(), R, flags, out, paths);
+}
+
+// ------------------------------------------------- allocation-free trace
+//
+// anacor_ctx_run_noalloc traces on a resident context with every per-call
+// buffer in one file-scope device array, for callers that may not allocate
+// (graph capture, serving loops).  Each process holds one copy of the array
+// per GPU for its lifetime; its address on a device comes from
+// hipGetSymbolAddress on that device's first call (host code must never pass
+// the array's own name to a kernel: that is a host-side stand-in).  The host
+// staging of the rules and the processing order lives in static host arrays.
+// One process-wide lock serialises the calls; two processes each have their
+// own arrays.  The error words and the step counter are cleared by a kernel
+// at the start of every call; everything else is written before it is read.
+constexpr int64_t kNoAllocMaxR = ANACOR_NOALLOC_MAX_REFLECTIONS;
+constexpr int64_t kNoAllocMaxPartials = ANACOR_NOALLOC_MAX_PARTIALS;
+constexpr size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+constexpr size_t kScratchRules = 0;
+constexpr size_t kScratchPartial = kScratchRules + align256(kNoAllocMaxR * sizeof(ReflRule));
+constexpr size_t kScratchOrder = kScratchPartial + align256(kNoAllocMaxPartials * 8);
+constexpr size_t kScratchErr = kScratchOrder + align256(kNoAllocMaxR * sizeof(int));
+constexpr size_t kScratchOut = kScratchErr + align256(kNoAllocMaxR * sizeof(unsigned));
+constexpr size_t kScratchSteps = kScratchOut + align256(kNoAllocMaxR * sizeof(double));
+constexpr size_t kScratchBytes = kScratchSteps + 256;
+static_assert(kScratchBytes == ANACOR_NOALLOC_SCRATCH_BYTES, "scratch size in the header");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static unsigned char* g_scratch_dev[kMaxDevices];   // device address of `scratch` per device
+static std::mutex g_noalloc_mu;
+static ReflRule g_noalloc_rules[kNoAllocMaxR];
+static int g_noalloc_order[kNoAllocMaxR];
+static uint64_t g_noalloc_key[kNoAllocMaxR];
+static unsigned g_noalloc_err[kNoAllocMaxR];
+
+// Device address of `scratch` on the current device (set by the caller),
+// looked up once per device.  Called with g_noalloc_mu held.
+static int scratch_address(int device, unsigned char** out) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error(ANACOR_ERR_ARG, "device %d beyond the %d the scratch table covers", device,
+              kMaxDevices);
+    return ANACOR_ERR_ARG;
+  }
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_error(ANACOR_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+      return ANACOR_ERR_HIP;
+    }
+    g_scratch_dev[device] = (unsigned char*)p;
+  }
+  *out = g_scratch_dev[device];
+  return ANACOR_OK;
+}
+
+static int ctx_run_noalloc(Context* c, const double* s1, const double* omega, int64_t R,
+                           const double* raw_xray, const double* omega_axis, const double* F,
+                           int flags, double* out) {
+  if (!c) {
+    set_error(ANACOR_ERR_ARG, "null context");
+    return ANACOR_ERR_ARG;
+  }
+  if (R < 0 || (R > 0 && (!s1 || !omega || !raw_xray || !omega_axis || !F || !out))) {
+    set_error(ANACOR_ERR_ARG, "bad reflection arguments");
+    return ANACOR_ERR_ARG;
+  }
+  if (R > kNoAllocMaxR) {
+    set_error(ANACOR_ERR_ARG, "%lld reflections exceed the %lld of one allocation-free call "
+              "(ANACOR_NOALLOC_MAX_REFLECTIONS)", (long long)R, (long long)kNoAllocMaxR);
+    return ANACOR_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk_all(g_noalloc_mu);
+  std::lock_guard<std::mutex> lk(c->mu_run);
+  HIP_TRY(hipSetDevice(c->device));
+  unsigned char* base = nullptr;
+  int rc = scratch_address(c->device, &base);
+  if (rc) return rc;
+  c->last_oob = 0;
+  if (R == 0) return ANACOR_OK;
+  if (c->N == 0) {
+    for (int64_t i = 0; i < R; ++i) out[i] = __builtin_nan("");
+    return ANACOR_OK;
+  }
+  // the tile of anacor_ctx_run, enlarged until R * tiles fits the partials
+  int64_t vpt = choose_tile(c, R);
+  const int64_t max_tiles = kNoAllocMaxPartials / R;
+  if ((c->N + vpt - 1) / vpt > max_tiles) vpt = (c->N + max_tiles - 1) / max_tiles;
+  rc = ensure_layout(c, /*may_build=*/false);
+  if (rc) return rc;
+  const auto t0 = std::chrono::steady_clock::now();
+  // one host thread: the setup spawns none
+  host_setup(s1, omega, R, raw_xray, omega_axis, F, max_dim(c), g_noalloc_rules, nullptr, 1);
+  direction_order(R, [](int64_t i, int k) { return g_noalloc_rules[i].ray[k].dir; },
+                  g_noalloc_order, g_noalloc_key, c->sort_major);
+  c->last_host_ms = ms_since(t0);
+  RunBuffers B;
+  B.rules = (ReflRule*)(base + kScratchRules);
+  B.partial = base + kScratchPartial;
+  B.order = (int*)(base + kScratchOrder);
+  B.err = (unsigned*)(base + kScratchErr);
+  B.out = (double*)(base + kScratchOut);
+  B.steps = (unsigned long long*)(base + kScratchSteps);
+  return trace_call(c, B, g_noalloc_rules, g_noalloc_order, R, (int)vpt, c->layout, flags, out,
+                    nullptr, nullptr, g_noalloc_err);
 }
 
 // Build a contiguous copy of an int8*** row-pointer table (python_2_c_3d,
@@ -3018,6 +3125,17 @@ int anacor_ctx_run(anacor_ctx* ctx, const double* s1, const double* omega, int64
     return ANACOR_ERR_ARG;
   }
   return ctx_run(ctx->c, s1, omega, R, raw_xray, omega_axis, F, flags, 0, out, nullptr);
+}
+
+int anacor_ctx_run_noalloc(anacor_ctx* ctx, const double* s1, const double* omega, int64_t R,
+                           const double* raw_xray, const double* omega_axis, const double* F,
+                           int flags, double* out) {
+  clear_error();
+  if (!ctx) {
+    set_error(ANACOR_ERR_ARG, "null context");
+    return ANACOR_ERR_ARG;
+  }
+  return ctx_run_noalloc(ctx->c, s1, omega, R, raw_xray, omega_axis, F, flags, out);
 }
 
 int anacor_ctx_paths(anacor_ctx* ctx, const double* s1, const double* omega, int64_t R,

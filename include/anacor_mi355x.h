@@ -150,6 +150,28 @@ int anacor_ctx_run(anacor_ctx *ctx, const double *s1, const double *omega, int64
                    const double raw_xray[3], const double omega_axis[3], const double F[9],
                    int flags, double *out);
 
+/* The same as anacor_ctx_run, allocating nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * per-call device buffer (rules, processing order, tile partials, error words,
+ * means, step counter) lives in one static device array of
+ * ANACOR_NOALLOC_SCRATCH_BYTES that each process holds per GPU until it
+ * exits; host staging is static too and the host setup runs on the calling
+ * thread.  At most ANACOR_NOALLOC_MAX_REFLECTIONS reflections per call
+ * (ANACOR_ERR_ARG beyond); the voxel tile is enlarged so that reflections x
+ * tiles fit ANACOR_NOALLOC_MAX_PARTIALS.  The context must already hold the
+ * volume layout the trace uses (a bricked copy is built by
+ * anacor_ctx_set_layout or an earlier anacor_ctx_run; ANACOR_ERR_ARG
+ * otherwise).  Calls in one process are serialised by a lock and every call
+ * rewrites the array (its error words and step counter are cleared first);
+ * any other use of the array by overlapping work on the same GPU would race. */
+#define ANACOR_NOALLOC_MAX_REFLECTIONS 16384
+#define ANACOR_NOALLOC_MAX_PARTIALS    (1 << 20)
+#define ANACOR_NOALLOC_SCRATCH_BYTES   17039616   /* 16.25 MiB */
+int anacor_ctx_run_noalloc(anacor_ctx *ctx, const double *s1, const double *omega, int64_t R,
+                           const double raw_xray[3], const double omega_axis[3],
+                           const double F[9], int flags, double *out);
+
 /* Per (reflection, voxel) path record, both rays ([0] incident, [1]
  * diffracted): exit face, len_path_2, boundary count, last path coordinate
  * (z, y, x) and the four lengths [li, lo, cr, bu] of cal_path2_plus. */
