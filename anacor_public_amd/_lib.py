@@ -32,8 +32,8 @@ LAYOUT_BRICK4 = 4
 RAY_ERR_FACE = 1
 RAY_ERR_ASSERT = 2
 NOALLOC_MAX_REFLECTIONS = 16384
-NOALLOC_MAX_PARTIALS = 1 << 20
-NOALLOC_SCRATCH_BYTES = 17039616
+NOALLOC_MAX_PARTIALS = 1 << 21
+NOALLOC_SCRATCH_BYTES = 25428224
 GATHER_LANES = 0
 GATHER_ROW = 1
 
@@ -48,6 +48,7 @@ EXPORTED = (
     "anacor_ctx_run_noalloc", "anacor_ctx_paths",
     "anacor_ctx_stats", "anacor_reflection_geometry", "anacor_batch_create",
     "anacor_batch_destroy", "anacor_batch_sample", "anacor_batch_run", "anacor_batch_stats",
+    "anacor_batch_set_graph", "anacor_batch_last_graph",
     "anacor_mlp_create", "anacor_mlp_destroy", "anacor_mlp_num_params", "anacor_mlp_get_params",
     "anacor_mlp_set_params", "anacor_mlp_set_scale", "anacor_mlp_set_features",
     "anacor_mlp_set_envelope", "anacor_mlp_set_bands", "anacor_mlp_features",
@@ -150,6 +151,10 @@ def _prototypes(lib):
     lib.anacor_batch_sample.argtypes = [vp, i32]
     lib.anacor_batch_run.restype = ct.c_int
     lib.anacor_batch_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32, vp]
+    lib.anacor_batch_set_graph.restype = ct.c_int
+    lib.anacor_batch_set_graph.argtypes = [vp, i32]
+    lib.anacor_batch_last_graph.restype = ct.c_int
+    lib.anacor_batch_last_graph.argtypes = [vp]
     lib.anacor_batch_stats.restype = ct.c_int
     lib.anacor_batch_stats.argtypes = [vp, ct.POINTER(Stats)]
     lib.anacor_reflection_geometry.restype = ct.c_int

@@ -135,6 +135,7 @@ struct TraceArgs {
   const double* incd;          // (double)i for i = 0 .. incd_last
   int incd_last;
   int group;                   // reflections per group of the launch order (<= 1: none)
+  int pos0;                    // processing position of this launch's first reflection
 };
 
 // Map hardware workgroup id to a logical one so that consecutive logical
@@ -903,6 +904,7 @@ void trace_kernel(TraceArgs a) {
   if (lb >= a.n_blocks) return;
   int pos, tile;                         // processing position, voxel tile
   block_work(lb, a.tiles, a.R, a.group, pos, tile);
+  pos += a.pos0;
   const int refl = a.order[pos];         // the caller's reflection index
   SampleView S;
   S.V = make_volume(a.vol, a.Z, a.Y, a.X, a.vol_bytes, a.cz, a.cy, a.incd, a.incd_last);
@@ -950,12 +952,17 @@ __global__ void finish_kernel(const float2* __restrict__ p32, const double* __re
 // Several independent samples (own volume, voxel list, reflections) traced
 // as a clear kernel, one trace_kernel launch per sample into its context's
 // buffers (each launch fills the GPU: thousands of workgroups), and one
-// finish kernel over every sample's reflections.  The host prepares sample
-// i + 1 while the GPU traces sample i; uploads run on a second stream ahead
-// of the launches.  (A persistent work-queue kernel over all samples
-// measured 96 ms against 77 ms for these launches on the eight-sample
-// workload: its state held 124 VGPRs and kept it from the walk's faster
-// forms.)
+// finish kernel over every sample's reflections.  The host setup of sample
+// i + 1 runs while the GPU traces sample i; uploads run on a second stream
+// ahead of the launches.  The same sequence can run as one hipGraph (host
+// setups as host nodes on a branch of their own that each sample's launch
+// waits for), captured once per set of reflection counts and buffers and
+// then relaunched; on the eight-sample workload it measured 72.8 ms per step
+// against 64.0 ms for the launches from the host, because the graph ran its
+// host nodes in line with the kernels (18.5 ms of setups + the traces), so
+// the host launches are the default.  (A persistent work-queue kernel over
+// all samples measured 96 ms against 77 ms for these launches: its state
+// held 124 VGPRs and kept it from the walk's faster forms.)
 
 // What the clear and finish kernels need of each sample.
 struct BatchSample {
@@ -968,7 +975,7 @@ struct BatchSample {
 };
 
 // Clears the step counter and every sample's error words (a kernel rather
-// than hipMemsetAsync, so the graph holds kernels only).
+// than hipMemsetAsync: the batch graph holds no memset node).
 __global__ void batch_clear_kernel(const BatchSample* __restrict__ samples, int S,
                                    unsigned long long* steps) {
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -1269,6 +1276,7 @@ struct Pinned {
   }
   T& operator[](size_t i) { return p[i]; }
   T* data() { return p; }
+  const T* data() const { return p; }
 };
 
 // Device allocation owned by one scope: freed on every way out of it.
@@ -1305,7 +1313,8 @@ struct Context {
   int full = 0;
   int tile_voxels = 0;          // 0: automatic
   int refl_group = 0;           // reflections per launch group (0: automatic, 1: none)
-  int sort_major = 0;           // direction sort key led by ray 0 (incident) or 1 (diffracted)
+  int sort_major = -1;          // direction sort key led by ray 0 (incident) or 1 (diffracted);
+                                // -1: automatic (sort_major_for)
   std::vector<int4> h_coords;   // (z, y, x, index) in the caller's order
   int block[3] = {0, 0, 0};     // voxel grouping in use (0,0,0: Morton)
   // reusable per-run buffers
@@ -1330,7 +1339,7 @@ struct Context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_kernel_ms = 0.f;
   unsigned long long last_steps = 0;
-  int last_blocks = 0;
+  int64_t last_blocks = 0;
   double last_host_ms = 0.0;
   std::mutex mu_run;
 };
@@ -1393,14 +1402,19 @@ static int fit_tile(int64_t n, int64_t R, int vpt) {
 }
 
 // Automatic: at most 4 voxels per lane (1,024 per tile; measured on the 256^3
-// workload: 80 ms at 1,024, 84 ms at 6,144, 108 ms from 8,192 up).
-static int choose_tile(const Context* c, int64_t R) {
+// workload: 80 ms at 1,024, 84 ms at 6,144, 108 ms from 8,192 up), 1 in the
+// bricked layouts, where a smaller tile narrows the bundle of rays whose
+// bricks a launch group shares (1000x1000x800, 10k reflections, groups of
+// 2,048, sort led by the diffracted ray: 1,064 ms at 512, 995 ms at 256).
+static bool is_bricked(int layout);
+static int choose_tile(const Context* c, int64_t R, int layout = -1) {
   if (c->tile_voxels > 0) return c->tile_voxels;
+  const int64_t cap = (int64_t)kBlock * (is_bricked(layout) ? 1 : 4);
   const int64_t target_blocks = 256 * 16;
   int64_t tiles = (target_blocks + R - 1) / std::max<int64_t>(R, 1);
   tiles = std::max<int64_t>(1, std::min<int64_t>(tiles, (c->N + kBlock - 1) / kBlock));
   int64_t vpt = (c->N + tiles - 1) / tiles;
-  vpt = std::min<int64_t>(std::max<int64_t>(vpt, kBlock), (int64_t)kBlock * 4);
+  vpt = std::min<int64_t>(std::max<int64_t>(vpt, kBlock), cap);
   return (int)vpt;
 }
 
@@ -1562,20 +1576,36 @@ static void direction_order(int64_t R, DirOf dir_of, int* order, uint64_t* key, 
 }
 
 template <class DirOf>
-static void order_by_direction(Context* c, int64_t R, DirOf dir_of) {
+static void order_by_direction(Context* c, int64_t R, DirOf dir_of, int major) {
   c->h_order.resize((size_t)R);
   std::vector<uint64_t> key((size_t)R);
-  direction_order(R, dir_of, c->h_order.data(), key.data(), c->sort_major);
+  direction_order(R, dir_of, c->h_order.data(), key.data(), major);
 }
 
-static void order_reflections(Context* c, const ReflRule* rules, int64_t R) {
-  order_by_direction(c, R, [rules](int64_t i, int k) { return rules[i].ray[k].dir; });
+static bool is_bricked(int layout) { return layout == kLayoutBrick2 || layout == kLayoutBrick4; }
+
+// The ray whose direction leads the sort key.  Automatic: the incident one in
+// the raw layouts; the diffracted one in the bricked layouts, whose launch
+// groups then share the diffracted rays' bricks (1000x1000x800, 10k
+// reflections, groups of 128, 512-voxel tiles: 1,156 ms led by the incident
+// ray, 1,090 ms by the diffracted one).
+static int sort_major_for(const Context* c, int layout) {
+  if (c->sort_major >= 0) return c->sort_major;
+  return is_bricked(layout) ? 1 : 0;
+}
+
+static void order_reflections(Context* c, const ReflRule* rules, int64_t R, int layout) {
+  order_by_direction(c, R, [rules](int64_t i, int k) { return rules[i].ray[k].dir; },
+                     sort_major_for(c, layout));
 }
 
 // Reflections per group of the trace launch order (block_work).  Automatic:
 // none in the raw layouts, whose volumes the L2s and the Infinity Cache
-// already hold; kBrickGroup in the bricked ones.
-constexpr int kBrickGroup = 128;
+// already hold; kBrickGroup in the bricked ones (1000x1000x800, 10k
+// reflections, 256-voxel tiles: 1,350 ms without groups and the incident
+// ray leading the sort, 1,025 ms with groups of 64, 1,014 ms of 512,
+// 995 ms of 2,048).
+constexpr int kBrickGroup = 2048;
 static int launch_group(const Context* c, int layout) {
   if (c->refl_group > 0) return c->refl_group;
   return (layout == kLayoutBrick2 || layout == kLayoutBrick4) ? kBrickGroup : 1;
@@ -1636,10 +1666,10 @@ static int trace_call(Context* c, const RunBuffers& B, const ReflRule* h_rules,
   a.coords = c->d_coords;
   a.N = c->N;
   a.rules = B.rules;
-  a.R = (int)R;
+  a.R = 0;                     // per launch, below
   a.tiles = (int)tiles;
   a.vox_per_tile = vpt;
-  a.n_blocks = (int)nblk;
+  a.n_blocks = 0;
   for (int i = 0; i < 3; ++i) a.vs[i] = c->vs[i];
   for (int i = 0; i < 4; ++i) a.mu[i] = c->mu[i];
   a.is_exp = (flags & ANACOR_FLAG_NO_EXP) ? 0 : 1;
@@ -1652,11 +1682,20 @@ static int trace_call(Context* c, const RunBuffers& B, const ReflRule* h_rules,
   a.incd = c->d_incd;
   a.incd_last = c->incd_last;
   a.group = launch_group(c, layout);
-  const int grid = (int)((nblk + 7) / 8 * 8);
 
+  // a dispatch holds at most kMaxBlocks workgroups: a longer call runs as
+  // several launches of whole reflections (consecutive processing positions)
+  const int64_t per_launch = std::max<int64_t>(1, kMaxBlocks / std::max<int64_t>(tiles, 1));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  dispatch_trace(a, grid, guard, c->full != 0, layout, f64, paths, !c->labels_over4, c->stream);
-  HIP_TRY(hipGetLastError());
+  for (int64_t p0 = 0; p0 < R; p0 += per_launch) {
+    const int64_t Rc = std::min<int64_t>(per_launch, R - p0);
+    a.R = (int)Rc;
+    a.pos0 = (int)p0;
+    a.n_blocks = (int)(Rc * tiles);
+    dispatch_trace(a, (int)((Rc * tiles + 7) / 8 * 8), guard, c->full != 0, layout, f64, paths,
+                   !c->labels_over4, c->stream);
+    HIP_TRY(hipGetLastError());
+  }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   if (!paths) {
     hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, c->stream,
@@ -1673,7 +1712,7 @@ static int trace_call(Context* c, const RunBuffers& B, const ReflRule* h_rules,
                          hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipEventElapsedTime(&c->last_kernel_ms, c->ev0, c->ev1));
-  c->last_blocks = (int)nblk;
+  c->last_blocks = nblk;
   if (paths) return ANACOR_OK;
   // report the first failing reflection (its result is already NaN); the
   // error words are read back only when some result is NaN
@@ -1709,25 +1748,25 @@ static int run_rules(Context* c, const ReflRule* h_rules, int64_t R, int flags, 
     for (int64_t i = 0; i < R; ++i) out ? (void)(out[i] = __builtin_nan("")) : (void)0;
     return ANACOR_OK;
   }
-  const int vpt = fit_tile(c->N, R, paths ? kBlock : choose_tile(c, R));
-  if (vpt == 0) {
-    set_error(ANACOR_ERR_ARG, "%lld reflections in one call exceed one dispatch (at most %lld)",
-              (long long)R, (long long)kMaxBlocks);
-    return ANACOR_ERR_ARG;
-  }
-  const int64_t tiles = (c->N + vpt - 1) / vpt;
-  int rc = ensure_run_buffers(c, R, R * tiles, f64);
-  if (rc) return rc;
-  order_reflections(c, h_rules, R);
-  HIP_TRY(c->h_order_pin.reserve((size_t)R));
-  for (int64_t i = 0; i < R; ++i) c->h_order_pin[(size_t)i] = c->h_order[(size_t)i];
-
   int layout = raw_needs_wide(c) ? kLayoutWide : kLayoutRaw;   // path records: raw layout
+  int rc = ANACOR_OK;
   if (!paths) {
     rc = ensure_layout(c);
     if (rc) return rc;
     layout = c->layout;
   }
+  const int vpt = paths ? kBlock : choose_tile(c, R, layout);
+  const int64_t tiles = (c->N + vpt - 1) / vpt;
+  if (R > (int64_t)INT32_MAX / 2 || R * tiles > ((int64_t)1 << 40)) {
+    set_error(ANACOR_ERR_ARG, "%lld reflections x %lld tiles in one call", (long long)R,
+              (long long)tiles);
+    return ANACOR_ERR_ARG;
+  }
+  rc = ensure_run_buffers(c, R, R * tiles, f64);
+  if (rc) return rc;
+  order_reflections(c, h_rules, R, layout);
+  HIP_TRY(c->h_order_pin.reserve((size_t)R));
+  for (int64_t i = 0; i < R; ++i) c->h_order_pin[(size_t)i] = c->h_order[(size_t)i];
   DeviceTemp paths_buf;   // path records, freed on every return below
   if (paths) HIP_TRY(hipMalloc(&paths_buf.p, (size_t)R * c->N * sizeof(PathRecord)));
   RunBuffers B;
@@ -2044,16 +2083,16 @@ static int ctx_run_noalloc(Context* c, const double* s1, const double* omega, in
     return ANACOR_OK;
   }
   // the tile of anacor_ctx_run, enlarged until R * tiles fits the partials
-  int64_t vpt = choose_tile(c, R);
-  const int64_t max_tiles = kNoAllocMaxPartials / R;
-  if ((c->N + vpt - 1) / vpt > max_tiles) vpt = (c->N + max_tiles - 1) / max_tiles;
   rc = ensure_layout(c, /*may_build=*/false);
   if (rc) return rc;
+  int64_t vpt = choose_tile(c, R, c->layout);
+  const int64_t max_tiles = kNoAllocMaxPartials / R;
+  if ((c->N + vpt - 1) / vpt > max_tiles) vpt = (c->N + max_tiles - 1) / max_tiles;
   const auto t0 = std::chrono::steady_clock::now();
   // one host thread: the setup spawns none
   host_setup(s1, omega, R, raw_xray, omega_axis, F, max_dim(c), g_noalloc_rules, nullptr, 1);
   direction_order(R, [](int64_t i, int k) { return g_noalloc_rules[i].ray[k].dir; },
-                  g_noalloc_order, g_noalloc_key, c->sort_major);
+                  g_noalloc_order, g_noalloc_key, sort_major_for(c, c->layout));
   c->last_host_ms = ms_since(t0);
   RunBuffers B;
   B.rules = (ReflRule*)(base + kScratchRules);
@@ -2090,6 +2129,20 @@ static const uint8_t* flatten_table(int8_t*** table, const int64_t shape[3],
 
 // ------------------------------------------------------------ batch (host)
 
+struct Batch;
+
+// What a sample's host node needs: the caller's arrays of the current run
+// (rewritten before every launch of the graph) and what the node reports.
+struct BatchJob {
+  Batch* b = nullptr;
+  int i = 0;                    // sample index
+  const double* s1 = nullptr;
+  const double* omega = nullptr;
+  int64_t R = 0;
+  bool guard = false;           // some ray needs the guarded kernel
+  double host_ms = 0.0;
+};
+
 struct Batch {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -2099,23 +2152,45 @@ struct Batch {
   BatchSample* d_desc = nullptr;
   size_t desc_cap = 0;
   unsigned long long* d_counters = nullptr;   // the step counter
-  hipStream_t copy_stream = nullptr;          // rule uploads, ahead of the kernels
+  hipStream_t copy_stream = nullptr;          // host setup and rule uploads, ahead of the kernels
   std::vector<hipEvent_t> copied;             // per sample: its upload is done
+  hipEvent_t fork = nullptr;                  // copy_stream joins the captured sequence
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
   unsigned long long last_steps = 0;
   double last_host_ms = 0.0;
   long long blocks = 0;                       // workgroups of the last run's launches
   std::vector<BatchSample> h_desc;
+  Pinned<BatchSample> h_desc_pin;             // its upload staging (a node of the graph)
+  std::vector<int> seq;                       // launch order of the samples
+  // the run as one graph: captured for one set of reflection counts, flags
+  // and work shapes, relaunched while they stay the same
+  std::vector<BatchJob> jobs;
+  double geo[15] = {};                        // raw_xray, omega_axis, F of the current run
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<int64_t> graph_key;
+  bool use_graph = false;                     // run as the batch graph (anacor_batch_set_graph)
+  bool last_graph = false;                    // the last run launched the graph
   std::mutex mu;
 };
+
+static void batch_drop_graph(Batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_key.clear();
+}
 
 static void batch_destroy(Batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->copy_stream) (void)hipStreamSynchronize(b->copy_stream);
+  batch_drop_graph(b);
   for (hipEvent_t e : b->copied) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->copy_stream) (void)hipStreamDestroy(b->copy_stream);
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_counters) (void)hipFree(b->d_counters);
@@ -2162,6 +2237,7 @@ static int batch_create(const anacor_sample_t* in, int n, int full, int device, 
   b->copied.assign((size_t)n, nullptr);
   for (int i = 0; i < n; ++i)
     BATCH_TRY(hipEventCreateWithFlags(&b->copied[(size_t)i], hipEventDisableTiming));
+  BATCH_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   BATCH_TRY(hipEventCreate(&b->ev0));
   BATCH_TRY(hipEventCreate(&b->ev1));
   BATCH_TRY(hipMalloc(&b->d_counters, sizeof(unsigned long long)));
@@ -2170,117 +2246,55 @@ static int batch_create(const anacor_sample_t* in, int n, int full, int device, 
   return ANACOR_OK;
 }
 
-static int batch_run(Batch* b, const double* const* s1, const double* const* omega,
-                     const int64_t* R, const double* raw_xray, const double* omega_axis,
-                     const double* F, int flags, double* const* out) {
-  std::lock_guard<std::mutex> lk(b->mu);
-  HIP_TRY(hipSetDevice(b->device));
-  const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
-  const int is_exp = (flags & ANACOR_FLAG_NO_EXP) ? 0 : 1;
+// Host setup of one sample: rotation, angles, stepping rules and direction
+// order into the sample's page-locked staging.  It makes no HIP call, so it
+// can run as a host node of the batch graph as well as on the calling thread.
+static void batch_setup_sample(void* p) {
+  BatchJob* j = static_cast<BatchJob*>(p);
+  Batch* b = j->b;
+  Context* c = b->samples[(size_t)j->i];
+  const auto t0 = std::chrono::steady_clock::now();
+  host_setup(j->s1, j->omega, j->R, b->geo, b->geo + 3, b->geo + 6, max_dim(c),
+             c->h_rules.data(), nullptr, host_threads(0));
+  order_reflections(c, c->h_rules.data(), j->R, kLayoutRaw);
+  bool guard = false;
+  for (int64_t r = 0; r < j->R; ++r) {
+    c->h_order_pin[(size_t)r] = c->h_order[(size_t)r];
+    guard = guard || c->h_rules[(size_t)r].ray[0].guard || c->h_rules[(size_t)r].ray[1].guard;
+  }
+  j->guard = guard;
+  j->host_ms = ms_since(t0);
+}
+
+// The batch's work on b->stream, the host setup and rule uploads of every
+// sample on copy_stream ahead of it: clear, then per sample (in b->seq
+// order) its setup, upload and trace launch behind that upload, then one
+// finish kernel over every sample.  capture: the setups become host nodes
+// (the stream is being captured into the batch graph); otherwise they run
+// on this thread while the GPU traces the samples before them.
+static int batch_enqueue(Batch* b, bool capture, bool f64, int is_exp, long long total_refl) {
   const int S = (int)b->samples.size();
-  long long items = 0, total_refl = 0;
-  b->h_desc.resize((size_t)S);
-  // Staging buffers, tiles and device buffers of every sample first: the
-  // allocations stay on this thread and ahead of every launch.
-  for (int i = 0; i < S; ++i) {
-    Context* c = b->samples[(size_t)i];
-    const int64_t Ri = R[i];
-    if (Ri < 0 || (Ri > 0 && (!s1[i] || !omega[i] || !out[i]))) {
-      set_error(ANACOR_ERR_ARG, "sample %d: bad reflection arguments", i);
-      return ANACOR_ERR_ARG;
-    }
-    HIP_TRY(c->h_rules.reserve((size_t)std::max<int64_t>(Ri, 1)));
-    HIP_TRY(c->h_order_pin.reserve((size_t)std::max<int64_t>(Ri, 1)));
-    int vpt = 1;
-    if (c->N > 0) {
-      vpt = fit_tile(c->N, Ri, choose_tile(c, std::max<int64_t>(Ri, 1)));
-      if (vpt == 0) {
-        set_error(ANACOR_ERR_ARG, "sample %d: %lld reflections exceed one dispatch", i,
-                  (long long)Ri);
-        return ANACOR_ERR_ARG;
-      }
-    }
-    const int tiles = c->N > 0 ? (c->N + vpt - 1) / vpt : 0;
-    const int rc = ensure_run_buffers(c, std::max<int64_t>(Ri, 1),
-                                      std::max<int64_t>(Ri * tiles, 1), f64);
-    if (rc) return rc;
-    BatchSample& d = b->h_desc[(size_t)i];
-    d.order = c->d_order;
-    d.N = c->N;
-    d.R = (int)Ri;
-    d.tiles = std::max(tiles, 1);
-    d.vpt = vpt;
-    d.p32 = (float2*)c->d_partial;
-    d.p64 = (double*)c->d_partial;
-    d.err = c->d_err;
-    d.out = c->d_out;
-    items += (c->N > 0) ? Ri * tiles : 0;
-    total_refl += Ri;
-  }
-  if ((size_t)S > b->desc_cap) {
-    if (b->d_desc) HIP_TRY(hipFree(b->d_desc));
-    b->d_desc = nullptr;
-    HIP_TRY(hipMalloc(&b->d_desc, (size_t)S * sizeof(BatchSample)));
-    b->desc_cap = (size_t)S;
-  }
-  b->blocks = items;
-  HIP_TRY(hipMemcpyAsync(b->d_desc, b->h_desc.data(), (size_t)S * sizeof(BatchSample),
+  HIP_TRY(hipMemcpyAsync(b->d_desc, b->h_desc_pin.data(), (size_t)S * sizeof(BatchSample),
                          hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(batch_clear_kernel, dim3(64), dim3(256), 0, b->stream, b->d_desc, S,
                      b->d_counters);
   HIP_TRY(hipGetLastError());
-  bool timing = false;   // ev0 goes in behind the first sample's upload
-  // Sample by sample: host setup and direction sort on every host thread,
-  // the upload on the copy stream, then the sample's trace launch behind
-  // that upload.  The host prepares the next sample while the GPU traces the
-  // previous one: a two-stage flow shop, ordered by Johnson's rule on
-  // estimated stage times (host ~0.085 us per reflection on eight threads;
-  // GPU ~1.3 ps per voxel step of either ray, a ray taking about a sixth of
-  // the extents' sum): samples whose trace outlasts their setup first, by
-  // increasing setup, then the others by decreasing trace.  Only the first
-  // setup is then exposed.  Results do not depend on the order.
-  std::vector<int> seq((size_t)S);
-  {
-    std::vector<double> setup((size_t)S), trace((size_t)S);
-    for (int i = 0; i < S; ++i) {
-      const Context* c = b->samples[(size_t)i];
-      setup[(size_t)i] = 0.085e-6 * (double)R[i];
-      trace[(size_t)i] = 1.3e-12 * (double)R[i] * (double)c->N * (c->Z + c->Y + c->X) / 6.0;
-    }
-    for (int i = 0; i < S; ++i) seq[(size_t)i] = i;
-    auto first = [&](int i) { return setup[(size_t)i] < trace[(size_t)i]; };
-    std::stable_sort(seq.begin(), seq.end(), [&](int x, int y) {
-      if (first(x) != first(y)) return first(x);
-      return first(x) ? setup[(size_t)x] < setup[(size_t)y] : trace[(size_t)x] > trace[(size_t)y];
-    });
-  }
-  double host_ms = 0.0;
+  HIP_TRY(hipEventRecord(b->fork, b->stream));
+  HIP_TRY(hipStreamWaitEvent(b->copy_stream, b->fork, 0));
   for (int j = 0; j < S; ++j) {
-    const int i = seq[(size_t)j];
+    const int i = b->seq[(size_t)j];
     Context* c = b->samples[(size_t)i];
-    const int64_t Ri = R[i];
-    const auto t0 = std::chrono::steady_clock::now();
-    host_setup(s1[i], omega[i], Ri, raw_xray, omega_axis, F, max_dim(c), c->h_rules.data(),
-               nullptr, host_threads(0));
-    order_reflections(c, c->h_rules.data(), Ri);
-    bool guard = false;
-    for (int64_t r = 0; r < Ri; ++r) {
-      c->h_order_pin[(size_t)r] = c->h_order[(size_t)r];
-      guard = guard || c->h_rules[(size_t)r].ray[0].guard || c->h_rules[(size_t)r].ray[1].guard;
-    }
-    host_ms += ms_since(t0);
-    if (Ri == 0) continue;
-    HIP_TRY(hipMemcpyAsync(c->d_rules, c->h_rules.data(), (size_t)Ri * sizeof(ReflRule),
+    BatchJob& job = b->jobs[(size_t)i];
+    if (capture) HIP_TRY(hipLaunchHostFunc(b->copy_stream, batch_setup_sample, &job));
+    else batch_setup_sample(&job);
+    if (job.R == 0) continue;
+    HIP_TRY(hipMemcpyAsync(c->d_rules, c->h_rules.data(), (size_t)job.R * sizeof(ReflRule),
                            hipMemcpyHostToDevice, b->copy_stream));
-    HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order_pin.data(), (size_t)Ri * sizeof(int),
+    HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order_pin.data(), (size_t)job.R * sizeof(int),
                            hipMemcpyHostToDevice, b->copy_stream));
     HIP_TRY(hipEventRecord(b->copied[(size_t)i], b->copy_stream));
     HIP_TRY(hipStreamWaitEvent(b->stream, b->copied[(size_t)i], 0));
     if (c->N == 0) continue;
-    if (!timing) {
-      HIP_TRY(hipEventRecord(b->ev0, b->stream));
-      timing = true;
-    }
     const BatchSample& d = b->h_desc[(size_t)i];
     TraceArgs t;
     t.vol = c->d_vol;
@@ -2306,29 +2320,193 @@ static int batch_run(Batch* b, const double* const* s1, const double* const* ome
     t.incd = c->d_incd;
     t.incd_last = c->incd_last;
     t.group = launch_group(c, kLayoutRaw);
+    t.pos0 = 0;
+    // a graph is captured before any setup has run: it holds the unguarded
+    // kernel, and a run whose rays need the guard is traced again without it
+    const bool guard = !capture && job.guard;
     dispatch_trace(t, (t.n_blocks + 7) / 8 * 8, guard, b->full != 0,
                    raw_needs_wide(c) ? kLayoutWide : kLayoutRaw, f64, false, !c->labels_over4,
                    b->stream);
     HIP_TRY(hipGetLastError());
   }
-  if (!timing) HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  // rejoin copy_stream (a captured fork must end joined)
+  HIP_TRY(hipEventRecord(b->fork, b->copy_stream));
+  HIP_TRY(hipStreamWaitEvent(b->stream, b->fork, 0));
   if (total_refl > 0) {
     hipLaunchKernelGGL(batch_finish_kernel, dim3((unsigned)((total_refl + 255) / 256)),
                        dim3(256), 0, b->stream, b->d_desc, S, f64 ? 1 : 0, total_refl);
     HIP_TRY(hipGetLastError());
   }
-  b->last_host_ms = host_ms;
-  HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  return ANACOR_OK;
+}
+
+// Capture the batch's sequence into a graph and instantiate it.
+static int batch_capture(Batch* b, bool f64, int is_exp, long long total_refl) {
+  batch_drop_graph(b);
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = batch_enqueue(b, true, f64, is_exp, total_refl);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc != ANACOR_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    set_error(ANACOR_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    return ANACOR_ERR_HIP;
+  }
+  b->graph = g;
+  HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return ANACOR_OK;
+}
+
+static int batch_run(Batch* b, const double* const* s1, const double* const* omega,
+                     const int64_t* R, const double* raw_xray, const double* omega_axis,
+                     const double* F, int flags, double* const* out) {
+  std::lock_guard<std::mutex> lk(b->mu);
+  HIP_TRY(hipSetDevice(b->device));
+  const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
+  const int is_exp = (flags & ANACOR_FLAG_NO_EXP) ? 0 : 1;
+  const int S = (int)b->samples.size();
+  long long items = 0, total_refl = 0;
+  b->h_desc.resize((size_t)S);
+  b->jobs.resize((size_t)S);
+  // Staging buffers, tiles and device buffers of every sample first: the
+  // allocations stay on this thread and ahead of every launch (and outside
+  // any capture).
   for (int i = 0; i < S; ++i) {
     Context* c = b->samples[(size_t)i];
-    if (R[i] > 0)
-      HIP_TRY(hipMemcpyAsync(out[i], c->d_out, (size_t)R[i] * sizeof(double),
-                             hipMemcpyDeviceToHost, b->stream));
+    const int64_t Ri = R[i];
+    if (Ri < 0 || (Ri > 0 && (!s1[i] || !omega[i] || !out[i]))) {
+      set_error(ANACOR_ERR_ARG, "sample %d: bad reflection arguments", i);
+      return ANACOR_ERR_ARG;
+    }
+    HIP_TRY(c->h_rules.reserve((size_t)std::max<int64_t>(Ri, 1)));
+    HIP_TRY(c->h_order_pin.reserve((size_t)std::max<int64_t>(Ri, 1)));
+    c->h_order.reserve((size_t)std::max<int64_t>(Ri, 1));
+    int vpt = 1;
+    if (c->N > 0) {
+      vpt = fit_tile(c->N, Ri, choose_tile(c, std::max<int64_t>(Ri, 1)));
+      if (vpt == 0) {
+        set_error(ANACOR_ERR_ARG, "sample %d: %lld reflections exceed one dispatch", i,
+                  (long long)Ri);
+        return ANACOR_ERR_ARG;
+      }
+    }
+    const int tiles = c->N > 0 ? (c->N + vpt - 1) / vpt : 0;
+    const int rc = ensure_run_buffers(c, std::max<int64_t>(Ri, 1),
+                                      std::max<int64_t>(Ri * tiles, 1), f64);
+    if (rc) return rc;
+    BatchSample& d = b->h_desc[(size_t)i];
+    d.order = c->d_order;
+    d.N = c->N;
+    d.R = (int)Ri;
+    d.tiles = std::max(tiles, 1);
+    d.vpt = vpt;
+    d.p32 = (float2*)c->d_partial;
+    d.p64 = (double*)c->d_partial;
+    d.err = c->d_err;
+    d.out = c->d_out;
+    BatchJob& job = b->jobs[(size_t)i];
+    job.b = b;
+    job.i = i;
+    job.s1 = s1[i];
+    job.omega = omega[i];
+    job.R = Ri;
+    job.guard = false;
+    job.host_ms = 0.0;
+    items += (c->N > 0) ? Ri * tiles : 0;
+    total_refl += Ri;
   }
-  HIP_TRY(hipMemcpyAsync(&b->last_steps, b->d_counters, sizeof(unsigned long long),
-                         hipMemcpyDeviceToHost, b->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
-  HIP_TRY(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+  if ((size_t)S > b->desc_cap) {
+    if (b->d_desc) HIP_TRY(hipFree(b->d_desc));
+    b->d_desc = nullptr;
+    HIP_TRY(hipMalloc(&b->d_desc, (size_t)S * sizeof(BatchSample)));
+    b->desc_cap = (size_t)S;
+  }
+  HIP_TRY(b->h_desc_pin.reserve((size_t)S));
+  for (int i = 0; i < S; ++i) b->h_desc_pin[(size_t)i] = b->h_desc[(size_t)i];
+  std::memcpy(b->geo, raw_xray, 3 * sizeof(double));
+  std::memcpy(b->geo + 3, omega_axis, 3 * sizeof(double));
+  std::memcpy(b->geo + 6, F, 9 * sizeof(double));
+  b->blocks = items;
+  // Sample order: a two-stage flow shop (host setup, then GPU trace) ordered
+  // by Johnson's rule on estimated stage times (host ~0.085 us per
+  // reflection on eight threads; GPU ~1.3 ps per voxel step of either ray, a
+  // ray taking about a sixth of the extents' sum): samples whose trace
+  // outlasts their setup first, by increasing setup, then the others by
+  // decreasing trace.  Only the first setup is then exposed.  Results do not
+  // depend on the order.
+  b->seq.resize((size_t)S);
+  {
+    std::vector<double> setup((size_t)S), trace((size_t)S);
+    for (int i = 0; i < S; ++i) {
+      const Context* c = b->samples[(size_t)i];
+      setup[(size_t)i] = 0.085e-6 * (double)R[i];
+      trace[(size_t)i] = 1.3e-12 * (double)R[i] * (double)c->N * (c->Z + c->Y + c->X) / 6.0;
+    }
+    for (int i = 0; i < S; ++i) b->seq[(size_t)i] = i;
+    auto first = [&](int i) { return setup[(size_t)i] < trace[(size_t)i]; };
+    std::stable_sort(b->seq.begin(), b->seq.end(), [&](int x, int y) {
+      if (first(x) != first(y)) return first(x);
+      return first(x) ? setup[(size_t)x] < setup[(size_t)y] : trace[(size_t)x] > trace[(size_t)y];
+    });
+  }
+  // The graph holds every buffer address and launch shape: it is captured
+  // again whenever one of them changes.
+  std::vector<int64_t> key = {S, f64 ? 1 : 0, is_exp, b->full};
+  for (int i = 0; i < S; ++i) {
+    const Context* c = b->samples[(size_t)i];
+    const BatchSample& d = b->h_desc[(size_t)i];
+    for (int64_t v : {(int64_t)R[i], (int64_t)d.vpt, (int64_t)launch_group(c, kLayoutRaw),
+                      (int64_t)(intptr_t)c->d_rules, (int64_t)(intptr_t)c->d_order,
+                      (int64_t)(intptr_t)c->d_partial, (int64_t)(intptr_t)c->d_err,
+                      (int64_t)(intptr_t)c->d_out, (int64_t)(intptr_t)c->d_coords,
+                      (int64_t)(intptr_t)c->h_rules.data(),
+                      (int64_t)(intptr_t)c->h_order_pin.data()})
+      key.push_back(v);
+  }
+  key.push_back((int64_t)(intptr_t)b->h_desc_pin.data());
+  const bool use_graph = b->use_graph;
+  if (use_graph && (!b->exec || key != b->graph_key)) {
+    const int rc = batch_capture(b, f64, is_exp, total_refl);
+    if (rc != ANACOR_OK) return rc;
+    b->graph_key = key;
+  }
+  auto launch = [&](bool graph) -> int {
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    if (graph) {
+      HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    } else {
+      const int rc = batch_enqueue(b, false, f64, is_exp, total_refl);
+      if (rc != ANACOR_OK) return rc;
+    }
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    for (int i = 0; i < S; ++i) {
+      Context* c = b->samples[(size_t)i];
+      if (R[i] > 0)
+        HIP_TRY(hipMemcpyAsync(out[i], c->d_out, (size_t)R[i] * sizeof(double),
+                               hipMemcpyDeviceToHost, b->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(&b->last_steps, b->d_counters, sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+    return ANACOR_OK;
+  };
+  int rc = launch(use_graph);
+  if (rc != ANACOR_OK) return rc;
+  bool guard = false;
+  for (const BatchJob& j : b->jobs) guard = guard || j.guard;
+  b->last_graph = use_graph;
+  if (use_graph && guard) {   // rays beyond the int64 range: the guarded kernels, live
+    rc = launch(false);
+    if (rc != ANACOR_OK) return rc;
+    b->last_graph = false;
+  }
+  double host_ms = 0.0;
+  for (const BatchJob& j : b->jobs) host_ms += j.host_ms;
+  b->last_host_ms = host_ms;
   for (int i = 0; i < S; ++i) {
     Context* c = b->samples[(size_t)i];
     if (c->N == 0)
@@ -2931,7 +3109,7 @@ static int ib_run(Context* c, int64_t R, int num_cls, int flags, double* out, do
   }
   if (!c->d_oob) HIP_TRY(hipMalloc(&c->d_oob, sizeof(unsigned long long)));
   const IbRule* rules = c->h_ib.data();
-  order_by_direction(c, R, [rules](int64_t i, int k) { return rules[i].ray[k].dir; });
+  order_by_direction(c, R, [rules](int64_t i, int k) { return rules[i].ray[k].dir; }, 0);
   HIP_TRY(c->h_order_pin.reserve((size_t)R));
   for (int64_t i = 0; i < R; ++i) c->h_order_pin[(size_t)i] = c->h_order[(size_t)i];
   // Automatic layout: bricks whenever they fit.  The bisection's gathers are
@@ -3009,7 +3187,7 @@ static int ib_run(Context* c, int64_t R, int num_cls, int flags, double* out, do
                          hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipEventElapsedTime(&c->last_kernel_ms, c->ev0, c->ev1));
-  c->last_blocks = (int)nblk;
+  c->last_blocks = nblk;
   return report_ray_errors(c, out, R);
 }
 
@@ -3084,7 +3262,7 @@ int anacor_ctx_set_voxel_block(anacor_ctx* ctx, int bz, int by, int bx) {
 
 int anacor_ctx_set_order(anacor_ctx* ctx, int refl_group, int direction_major) {
   if (!ctx || !ctx->c || refl_group < 0 || refl_group > (1 << 16) ||
-      (direction_major != 0 && direction_major != 1)) {
+      direction_major < -1 || direction_major > 1) {
     set_error(ANACOR_ERR_ARG, "bad context or launch order (%d, %d)", refl_group,
               direction_major);
     return ANACOR_ERR_ARG;
@@ -3106,7 +3284,7 @@ int anacor_ctx_get_work_shape(const anacor_ctx* ctx, int block[3], int* tile, in
     for (int i = 0; i < 3; ++i) block[i] = c->block[i];
   if (tile) *tile = c->tile_voxels;
   if (refl_group) *refl_group = launch_group(c, c->layout);
-  if (direction_major) *direction_major = c->sort_major;
+  if (direction_major) *direction_major = sort_major_for(c, c->layout);
   return ANACOR_OK;
 }
 
@@ -3190,6 +3368,20 @@ int anacor_batch_run(anacor_batch* batch, const double* const* s1, const double*
     return ANACOR_ERR_ARG;
   }
   return batch_run(batch->b, s1, omega, R, raw_xray, omega_axis, F, flags, out);
+}
+
+int anacor_batch_set_graph(anacor_batch* batch, int use_graph) {
+  if (!batch || !batch->b) {
+    set_error(ANACOR_ERR_ARG, "null batch");
+    return ANACOR_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(batch->b->mu);
+  batch->b->use_graph = use_graph != 0;
+  return ANACOR_OK;
+}
+
+int anacor_batch_last_graph(const anacor_batch* batch) {
+  return (batch && batch->b && batch->b->last_graph) ? 1 : 0;
 }
 
 int anacor_batch_stats(const anacor_batch* batch, anacor_stats_t* st) {

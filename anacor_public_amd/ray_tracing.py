@@ -225,10 +225,10 @@ class Tracer:
     def set_tile(self, voxels_per_tile: int):
         self._lib.anacor_ctx_set_tile(self.handle, int(voxels_per_tile))
 
-    def set_order(self, refl_group: int = 0, direction_major: int = 0):
+    def set_order(self, refl_group: int = 0, direction_major: int = -1):
         """Launch order of the trace kernel: groups of refl_group direction-sorted
         reflections run tile by tile (0 automatic, 1 none); the sort key is led
-        by the incident (0) or the diffracted (1) ray.  Speed only."""
+        by the incident (0) or the diffracted (1) ray (-1 automatic).  Speed only."""
         _lib.check(self._lib.anacor_ctx_set_order(self.handle, int(refl_group),
                                                   int(direction_major)), self._lib)
 
@@ -365,9 +365,12 @@ def reflection_geometry(s1, omega, raw_xray, omega_axis, F, max_dim: int):
 
 
 class Batch:
-    """Several independent samples traced by per-sample trace launches
-    pipelined behind the host setup (``anacor_batch_*``).  ``samples`` are Workload-like objects
-    (volume, coords, voxel_size, coefficients); all share full_iteration."""
+    """Several independent samples traced together (``anacor_batch_*``): per
+    sample its host setup, rule upload and trace launch, each setup
+    overlapping the trace before it; launched from the host or, with
+    ``set_graph(True)``, as one hipGraph.  ``samples`` are Workload-like
+    objects (volume, coords, voxel_size, coefficients); all share
+    full_iteration."""
 
     def __init__(self, samples, full_iteration: int = 0, device: int = 0):
         self._lib = _lib.load()
@@ -416,6 +419,15 @@ class Batch:
         _lib.check(rc, self._lib, allow=(_lib.ANACOR_OK,) if strict else
                    (_lib.ANACOR_OK, _lib.ANACOR_ERR_GEOMETRY))
         return outs
+
+    def set_graph(self, use_graph: bool) -> None:
+        """Run as one hipGraph (True) or launch from the host (False, default)."""
+        _lib.check(self._lib.anacor_batch_set_graph(self._h, int(bool(use_graph))), self._lib)
+
+    @property
+    def last_graph(self) -> bool:
+        """Whether the most recent run launched the batch graph."""
+        return bool(self._lib.anacor_batch_last_graph(self._h))
 
     def stats(self) -> dict:
         st = _lib.Stats()

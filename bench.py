@@ -505,10 +505,30 @@ def measure_plain(config: str, steps: int, warmup: int, cpu_budget: float,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
     }
     if config == "batched":
-        rec["graph"] = False
-        rec["launch"] = ("per-sample trace launches on one stream behind each sample's "
-                         "upload, host setup of the next sample overlapped")
-        rec["roofline"]["kernel_ms_covers"] = "first sample's trace to the finish kernel"
+        rec["graph"] = runner.last_graph
+        rec["launch"] = ("from the host: clear, per sample its host setup, rule upload and "
+                         "trace launch (each setup overlapping the trace before it), one "
+                         "finish kernel")
+        rec["roofline"]["kernel_ms_covers"] = ("the whole sequence, the first sample's host "
+                                               "setup included")
+        # the same sequence as one hipGraph (host setups as host nodes)
+        runner.set_graph(True)
+        call()                                   # capture and first launch
+        g_steps = max(1, min(steps, 5))
+        t_g = time.perf_counter()
+        g_kms = []
+        for _ in range(g_steps):
+            g_outs = call()
+            g_kms.append(runner.stats()["kernel_ms"])
+        g_el = time.perf_counter() - t_g
+        same = all(np.array_equal(a.view(np.uint64), b.view(np.uint64))
+                   for a, b in zip(g_outs, outs))
+        rec["as_graph"] = {"graph": runner.last_graph, "steps": g_steps,
+                           "ms_per_step": 1e3 * g_el / g_steps, "value": R * g_steps / g_el,
+                           "kernel_ms": float(np.mean(g_kms)), "results_identical": same}
+        runner.set_graph(False)
+        log(f"# {name} as one hipGraph: {rec['as_graph']['ms_per_step']:.2f} ms/step "
+            f"(results identical: {same})")
     if config != "batched" and gather:
         rec["roofline"]["gather_ceiling"] = gather_ceiling(
             volume_bytes(wls[0].volume.shape, layout), achieved)
@@ -518,6 +538,30 @@ def measure_plain(config: str, steps: int, warmup: int, cpu_budget: float,
     rec.update(extra)
     runner.close()
     return rec
+
+
+def profile_shapes() -> None:
+    """One trace launch per autotuner candidate (layout, voxel block, tile)
+    of the 256^3 workload in a fixed order, printing one JSON line per launch
+    so that the per-dispatch counters of a rocprofv3 --pmc run can be matched
+    to work shapes (the figures of TRAFFIC_BYTES_PER_STEP)."""
+    from anacor_public_amd import autotune
+    from anacor_public_amd import workload as W
+    from anacor_public_amd.ray_tracing import Tracer
+    wl = W.single_gpu_config()
+    with Tracer.for_workload(wl) as t:
+        for lay in autotune.LAYOUTS:
+            t.set_layout(lay)
+            for blk in autotune.BLOCKS:
+                t.set_voxel_block(*blk)
+                for tile in autotune.TILES:
+                    t.set_tile(tile)
+                    t.run_workload(wl)
+                    st, ws = t.stats(), t.work_shape()
+                    print(json.dumps({"workload": wl.name, "layout": t.layout,
+                                      "block": ws["block"], "tile": ws["tile"],
+                                      "group": ws["group"], "voxel_steps": st["voxel_steps"],
+                                      "kernel_ms": st["kernel_ms"]}), flush=True)
 
 
 def main():
@@ -543,11 +587,22 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--reflections", type=int, default=0,
                     help="trace only the first R reflections of each sample (profiling runs)")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time every (layout, voxel block, tile) candidate on a slice of the "
+                         "reflections first and keep the fastest (default: the library's "
+                         "automatic work shape, whose traffic figure is on record)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="the batched record's per-sample tuning off")
+    ap.add_argument("--profile-shapes", action="store_true",
+                    help="one trace launch per autotuner work shape of the 256^3 workload, "
+                         "in a fixed order, one JSON line each (for rocprofv3 --pmc passes)")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.profile_shapes:
+        profile_shapes()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards "
@@ -575,7 +630,8 @@ def main():
 
     rec = measure_plain(args.config, args.steps, args.warmup, budget,
                         reflections=args.reflections, tile=args.tile,
-                        autotune_on=not args.no_autotune)
+                        autotune_on=(args.autotune if args.config != "batched"
+                                     else not args.no_autotune))
     head = {
         "metric": METRIC,
         "value": rec.pop("value"),
@@ -605,7 +661,7 @@ def main():
         head["surrogate"] = run_surrogate(args.steps, args.warmup, epochs=args.epochs)
     if default_run and not args.no_batched:
         head["batched"] = measure_plain("batched", min(args.steps, 10), min(args.warmup, 2),
-                                        min(budget, 10.0))
+                                        min(budget, 10.0), autotune_on=not args.no_autotune)
     if default_run and not args.no_large:
         head["large"] = measure_plain("large", min(args.steps, 3), min(args.warmup, 1),
                                       min(budget, 6.0), reflections=args.large_reflections,

@@ -104,16 +104,18 @@ anacor_ctx *anacor_ctx_create(const uint8_t *volume, const int64_t shape[3],
                               int full_iteration, int device);
 void anacor_ctx_destroy(anacor_ctx *ctx);
 
-/* Voxels per workgroup tile (0 = automatic). */
+/* Voxels per workgroup tile (0 = automatic: at most 1,024, 512 in the
+ * bricked layouts). */
 int anacor_ctx_set_tile(anacor_ctx *ctx, int voxels_per_tile);
 
 /* Launch order of the trace kernel (speed only, never results): reflections
  * are sorted by the directions of their rays, led by the incident ray's
- * (direction_major 0) or the diffracted ray's (1), and launched in groups of
- * refl_group consecutive reflections, tile by tile within a group, so that
- * the workgroups resident on an XCD share the bricks their rays cross
- * (0: automatic, 128 in the bricked layouts and none in the raw one; 1: no
- * groups, every tile of a reflection in turn). */
+ * (direction_major 0) or the diffracted ray's (1; -1: automatic, the
+ * diffracted one in the bricked layouts, else the incident one), and launched
+ * in groups of refl_group consecutive reflections, tile by tile within a
+ * group, so that the workgroups resident on an XCD share the bricks their
+ * rays cross (0: automatic, 128 in the bricked layouts and none in the raw
+ * one; 1: no groups, every tile of a reflection in turn). */
 int anacor_ctx_set_order(anacor_ctx *ctx, int refl_group, int direction_major);
 
 /* The work shape in use: voxel block (bz, by, bx; 0, 0, 0 = Morton), tile
@@ -166,8 +168,8 @@ int anacor_ctx_run(anacor_ctx *ctx, const double *s1, const double *omega, int64
  * rewrites the array (its error words and step counter are cleared first);
  * any other use of the array by overlapping work on the same GPU would race. */
 #define ANACOR_NOALLOC_MAX_REFLECTIONS 16384
-#define ANACOR_NOALLOC_MAX_PARTIALS    (1 << 20)
-#define ANACOR_NOALLOC_SCRATCH_BYTES   17039616   /* 16.25 MiB */
+#define ANACOR_NOALLOC_MAX_PARTIALS    (1 << 21)
+#define ANACOR_NOALLOC_SCRATCH_BYTES   25428224   /* 24.25 MiB */
 int anacor_ctx_run_noalloc(anacor_ctx *ctx, const double *s1, const double *omega, int64_t R,
                            const double raw_xray[3], const double omega_axis[3],
                            const double F[9], int flags, double *out);
@@ -215,9 +217,12 @@ typedef struct {
     double         coefficients[4];
 } anacor_sample_t;
 
-/* Several samples traced by one trace launch per sample, each behind its
- * own upload and overlapping the host setup of the next, then one finish
- * kernel over all of them.
+/* Several samples traced as a clear kernel, per sample its host setup, rule
+ * upload and trace launch (the setup of each sample overlapping the trace of
+ * the one before), then one finish kernel over all of them: launched from
+ * the host, or (anacor_batch_set_graph) as one hipGraph with the setups as
+ * host nodes, captured on the first run and relaunched while the reflection
+ * counts and buffers stay the same.
  * No counterpart in the reference, whose driver runs samples one after the
  * other (main_lite.py:276-341).  All samples share full_iteration. */
 typedef struct anacor_batch anacor_batch;
@@ -236,8 +241,19 @@ int anacor_batch_run(anacor_batch *batch, const double *const *s1, const double 
                      const int64_t *R, const double raw_xray[3], const double omega_axis[3],
                      const double F[9], int flags, double *const *out);
 
-/* Statistics of the most recent batch run (kernel_ms covers the whole graph). */
+/* Statistics of the most recent batch run (kernel_ms covers the whole
+ * sequence, the first sample's host setup included). */
 int anacor_batch_stats(const anacor_batch *batch, anacor_stats_t *stats);
+
+/* Run the batch as one hipGraph (use_graph 1) or launch its sequence from
+ * the host (0, the default: faster where the runtime runs the graph's host
+ * nodes in line with its kernels).  Results are the same bit for bit. */
+int anacor_batch_set_graph(anacor_batch *batch, int use_graph);
+
+/* 1 when the most recent batch run launched the batch graph, 0 when it ran
+ * the sequence from the host (graph off, or rays that need the guarded
+ * kernel). */
+int anacor_batch_last_graph(const anacor_batch *batch);
 
 /* ------------------------------------------------------------ gridding path */
 

@@ -1,5 +1,6 @@
-"""GPU tests of the batched path (several samples, per-sample trace launches
-pipelined behind the host setup) and of the autotuner.
+"""GPU tests of the batched path (several samples: per-sample host setup,
+upload and trace launch, from the host or as one hipGraph) and of the
+autotuner.
 
 Each sample's result must equal the unbatched path on the same sample
 bit for bit (same voxel order and tiles, so the same partial sums), across
@@ -80,6 +81,35 @@ def test_repeated_runs_with_eager_work_between(gpu, samples):
         f64 = b.run(*args, f64_mean=True)
         for i in range(len(samples)):
             np.testing.assert_allclose(f64[i], first[i], rtol=1e-6, atol=0)
+
+
+def test_graph_replays_equal_live_launches(gpu, samples):
+    """The batch graph, captured once and relaunched with eager work and
+    synchronisations between the replays, gives on every replay the results
+    of the same sequence launched from the host, bit for bit; new reflection
+    arrays of the same sizes reuse the graph."""
+    from anacor_public_amd import workload as W
+    from anacor_public_amd.ray_tracing import Batch, Tracer
+    say("case: batch graph replays")
+    wl0 = samples[0]
+    geo = (wl0.raw_xray, wl0.omega_axis, wl0.F)
+    args = ([w.s1 for w in samples], [w.omega for w in samples], *geo)
+    other = [W.make_reflections(len(w.omega), 900 + i, stream=404) for i, w in enumerate(samples)]
+    args2 = ([o[0] for o in other], [o[1] for o in other], *geo)
+    with Batch(samples) as b, Tracer.for_workload(samples[3]) as eager:
+        live = b.run(*args)                      # launched from the host (the default)
+        live2 = b.run(*args2)
+        assert not b.last_graph
+        b.set_graph(True)
+        for rep in range(3):
+            cur = args if rep != 1 else args2
+            got = b.run(*cur)
+            assert b.last_graph
+            ref = live if rep != 1 else live2
+            for i in range(len(samples)):
+                np.testing.assert_array_equal(bits(got[i]), bits(ref[i]),
+                                              err_msg=f"replay {rep} sample {i}")
+            eager.run_workload(samples[3])            # eager work between the replays
 
 
 def test_autotuner_keeps_results(gpu, oracle, tmp_path, monkeypatch, samples):
