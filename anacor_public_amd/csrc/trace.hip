@@ -1541,7 +1541,7 @@ static int ensure_layout(Context* c, bool may_build = true) {
 }
 
 // Processing order of the reflections: sorted by the directions of their two
-// rays (7-bit Morton code of each unit vector, incident first), so the
+// rays (7-bit Morton code of each unit vector, ray `major` first), so the
 // workgroups resident on an XCD at one time trace nearly parallel rays whose
 // label gathers share that XCD's L2.  Results are written back through the
 // inverse permutation, so the order is invisible to callers.

@@ -44,14 +44,55 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 MLP_FLOPS_PER_ROW = 2 * (12 * 256 + 3 * 256 * 256 + 256)   # 2 x 199,936 MACs
-# L2<->fabric bytes per voxel step of trace_kernel on the 256^3 workload,
-# by voxel tile (0: automatic, 1,024 voxels here): FETCH_SIZE x 2.000 +
-# WRITE_SIZE x 1.000 (factors from a 1 GiB copy) over 1.017e11 voxel steps;
-# 512 with (1, 2, 64) voxel blocks from profiles/trace_kernel_plain-256cube.md,
-# automatic with (2, 2, 32) blocks from a FETCH_SIZE pass of the autotuned
-# default run.  Larger tiles spread a reflection's workgroups over more XCDs.
-TRAFFIC_BYTES_PER_STEP = {("plain-256^3-stride2", 512): 0.093,
-                          ("plain-256^3-stride2", 0): 0.200}
+# L2<->fabric bytes per voxel step of trace_kernel, by exact work shape
+# (workload, layout, voxel block, tile, reflection group; tile 0 is the
+# library's automatic choice): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (the
+# factors of a 1 GiB copy) over the launch's voxel steps.  The 256^3 rows are
+# profiles/trace_kernel_work-shapes_plain-256cube.md (one launch per
+# autotuner candidate), the 1000x1000x800 row
+# profiles/trace_kernel_brick2_large-1000x1000x800.md.  A shape not listed
+# here reports traffic null.
+_HEAD = "plain-256^3-stride2"
+_LARGE10K = "plain-1000x1000x800-100k-first10000"
+TRAFFIC_BYTES_PER_STEP = {
+    (_HEAD, 0, (1, 2, 64), 0, 1): 0.2099,
+    (_HEAD, 0, (1, 2, 64), 512, 1): 0.0961,
+    (_HEAD, 0, (1, 2, 64), 2048, 1): 0.4298,
+    (_HEAD, 0, (1, 1, 128), 0, 1): 0.2099,
+    (_HEAD, 0, (1, 1, 128), 512, 1): 0.0960,
+    (_HEAD, 0, (1, 1, 128), 2048, 1): 0.4299,
+    (_HEAD, 0, (2, 2, 32), 0, 1): 0.2100,
+    (_HEAD, 0, (2, 2, 32), 512, 1): 0.0960,
+    (_HEAD, 0, (2, 2, 32), 2048, 1): 0.4298,
+    (_HEAD, 0, (1, 4, 32), 0, 1): 0.2213,
+    (_HEAD, 0, (1, 4, 32), 512, 1): 0.1054,
+    (_HEAD, 0, (1, 4, 32), 2048, 1): 0.4544,
+    (_HEAD, 0, (2, 4, 16), 0, 1): 0.2240,
+    (_HEAD, 0, (2, 4, 16), 512, 1): 0.1089,
+    (_HEAD, 0, (2, 4, 16), 2048, 1): 0.4615,
+    (_HEAD, 0, (0, 0, 0), 0, 1): 0.2018,
+    (_HEAD, 0, (0, 0, 0), 512, 1): 0.1124,
+    (_HEAD, 0, (0, 0, 0), 2048, 1): 0.4745,
+    (_HEAD, 2, (1, 2, 64), 0, 2048): 0.0019,
+    (_HEAD, 2, (1, 2, 64), 512, 2048): 0.0013,
+    (_HEAD, 2, (1, 2, 64), 2048, 2048): 0.0007,
+    (_HEAD, 2, (1, 1, 128), 0, 2048): 0.0019,
+    (_HEAD, 2, (1, 1, 128), 512, 2048): 0.0013,
+    (_HEAD, 2, (1, 1, 128), 2048, 2048): 0.0007,
+    (_HEAD, 2, (2, 2, 32), 0, 2048): 0.0019,
+    (_HEAD, 2, (2, 2, 32), 512, 2048): 0.0013,
+    (_HEAD, 2, (2, 2, 32), 2048, 2048): 0.0007,
+    (_HEAD, 2, (1, 4, 32), 0, 2048): 0.0019,
+    (_HEAD, 2, (1, 4, 32), 512, 2048): 0.0013,
+    (_HEAD, 2, (1, 4, 32), 2048, 2048): 0.0007,
+    (_HEAD, 2, (2, 4, 16), 0, 2048): 0.0019,
+    (_HEAD, 2, (2, 4, 16), 512, 2048): 0.0013,
+    (_HEAD, 2, (2, 4, 16), 2048, 2048): 0.0007,
+    (_HEAD, 2, (0, 0, 0), 0, 2048): 0.0019,
+    (_HEAD, 2, (0, 0, 0), 512, 2048): 0.0013,
+    (_HEAD, 2, (0, 0, 0), 2048, 2048): 0.0007,
+    (_LARGE10K, 2, (0, 0, 0), 0, 2048): 0.6564,
+}
 # The same per label read of ib_kernel (2-bit bricks), over 2.207e10 reads,
 # profiles/ib_kernel_bisection-256cube.md.
 IB_TRAFFIC_BYTES_PER_READ = {"plain-256^3-stride2": 0.0024}
