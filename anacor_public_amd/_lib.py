@@ -51,7 +51,8 @@ EXPORTED = (
     "anacor_batch_set_graph", "anacor_batch_last_graph",
     "anacor_mlp_create", "anacor_mlp_destroy", "anacor_mlp_num_params", "anacor_mlp_get_params",
     "anacor_mlp_set_params", "anacor_mlp_set_scale", "anacor_mlp_set_features",
-    "anacor_mlp_set_envelope", "anacor_mlp_set_bands", "anacor_mlp_features",
+    "anacor_mlp_set_envelope", "anacor_mlp_set_bands", "anacor_mlp_set_risk", "anacor_mlp_risk",
+    "anacor_risk_fit", "anacor_mlp_features",
     "anacor_mlp_forward",
     "anacor_mlp_gradients", "anacor_mlp_train", "anacor_mlp_predict", "anacor_mlp_stats",
     "ray_tracing_single_gridding", "nearest_neighbor_interpolate_overall", "anacor_grid_create",
@@ -95,6 +96,9 @@ class Band(ct.Structure):
 
 
 MLP_MAX_BANDS = 32
+RISK_FEATURES = 60
+RISK_MAX_DEPTH = 8
+RISK_MAX_TREES = 1024
 
 
 class MlpStats(ct.Structure):
@@ -178,6 +182,13 @@ def _prototypes(lib):
     lib.anacor_mlp_set_envelope.argtypes = [vp, ct.POINTER(Envelope), vp]
     lib.anacor_mlp_set_bands.restype = ct.c_int
     lib.anacor_mlp_set_bands.argtypes = [vp, ct.POINTER(Band), i32]
+    lib.anacor_mlp_set_risk.restype = ct.c_int
+    lib.anacor_mlp_set_risk.argtypes = [vp, i32, i32, vp, vp, vp, ct.c_float, ct.c_float]
+    lib.anacor_mlp_risk.restype = ct.c_int
+    lib.anacor_mlp_risk.argtypes = [vp, vp, i64, vp, vp]
+    lib.anacor_risk_fit.restype = ct.c_int
+    lib.anacor_risk_fit.argtypes = [vp, vp, vp, i64, i32, i32, i32, ct.c_float, ct.c_float,
+                                    ct.c_float, i32, i32, vp, vp, vp, ct.POINTER(ct.c_float)]
     lib.anacor_mlp_features.restype = ct.c_int
     lib.anacor_mlp_features.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, vp]
     lib.anacor_mlp_forward.restype = ct.c_int

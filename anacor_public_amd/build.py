@@ -2,8 +2,9 @@
 
     python -m anacor_public_amd.build
 
-Host geometry (csrc/geometry.cpp) is compiled with g++ against glibc libm and
-without FMA contraction, so its FP64 results carry the reference's bits; the
+Host code (csrc/geometry.cpp, csrc/risk_fit.cpp) is compiled with g++ against
+glibc libm and without FMA contraction, so the geometry's FP64 results carry
+the reference's bits; the
 HIP kernels and the C-ABI (csrc/trace.hip, csrc/mlp.hip) are compiled with hipcc for gfx950,
 also with -ffp-contract=off (bit-exact stepping) and packed-FP32 code
 generation disabled.
@@ -37,6 +38,7 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contr
 
 
 HIP_SOURCES = ("trace.hip", "mlp.hip", "probe.hip")   # tracer + C-ABI; surrogate; gather probe
+HOST_SOURCES = ("geometry.cpp", "risk_fit.cpp")       # FP64 geometry; risk-tree trainer
 
 
 def _run(cmd):
@@ -63,13 +65,13 @@ def build(verbose: bool = True, sanitize: bool = False) -> str:
     os.makedirs(out_dir, exist_ok=True)
     os.makedirs(lib_dir, exist_ok=True)
     lib = os.path.join(lib_dir, "libanacor_mi355x.so")
-    geo_o = os.path.join(out_dir, "geometry.o")
-    if sanitize:
-        _run([SAN_CXX, *HOST_FLAGS, *SAN, "-c", os.path.join(CSRC, "geometry.cpp"), "-o", geo_o])
-    else:
-        _run([CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, "geometry.cpp"), "-o", geo_o])
+    objs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(out_dir, os.path.splitext(src)[0] + ".o")
+        cxx = [SAN_CXX, *HOST_FLAGS, *SAN] if sanitize else [CXX, *HOST_FLAGS]
+        _run([*cxx, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     host_san = [a for f in SAN for a in ("-Xarch_host", f)] if sanitize else []
-    objs = [geo_o]
     for src in HIP_SOURCES:
         obj = os.path.join(out_dir, os.path.splitext(src)[0] + ".o")
         _run([HIPCC, *HIP_FLAGS, *host_san, "-I", os.path.join(ROOT, "include"), "-c",

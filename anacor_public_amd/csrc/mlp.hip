@@ -733,6 +733,89 @@ __global__ void features_kernel(const double* __restrict__ s1, const double* __r
   }
 }
 
+// ------------------------------------------------------------- risk trees
+//
+// A boosted ensemble of complete binary trees over 60 resonance features of
+// the two unit vectors (layout in anacor_mi355x.h): the ratios t of two
+// components of a ray and t's distance to the nearest p / q, q <= 8.  Near
+// such ratios the voxel steps of all crystal voxels fall into step together
+// and the traced A_h carries a moire the network does not resolve; the trees
+// score where its error is likely to exceed the bound, and rows scoring at
+// least `cut` leave the envelope (the tracer takes them).  The trees are
+// fitted on the host (risk_fit.cpp) to the training rows' errors.
+//
+// Every feature is one correctly rounded FP32 operation chain (|.|, min, max,
+// one division, q * t, rint, an exact subtraction, one multiplication by the
+// rounded 1/q), so a host restatement in FP32 gives the same bits.
+
+constexpr int kRiskF = ANACOR_RISK_FEATURES;
+static_assert(kRiskF == 60, "risk feature layout");
+
+struct RiskNode {
+  float thr;
+  int feat;
+};
+
+struct RiskArgs {
+  const RiskNode* nodes;   // [n_trees][n_int]
+  const float* leaf;       // [n_trees][n_int + 1]
+  int n_trees, depth, n_int;
+  float base, cut;
+};
+
+__device__ __forceinline__ void risk_features(const float* v, float* o, int stride) {
+  constexpr float inv_q[8] = {1.0f, 1.0f / 2.0f, 1.0f / 3.0f, 1.0f / 4.0f,
+                              1.0f / 5.0f, 1.0f / 6.0f, 1.0f / 7.0f, 1.0f / 8.0f};
+  constexpr int pi[3] = {0, 0, 1}, pj[3] = {1, 2, 2};
+  for (int k = 0; k < 6; ++k) o[k * stride] = v[k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int r = k / 3, p = k % 3;
+    const float a = fabsf(v[3 * r + pi[p]]), b = fabsf(v[3 * r + pj[p]]);
+    const float hi = fmaxf(a, b);
+    const float t = hi > 0.0f ? fminf(a, b) / hi : 0.0f;
+    o[(6 + k) * stride] = t;
+#pragma unroll
+    for (int q = 1; q <= 8; ++q) {
+      const float u = (float)q * t;
+      o[(12 + 8 * k + q - 1) * stride] = fabsf(u - rintf(u)) * inv_q[q - 1];
+    }
+  }
+}
+
+// One thread per row of network inputs X[n][12]: the features into LDS
+// (column per thread, so the 64 lanes of a wave read 64 consecutive banks),
+// optionally out to feat_out[n][60]; with trees the score (score[n],
+// nullable) and flag[i] cleared when score >= cut (flag nullable).  No thread
+// reads another's column, so the kernel has no barrier.
+__global__ __launch_bounds__(256) void risk_kernel(const float* __restrict__ X, int64_t n,
+                                                   RiskArgs r, float* __restrict__ feat_out,
+                                                   float* __restrict__ score,
+                                                   uint8_t* __restrict__ flag) {
+  __shared__ float lds[kRiskF * 256];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float v[6];
+  for (int k = 0; k < 6; ++k) v[k] = X[i * kIn + k];
+  float* col = lds + threadIdx.x;
+  risk_features(v, col, 256);
+  if (feat_out)
+    for (int k = 0; k < kRiskF; ++k) feat_out[i * kRiskF + k] = col[k * 256];
+  if (r.n_trees <= 0) return;
+  float s = r.base;
+  for (int t = 0; t < r.n_trees; ++t) {
+    const RiskNode* nd = r.nodes + (size_t)t * r.n_int;
+    int j = 0;
+    for (int d = 0; d < r.depth; ++d) {
+      const RiskNode e = nd[j];
+      j = 2 * j + 1 + (col[e.feat * 256] > e.thr ? 1 : 0);
+    }
+    s += r.leaf[(size_t)t * (r.n_int + 1) + (j - r.n_int)];
+  }
+  if (score) score[i] = s;
+  if (flag && s >= r.cut) flag[i] = 0;
+}
+
 // ---------------------------------------------------------------- model
 
 template <typename T>
@@ -777,6 +860,10 @@ struct Model {
   anacor_envelope_t env{};
   int n_bands = 0;
   anacor_band_t bands[ANACOR_MLP_MAX_BANDS] = {};
+  DevBuf<RiskNode> r_nodes;       // risk trees (none while r_trees == 0)
+  DevBuf<float> r_leaf, r_feat, r_score;
+  int r_trees = 0, r_depth = 0;
+  float r_base = 0.f, r_cut = 0.f;
   double y_mean = 0.0, y_std = 1.0;
   long long adam_t = 0;
   double last_mlp_ms = 0.0, last_total_ms = 0.0;
@@ -970,6 +1057,18 @@ static int geo_from(const Model* M, const double* raw, const double* axis, const
   return ANACOR_OK;
 }
 
+static RiskArgs risk_args(const Model* M) {
+  RiskArgs r{};
+  r.nodes = M->r_nodes.p;
+  r.leaf = M->r_leaf.p;
+  r.n_trees = M->r_trees;
+  r.depth = M->r_depth;
+  r.n_int = (1 << M->r_depth) - 1;
+  r.base = M->r_base;
+  r.cut = M->r_cut;
+  return r;
+}
+
 // s1/omega of R reflections to the device and the features kernel over them
 // (inv and cell only when requested).
 static int run_features(Model* M, const double* s1, const double* omega, int64_t R,
@@ -987,6 +1086,9 @@ static int run_features(Model* M, const double* s1, const double* omega, int64_t
   hipLaunchKernelGGL(features_kernel, dim3(nblocks(R, 256)), dim3(256), 0, M->stream, M->s1.p,
                      M->omega.p, R, g, M->X.p, want_inv ? M->inv.p : nullptr,
                      want_cell ? M->cell.p : nullptr, M->flag.p);
+  if (g.use_env && M->r_trees > 0)   // rows the trees score at or above the cut leave
+    hipLaunchKernelGGL(risk_kernel, dim3(nblocks(R, 256)), dim3(256), 0, M->stream, M->X.p, R,
+                       risk_args(M), (float*)nullptr, (float*)nullptr, M->flag.p);
   MLP_TRY(hipGetLastError());
   return ANACOR_OK;
 }
@@ -1080,6 +1182,7 @@ void anacor_mlp_destroy(anacor_mlp* mlp) {
     M->s1.release(); M->omega.release(); M->inv.release(); M->A.release();
     M->X.release(); M->y.release(); M->T.release(); M->flag.release(); M->trust.release();
     M->cell.release();
+    M->r_nodes.release(); M->r_leaf.release(); M->r_feat.release(); M->r_score.release();
     for (hipEvent_t e : M->ev)
       if (e) (void)hipEventDestroy(e);
     if (M->stream) (void)hipStreamDestroy(M->stream);
@@ -1200,6 +1303,85 @@ int anacor_mlp_set_bands(anacor_mlp* mlp, const anacor_band_t* bands, int n) {
   M->n_bands = n;
   std::memset(M->bands, 0, sizeof(M->bands));
   if (n > 0) std::memcpy(M->bands, bands, (size_t)n * sizeof(anacor_band_t));
+  return ANACOR_OK;
+}
+
+int anacor_mlp_set_risk(anacor_mlp* mlp, int n_trees, int depth, const uint8_t* feat,
+                        const float* thr, const float* leaf, float base, float cut) {
+  Model* M = model_of(mlp);
+  if (!M || n_trees < 0 || n_trees > ANACOR_RISK_MAX_TREES ||
+      (n_trees > 0 && (depth < 1 || depth > ANACOR_RISK_MAX_DEPTH || !feat || !thr || !leaf ||
+                       !std::isfinite(base) || std::isnan(cut)))) {
+    set_error(ANACOR_ERR_ARG, "mlp set_risk: null model or bad trees (%d trees, depth %d)",
+              n_trees, depth);
+    return ANACOR_ERR_ARG;
+  }
+  const int n_int = n_trees > 0 ? (1 << depth) - 1 : 0;
+  std::vector<RiskNode> nodes((size_t)n_trees * n_int);
+  for (size_t k = 0; k < nodes.size(); ++k) {
+    if (feat[k] >= kRiskF || std::isnan(thr[k])) {
+      set_error(ANACOR_ERR_ARG, "mlp set_risk: node %zu: feature %d (of %d) or NaN threshold",
+                k, (int)feat[k], kRiskF);
+      return ANACOR_ERR_ARG;
+    }
+    nodes[k].thr = thr[k];
+    nodes[k].feat = feat[k];
+  }
+  for (size_t k = 0; k < (size_t)n_trees * (n_int + 1); ++k)
+    if (!std::isfinite(leaf[k])) {
+      set_error(ANACOR_ERR_ARG, "mlp set_risk: leaf %zu is not finite", k);
+      return ANACOR_ERR_ARG;
+    }
+  std::lock_guard<std::mutex> lk(M->mu);
+  MLP_TRY(hipSetDevice(M->device));
+  if (n_trees > 0) {
+    MLP_CHECK(M->r_nodes.grow(nodes.size()));
+    MLP_CHECK(M->r_leaf.grow((size_t)n_trees * (n_int + 1)));
+    MLP_TRY(hipMemcpyAsync(M->r_nodes.p, nodes.data(), nodes.size() * sizeof(RiskNode),
+                           hipMemcpyHostToDevice, M->stream));
+    MLP_TRY(hipMemcpyAsync(M->r_leaf.p, leaf, (size_t)n_trees * (n_int + 1) * sizeof(float),
+                           hipMemcpyHostToDevice, M->stream));
+    MLP_TRY(hipStreamSynchronize(M->stream));
+  }
+  M->r_trees = n_trees;
+  M->r_depth = n_trees > 0 ? depth : 0;
+  M->r_base = base;
+  M->r_cut = cut;
+  return ANACOR_OK;
+}
+
+int anacor_mlp_risk(anacor_mlp* mlp, const float* x, int64_t n, float* features,
+                    float* scores) {
+  Model* M = model_of(mlp);
+  if (!M || n < 0 || n > ((int64_t)1 << 30) || (n > 0 && !x)) {
+    set_error(ANACOR_ERR_ARG, "mlp risk: bad arguments");
+    return ANACOR_ERR_ARG;
+  }
+  if (n == 0) return ANACOR_OK;
+  std::lock_guard<std::mutex> lk(M->mu);
+  if (scores && M->r_trees <= 0) {
+    set_error(ANACOR_ERR_ARG, "mlp risk: scores asked for but no trees are set");
+    return ANACOR_ERR_ARG;
+  }
+  MLP_TRY(hipSetDevice(M->device));
+  MLP_CHECK(M->X.grow((size_t)n * kIn));
+  if (features) MLP_CHECK(M->r_feat.grow((size_t)n * kRiskF));
+  if (scores) MLP_CHECK(M->r_score.grow((size_t)n));
+  MLP_TRY(hipMemcpyAsync(M->X.p, x, (size_t)n * kIn * sizeof(float), hipMemcpyHostToDevice,
+                         M->stream));
+  RiskArgs r = risk_args(M);
+  if (!scores) r.n_trees = 0;
+  hipLaunchKernelGGL(risk_kernel, dim3(nblocks(n, 256)), dim3(256), 0, M->stream, M->X.p, n, r,
+                     features ? M->r_feat.p : nullptr, scores ? M->r_score.p : nullptr,
+                     (uint8_t*)nullptr);
+  MLP_TRY(hipGetLastError());
+  if (features)
+    MLP_TRY(hipMemcpyAsync(features, M->r_feat.p, (size_t)n * kRiskF * sizeof(float),
+                           hipMemcpyDeviceToHost, M->stream));
+  if (scores)
+    MLP_TRY(hipMemcpyAsync(scores, M->r_score.p, (size_t)n * sizeof(float),
+                           hipMemcpyDeviceToHost, M->stream));
+  MLP_TRY(hipStreamSynchronize(M->stream));
   return ANACOR_OK;
 }
 

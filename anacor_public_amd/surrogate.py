@@ -125,6 +125,34 @@ class MLP:
             arr[k].ratio, arr[k].width = float(ratio), float(width)
         self._check(self._lib.anacor_mlp_set_bands(self.handle, arr, n))
 
+    def set_risk(self, trees: Optional[dict], cut: float = float("inf")) -> None:
+        """Risk trees (risk_fit's dict) with rows scoring >= cut sent to the
+        tracer; None removes them."""
+        if trees is None:
+            self._check(self._lib.anacor_mlp_set_risk(self.handle, 0, 0, None, None, None,
+                                                      0.0, 0.0))
+            return
+        feat = np.ascontiguousarray(trees["feat"], dtype=np.uint8)
+        thr = _f32(trees["thr"])
+        leaf = _f32(trees["leaf"])
+        D = int(trees["depth"])
+        T = feat.shape[0]
+        if feat.shape != (T, (1 << D) - 1) or thr.shape != feat.shape or \
+                leaf.shape != (T, 1 << D):
+            raise ValueError("risk trees: inconsistent array shapes")
+        self._check(self._lib.anacor_mlp_set_risk(self.handle, T, D, _p(feat), _p(thr),
+                                                  _p(leaf), float(trees["base"]), float(cut)))
+
+    def risk(self, X, scores: bool = True):
+        """(features float32[n][60], scores float32[n] or None) of rows of
+        network inputs, on the GPU; scores need trees set."""
+        X = _f32(X).reshape(-1, 12)
+        Fr = np.empty((len(X), _lib.RISK_FEATURES), dtype=np.float32)
+        sc = np.empty(len(X), dtype=np.float32) if scores else None
+        self._check(self._lib.anacor_mlp_risk(self.handle, _p(X), len(X), _p(Fr),
+                                              _p(sc) if sc is not None else None))
+        return Fr, sc
+
     def features(self, s1, omega, raw_xray, omega_axis, F, cells: bool = False,
                  flags: bool = False):
         """X float32[R][12], invariants float64[R][4] (axial, two_theta, alpha,
@@ -259,6 +287,32 @@ def in_bands(bands, x: np.ndarray, s: np.ndarray) -> np.ndarray:
     return out
 
 
+def risk_fit(features: np.ndarray, labels: np.ndarray, weights: Optional[np.ndarray] = None,
+             n_trees: int = 200, depth: int = 6, learning_rate: float = 0.1, l2: float = 1.0,
+             min_hess: float = 1e-3, n_bins: int = 256, threads: int = 0) -> dict:
+    """Risk trees fitted on the host by the library (anacor_risk_fit):
+    Newton-boosted complete trees of the logistic loss of labels (0/1) over
+    feature rows [n][F].  Returns feat / thr [n_trees][2^depth - 1], leaf
+    [n_trees][2^depth], base and depth."""
+    lib = _lib.load()
+    Fm = _f32(features)
+    n, nf = Fm.shape
+    y = np.ascontiguousarray(labels, dtype=np.uint8).reshape(-1)
+    if len(y) != n:
+        raise ValueError("one label per feature row")
+    w = None if weights is None else _f32(weights).reshape(-1)
+    n_int = (1 << depth) - 1
+    feat = np.zeros((n_trees, n_int), dtype=np.uint8)
+    thr = np.zeros((n_trees, n_int), dtype=np.float32)
+    leaf = np.zeros((n_trees, n_int + 1), dtype=np.float32)
+    base = ct.c_float()
+    _lib.check(lib.anacor_risk_fit(_p(Fm), _p(y), None if w is None else _p(w), n, nf, n_trees,
+                                   depth, learning_rate, l2, min_hess, n_bins, threads,
+                                   _p(feat), _p(thr), _p(leaf), ct.byref(base)), lib)
+    return {"feat": feat, "thr": thr, "leaf": leaf, "base": float(base.value),
+            "depth": int(depth)}
+
+
 def crystal_axis(omega_axis, F) -> np.ndarray:
     """The goniometer axis in the crystal frame, unit F^T omega_axis (the
     rotation about omega_axis leaves it fixed)."""
@@ -300,6 +354,8 @@ class Surrogate:
         self.envelope: Optional[_lib.Envelope] = None
         self.trust: Optional[np.ndarray] = None
         self.bands: list = []
+        self.risk: Optional[dict] = None
+        self.risk_cut = float("inf")
         self.bounded = False
         self.y_mean = 0.0
         self.y_std = 1.0
@@ -326,30 +382,46 @@ class Surrogate:
     def fit(self, s1, omega, epochs: int = 600, batch: int = 1024, lr: float = 1e-3,
             lr_min: float = 1e-6, seed: int = 0, targets: Optional[np.ndarray] = None,
             cells=(64, 8, 32), tau: Optional[float] = None, min_count: int = 4,
-            axial_tol: float = 1e-9, bound: Optional[float] = 1e-3,
-            band_target: float = 0.85, band_grow: float = 1.1) -> dict:
+            axial_tol: float = 1e-9, bound: Optional[float] = 1e-3, trust: str = "trees",
+            band_target: float = 0.85, band_grow: float = 1.1, risk_trees: int = 200,
+            risk_depth: int = 6, risk_folds: int = 5, risk_label: float = 0.5,
+            risk_cut: float = 0.9, risk_quantile: float = 0.01,
+            risk_keep: float = 0.3) -> dict:
         """Trace the training reflections (unless targets are given), train
         with Adam, and set the envelope: the training axial and two-theta
         ranges, and trust only in the (alpha, two_theta, beta) cells holding at
         least min_count training reflections (and, with tau, whose training
         relative errors are all at most tau).
 
-        With a bound, resonance bands are fitted to the training errors
-        (fit_bands with target band_target * bound, widths grown by
-        band_grow) and the surrogate starts in bounded mode: reflections in
-        a band go to the tracer (set_bounded(False) serves them from the
-        network again)."""
+        With a bound the surrogate starts in bounded mode, in which the
+        reflections where the network's error is likely to exceed the bound
+        go to the tracer:
+        * trust="trees" (default): risk trees (risk_fit) over the resonance
+          features learn where the training errors exceed risk_label * bound;
+          each training row is scored by trees fitted without its fold
+          (risk_folds folds, negatives subsampled to risk_keep and weighted
+          back), and the cut is the risk_quantile quantile of those scores
+          over the rows whose error exceeds risk_cut * bound; the trees set
+          on the GPU are refitted on every row.
+        * trust="bands": resonance bands (fit_bands, target band_target *
+          bound, widths grown by band_grow).
+        set_bounded(False) lets the network answer every row of its envelope."""
         t0 = time.perf_counter()
         A = self.trace(s1, omega) if targets is None else _f64(targets)
         t_trace = time.perf_counter() - t0
         ok = np.isfinite(A) & (A > 0)
         if not np.all(ok):
             raise ValueError(f"{int((~ok).sum())} training reflections have no finite A_h > 0")
+        if trust not in ("trees", "bands"):
+            raise ValueError(f"trust must be 'trees' or 'bands', not {trust!r}")
         y = np.log(A)
         self.y_mean = float(y.mean())
         self.y_std = float(max(y.std(), 1e-12))
         self.mlp.set_scale(self.y_mean, self.y_std)
         self.mlp.set_envelope(None)
+        self.mlp.set_bands([])
+        self.mlp.set_risk(None)
+        self.bands, self.risk, self.risk_cut = [], None, float("inf")
         _, inv, _, _ = self.mlp.features(s1, omega, *self._geom())
         env = make_envelope(crystal_axis(self.omega_axis, self.F),
                             (inv[:, 0].min() - axial_tol, inv[:, 0].max() + axial_tol),
@@ -369,37 +441,76 @@ class Surrogate:
         ok = count >= min_count
         if tau is not None:
             ok &= worst <= tau
-        bands, left = [], np.zeros(0, dtype=np.int64)
-        if bound is not None:
+        rep_trust = {}
+        left = np.zeros(0, dtype=np.int64)
+        t2 = time.perf_counter()
+        if bound is not None and trust == "bands":
             x64, s64 = X[:, 0:3].astype(np.float64), X[:, 3:6].astype(np.float64)
-            bands, left = fit_bands(x64, s64, res, band_target * bound, grow=band_grow)
+            self.bands, left = fit_bands(x64, s64, res, band_target * bound, grow=band_grow)
             ok[cell[left]] = False   # errors no band covers: their cells go to the tracer
-        trust = ok.astype(np.uint8)
-        self.mlp.set_envelope(env, trust)
-        self.envelope, self.trust = env, trust
-        self.bands = bands
+            band_out = (in_bands(self.bands, x64, s64) if self.bands
+                        else np.zeros(len(y), dtype=bool))
+            rep_trust = {"bands": [list(b) for b in self.bands],
+                         "band_train_fallback": float(band_out.mean()),
+                         "band_uncovered": int(len(left))}
+        elif bound is not None:
+            rep_trust = self._fit_risk(X, res, bound, seed, risk_trees, risk_depth, risk_folds,
+                                       risk_label, risk_cut, risk_quantile, risk_keep)
+        t_trust = time.perf_counter() - t2
+        trust_tab = ok.astype(np.uint8)
+        self.mlp.set_envelope(env, trust_tab)
+        self.envelope, self.trust = env, trust_tab
         self.set_bounded(bound is not None)
-        served = trust[cell].astype(bool)
-        band_out = (in_bands(bands, X[:, 0:3].astype(np.float64), X[:, 3:6].astype(np.float64))
-                    if bands else np.zeros(len(y), dtype=bool))
+        served = trust_tab[cell].astype(bool)
         return {"n_train": int(len(y)), "trace_s": t_trace, "train_s": t_train,
-                "epochs": int(epochs), "batch": int(batch), "lr": lr, "lr_min": lr_min,
-                "final_loss": float(losses[-1]), "losses": losses,
+                "trust_fit_s": t_trust, "epochs": int(epochs), "batch": int(batch), "lr": lr,
+                "lr_min": lr_min, "final_loss": float(losses[-1]), "losses": losses,
                 "y_mean": self.y_mean, "y_std": self.y_std,
                 "two_theta_deg": (float(np.degrees(env.two_theta[0])),
                                   float(np.degrees(env.two_theta[1]))),
                 "train_rel_max": float(res.max()), "train_rel_mean": float(res.mean()),
                 "cells": list(cells), "tau": tau, "min_count": min_count,
-                "trusted_cells": float(trust.mean()), "train_served": float(served.mean()),
-                "bound": bound, "bands": [list(b) for b in bands],
-                "band_train_fallback": float(band_out.mean()),
-                "band_uncovered": int(len(left))}
+                "trusted_cells": float(trust_tab.mean()), "train_served": float(served.mean()),
+                "bound": bound, "trust": trust if bound is not None else None, **rep_trust}
+
+    def _fit_risk(self, X, res, bound, seed, n_trees, depth, folds, label, cut_frac, quantile,
+                  keep) -> dict:
+        """Risk trees and their cut (see fit); returns the report entries."""
+        Fr, _ = self.mlp.risk(X, scores=False)
+        n = len(res)
+        rng = np.random.default_rng([int(seed), 7919])
+        fold = rng.integers(0, folds, size=n)
+        drop = rng.random(n)                       # negative subsampling draw per row
+        pos = res > label * bound
+
+        def fit_on(rows):
+            m = rows[pos[rows] | (drop[rows] < keep)]
+            w = np.where(pos[m], 1.0, 1.0 / keep).astype(np.float32)
+            return risk_fit(Fr[m], pos[m].astype(np.uint8), w, n_trees=n_trees, depth=depth)
+
+        cv = np.empty(n, dtype=np.float32)
+        all_rows = np.arange(n)
+        for k in range(folds):
+            self.mlp.set_risk(fit_on(all_rows[fold != k]))
+            _, cv[fold == k] = self.mlp.risk(X[fold == k])
+        bad = np.sort(cv[res > cut_frac * bound])
+        cut = float(bad[int(quantile * len(bad))]) if len(bad) else float("inf")
+        self.risk = fit_on(all_rows)
+        self.risk_cut = cut
+        self.mlp.set_risk(None)
+        return {"risk_trees": int(n_trees), "risk_depth": int(depth), "risk_cut": cut,
+                "risk_positives": int(pos.sum()),
+                "risk_train_fallback": float(np.mean(cv >= cut)),
+                "risk_train_missed": int(np.sum((cv < cut) & (res > bound)))}
 
     def set_bounded(self, bounded: bool) -> None:
-        """Bounded mode: the fitted resonance bands send their reflections to
-        the tracer.  Fast mode: the network answers them too."""
-        self.mlp.set_bands(self.bands if bounded else [])
-        self.bounded = bool(bounded) and bool(self.bands)
+        """Bounded mode: the risk trees (or resonance bands) send their
+        reflections to the tracer.  Fast mode: the network answers every
+        reflection of its envelope."""
+        on = bool(bounded)
+        self.mlp.set_bands(self.bands if on else [])
+        self.mlp.set_risk(self.risk if on and self.risk is not None else None, self.risk_cut)
+        self.bounded = on and (bool(self.bands) or self.risk is not None)
 
     def predict(self, s1, omega, fallback: bool = True):
         """(A float64[R], used bool[R]): the network's A_h where it may answer
@@ -422,7 +533,14 @@ class Surrogate:
                 "cells": np.array(e.cells[:]), "trust": self.trust,
                 "feature_set": self.feature_set,
                 "bands": np.array(self.bands, dtype=np.float64).reshape(-1, 5),
-                "bounded": self.bounded}
+                "bounded": self.bounded, **self._risk_state()}
+
+    def _risk_state(self) -> dict:
+        if self.risk is None:
+            return {"risk_depth": 0}
+        r = self.risk
+        return {"risk_feat": r["feat"], "risk_thr": r["thr"], "risk_leaf": r["leaf"],
+                "risk_base": r["base"], "risk_depth": r["depth"], "risk_cut": self.risk_cut}
 
     def load_state(self, st: dict) -> None:
         self.set_feature_set(int(st["feature_set"]))
@@ -442,4 +560,13 @@ class Surrogate:
         self.mlp.set_envelope(env, self.trust)
         self.bands = [(int(b[0]), int(b[1]), int(b[2]), float(b[3]), float(b[4]))
                       for b in np.asarray(st.get("bands", np.zeros((0, 5)))).reshape(-1, 5)]
+        D = int(st.get("risk_depth", 0))
+        if D > 0:
+            self.risk = {"feat": np.asarray(st["risk_feat"], dtype=np.uint8),
+                         "thr": np.asarray(st["risk_thr"], dtype=np.float32),
+                         "leaf": np.asarray(st["risk_leaf"], dtype=np.float32),
+                         "base": float(st["risk_base"]), "depth": D}
+            self.risk_cut = float(st["risk_cut"])
+        else:
+            self.risk, self.risk_cut = None, float("inf")
         self.set_bounded(bool(st.get("bounded", False)))

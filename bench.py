@@ -236,18 +236,20 @@ def gridding_baseline(tracer, sw, geo, A_true, steps: int, n_theta: int = 360,
 
 def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int = 600,
                   cpu_budget: float = 0.0):
-    """Fit the surrogate, judge it on the held-out set, time 1M inferences
-    (features, envelope test, MLP on MFMA, tracer fallback) and the plain GPU
-    path on the same sample.  Two operating points: bounded (the resonance
-    bands fitted on the training errors send their reflections to the
-    tracer, so the network answers only where it met the 1e-3 bound) and
-    fast (the network answers every reflection inside the training range).
-    Returns the surrogate record, bounded mode at its top level."""
+    """Fit the surrogate, judge it on the held-out set (10k reflections) and
+    on an audit set (200k more, its own stream) for the tail of the error,
+    time 1M inferences (features, envelope and risk-tree tests, MLP on MFMA,
+    tracer fallback) and the plain GPU path on the same sample.  Two
+    operating points: bounded (risk trees fitted on the training errors send
+    the reflections where the network's error is likely to exceed the 1e-3
+    bound to the tracer) and fast (the network answers every reflection
+    inside the training range).  Returns the surrogate record, bounded mode
+    at its top level."""
     from anacor_public_amd import workload as W
     from anacor_public_amd.ray_tracing import Tracer
     from anacor_public_amd.surrogate import Surrogate
     t0 = time.perf_counter()
-    sw = W.surrogate_config(n_infer=n_infer)
+    sw = W.surrogate_config(n_infer=n_infer, n_audit=200_000)
     wl = sw.sample
     geo = (wl.raw_xray, wl.omega_axis, wl.F)
     log(f"# surrogate workload {wl.name}: N={len(wl.coords)}, train {len(wl.omega)}, "
@@ -264,14 +266,21 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
         plain_rate = 3 * len(pom) / (time.perf_counter() - t1)
         rep = sur.fit(wl.s1, wl.omega, epochs=epochs, bound=SURROGATE_BOUND)
         log(f"# surrogate fit: trace {rep['trace_s']:.2f}s, train {rep['train_s']:.1f}s "
-            f"({epochs} epochs), final loss {rep['final_loss']:.3e}, {len(rep['bands'])} "
-            f"resonance bands ({rep['band_train_fallback'] * 100:.1f}% of training reflections)")
+            f"({epochs} epochs), final loss {rep['final_loss']:.3e}, risk trees "
+            f"{rep['trust_fit_s']:.1f}s (cross-fitted fallback "
+            f"{rep['risk_train_fallback'] * 100:.1f}% of training reflections)")
         A_true = tr.run(sw.heldout_s1, sw.heldout_omega, *geo)
+        A_audit = tr.run(sw.audit_s1, sw.audit_omega, *geo)
 
         def mode(bounded: bool) -> dict:
             sur.set_bounded(bounded)
             A_sur, used = sur.predict(sw.heldout_s1, sw.heldout_omega)
             err = rel_stats(A_sur[used], A_true[used])
+            A_a, used_a = sur.predict(sw.audit_s1, sw.audit_omega, fallback=False)
+            ra = np.abs(A_a[used_a] - A_audit[used_a]) / A_audit[used_a]
+            audit = {"reflections": int(len(used_a)), "answered": int(used_a.sum()),
+                     "rel_error": rel_stats(A_a[used_a], A_audit[used_a]),
+                     "above_bound": int(np.sum(ra > SURROGATE_BOUND))}
             for _ in range(warmup):
                 sur.predict(sw.infer_s1, sw.infer_omega)
             mlp_ms, fb = [], 0
@@ -284,12 +293,14 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
             value = n_infer * steps / elapsed
             out = {"value": value, "ms_per_step": 1e3 * elapsed / steps,
                    "heldout_answered": int(used.sum()), "heldout_rel_error": err,
-                   "fallback_reflections": fb, "speedup_vs_plain_gpu": value / plain_rate,
+                   "audit": audit, "fallback_reflections": fb,
+                   "fallback_fraction": fb / n_infer,
+                   "speedup_vs_plain_gpu": value / plain_rate,
                    "mlp_ms": float(np.mean(mlp_ms))}
             name = "bounded" if bounded else "fast"
             log(f"# surrogate {name}: held-out ({out['heldout_answered']}/{len(used)} answered) "
-                f"{err}; {value:.3e} A_h/s ({out['speedup_vs_plain_gpu']:.1f}x the plain GPU "
-                f"path's {plain_rate:.3e}), fallback {fb}")
+                f"{err}; audit {audit}; {value:.3e} A_h/s ({out['speedup_vs_plain_gpu']:.1f}x "
+                f"the plain GPU path's {plain_rate:.3e}), fallback {fb}")
             return out
 
         fast = mode(False)
@@ -302,19 +313,22 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
             "bound": SURROGATE_BOUND, "dtype": "bf16", "workload": wl.name,
             "n_voxels": int(len(wl.coords)), "inferences": n_infer,
             "fallback_reflections": bounded["fallback_reflections"],
+            "fallback_fraction": bounded["fallback_fraction"],
             "training_pairs": int(len(wl.omega)), "epochs": epochs,
             "train_s": rep["train_s"], "train_trace_s": rep["trace_s"],
-            "resonance_bands": rep["bands"],
-            "band_training_fallback": rep["band_train_fallback"],
+            "trust": {k: rep[k] for k in ("trust", "risk_trees", "risk_depth", "risk_cut",
+                                          "risk_train_fallback", "risk_train_missed")},
+            "trust_fit_s": rep["trust_fit_s"],
             "heldout": int(len(sw.heldout_omega)),
             "heldout_answered": bounded["heldout_answered"],
             "heldout_rel_error": bounded["heldout_rel_error"],
+            "audit": bounded["audit"],
             "plain_gpu_same_sample": plain_rate,
             "speedup_vs_plain_gpu": bounded["speedup_vs_plain_gpu"],
             "mlp_only_rate": n_infer / (mms * 1e-3),
             "fast": {k: fast[k] for k in ("value", "ms_per_step", "heldout_answered",
-                                          "heldout_rel_error", "fallback_reflections",
-                                          "speedup_vs_plain_gpu")},
+                                          "heldout_rel_error", "audit", "fallback_reflections",
+                                          "fallback_fraction", "speedup_vs_plain_gpu")},
             "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS,
                          "traffic": None, "kernel": "MLP forward (HIP events)",

@@ -410,6 +410,53 @@ typedef struct {
 #define ANACOR_MLP_MAX_BANDS 32
 int anacor_mlp_set_bands(anacor_mlp *mlp, const anacor_band_t *bands, int n);
 
+/* Risk trees: a boosted ensemble of complete binary decision trees over 60
+ * resonance features of a reflection, computed in FP32 from the network
+ * inputs x = features[0..2], s = features[3..5]:
+ *   [0, 6)    x0 x1 x2 s0 s1 s2
+ *   [6, 12)   t = min(|v_i|, |v_j|) / max(|v_i|, |v_j|) (0 when both are 0) for
+ *             (v, (i, j)) = (x, (0,1)), (x, (0,2)), (x, (1,2)), (s, (0,1)), (s, (0,2)),
+ *             (s, (1,2)), in that order
+ *   [12, 60)  for each of those six ratios k and q = 1..8:
+ *             |q t - rint(q t)| * (1 / q) at index 12 + 8 k + q - 1
+ *             (the distance of t to the nearest fraction p / q)
+ * Tree `t` of depth D has 2^D - 1 internal nodes in level order and 2^D
+ * leaves: from node 0, node = 2 node + 1 + (f[feat[node]] > thr[node]),
+ * and after D levels the leaf is node - (2^D - 1).  score = base + the sum of
+ * the trees' leaves in tree order (FP32).  A reflection whose score is at
+ * least `cut` lies outside the envelope and goes to the tracer: the trees
+ * estimate where the network's error exceeds the surrogate's bound (the
+ * moire of the voxel steps near rational direction ratios).  No counterpart
+ * in the reference. */
+#define ANACOR_RISK_FEATURES  60
+#define ANACOR_RISK_MAX_DEPTH 8
+#define ANACOR_RISK_MAX_TREES 1024
+
+/* Arrays feat[n_trees][2^depth - 1] (feature indices), thr[n_trees][2^depth - 1],
+ * leaf[n_trees][2^depth]; n_trees = 0 removes the trees. */
+int anacor_mlp_set_risk(anacor_mlp *mlp, int n_trees, int depth, const uint8_t *feat,
+                        const float *thr, const float *leaf, float base, float cut);
+
+/* The 60 resonance features (out[n][60]) and the trees' scores (scores[n],
+ * the trees set by anacor_mlp_set_risk; nullable) of n rows of network
+ * inputs x[n][12], on the GPU. */
+int anacor_mlp_risk(anacor_mlp *mlp, const float *x, int64_t n, float *features,
+                    float *scores);
+
+/* Fits risk trees on the host (no GPU): Newton-boosted trees of the logistic
+ * loss of labels[n] (0/1) over features[n][n_features] (row-major), n_bins
+ * (2..256) quantile bins per feature, L2 penalty l2 on the leaf values,
+ * splits whose children both carry a Hessian of at least min_hess, leaves
+ * scaled by learning_rate; optional per-row weights (NULL: 1).  Writes
+ * feat, thr, leaf (layouts of anacor_mlp_set_risk) and base (the log-odds of
+ * the weighted label mean).  A node without a gainful split sends every row
+ * left (thr = +inf).  Deterministic: the same inputs give the same trees on
+ * any number of host threads (threads <= 0: the machine's). */
+int anacor_risk_fit(const float *features, const uint8_t *labels, const float *weights,
+                    int64_t n, int n_features, int n_trees, int depth, float learning_rate,
+                    float l2, float min_hess, int n_bins, int threads, uint8_t *feat,
+                    float *thr, float *leaf, float *base);
+
 /* Network inputs of R reflections (features[R][12]) and invariants[R][4] =
  * (axial, two_theta, alpha, beta); with an envelope set also the cell index
  * and the in-envelope flag.  Every output is nullable; without an envelope

@@ -8,6 +8,9 @@ this module is the checker for the GPU kernels of anacor_public_amd/csrc/mlp.hip
   incident and diffracted unit vectors, rotated as the tracer's setup does
   (Rodrigues matrix of utils/utils_os.py:26-43, ``(R(omega) F)^T v`` as
   ray_tracing.c:397-405 applies it), plus their second harmonics;
+* ``risk_features`` / ``risk_scores`` -- the resonance features of the
+  surrogate's risk trees and the trees' FP32 scores, the checker of
+  mlp.hip's risk_kernel;
 * ``forward`` / ``backward`` -- MLP 12 -> 256 x 4 -> 1 with SiLU and the mean
   squared error, in FP64; with ``bf16=True`` the three 256x256 weight matrices
   are first rounded to BF16 (round to nearest even) exactly as the GPU uses
@@ -225,3 +228,56 @@ def cell_index(inv, two_theta_range, cells):
     it = bin_(inv[:, 1], two_theta_range[0], two_theta_range[1], cells[1])
     ib = bin_(inv[:, 3], -np.pi, np.pi, cells[2])
     return (ia * cells[1] + it) * cells[2] + ib
+
+
+# ------------------------------------------------------------ risk trees
+_RISK_PAIRS = ((0, 1), (0, 2), (1, 2))
+
+
+def risk_features(X) -> np.ndarray:
+    """The 60 resonance features of rows of network inputs X[n][12] (x = X[:, 0:3],
+    s = X[:, 3:6]) in FP32, the operation chain of mlp.hip risk_features:
+    the six components, per ray and pair (i, j) the ratio
+    t = min(|v_i|, |v_j|) / max(|v_i|, |v_j|), and |q t - rint(q t)| * RN(1/q)
+    for q = 1..8."""
+    X = np.asarray(X, dtype=np.float32)
+    n = X.shape[0]
+    out = np.empty((n, 60), dtype=np.float32)
+    out[:, 0:6] = X[:, 0:6]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        for k in range(6):
+            r, p = divmod(k, 3)
+            i, j = _RISK_PAIRS[p]
+            a = np.abs(X[:, 3 * r + i])
+            b = np.abs(X[:, 3 * r + j])
+            hi = np.maximum(a, b)
+            t = np.where(hi > 0, np.minimum(a, b) / np.where(hi > 0, hi, np.float32(1)),
+                         np.float32(0)).astype(np.float32)
+            out[:, 6 + k] = t
+            for q in range(1, 9):
+                u = np.float32(q) * t
+                inv_q = np.float32(1) / np.float32(q)
+                out[:, 12 + 8 * k + q - 1] = np.abs(u - np.rint(u)) * inv_q
+    return out
+
+
+def risk_scores(trees: dict, F) -> np.ndarray:
+    """FP32 scores of complete binary trees (feat/thr [T][2^D - 1], leaf
+    [T][2^D], base, depth) over feature rows F[n][60]: from node 0,
+    node = 2 node + 1 + (f > thr), the leaf node - (2^D - 1); base plus the
+    leaves in tree order."""
+    F = np.asarray(F, dtype=np.float32)
+    D = int(trees["depth"])
+    n_int = (1 << D) - 1
+    feat = np.asarray(trees["feat"]).reshape(-1, n_int)
+    thr = np.asarray(trees["thr"], dtype=np.float32).reshape(-1, n_int)
+    leaf = np.asarray(trees["leaf"], dtype=np.float32).reshape(-1, n_int + 1)
+    rows = np.arange(F.shape[0])
+    s = np.full(F.shape[0], np.float32(trees["base"]), dtype=np.float32)
+    for t in range(feat.shape[0]):
+        j = np.zeros(F.shape[0], dtype=np.int64)
+        for _ in range(D):
+            f = feat[t][j]
+            j = 2 * j + 1 + (F[rows, f] > thr[t][j])
+        s = (s + leaf[t][j - n_int]).astype(np.float32)
+    return s

@@ -5,8 +5,9 @@ Bars: features to FP32 rounding (inputs) and 1e-12 (FP64 invariants); the
 forward pass within 5e-5 absolute of the oracle with BF16-rounded weights
 (and measurably away from FP64 weights, so the BF16 path is what runs);
 gradients within 1e-4 relative L2 per parameter block; the envelope flags and
-cells exactly as the oracle assigns them; reflections outside the envelope
-come from the tracer bit for bit."""
+cells exactly as the oracle assigns them; the risk trees' resonance features
+and scores bit for bit as the oracle's FP32 restatement computes them;
+reflections outside the envelope come from the tracer bit for bit."""
 import sys
 
 import numpy as np
@@ -106,6 +107,57 @@ def test_envelope_bands(gpu, refl):
     np.testing.assert_array_equal(flags, expect)
     assert all_in.all()
     assert 0.1 < expect.mean() < 0.95
+
+
+def test_risk_kernel_matches_oracle(gpu, refl):
+    """The risk kernel's 60 resonance features and its tree scores equal the
+    oracle's FP32 restatement bit for bit, and with trees set the in-envelope
+    flag is the envelope's flag and score < cut."""
+    from anacor_public_amd.surrogate import MLP, risk_fit
+    s1, om, F = refl
+    raw = np.array([0.0, 0.0, -1.0])
+    axis = np.array([1.0, 0.0, 0.0])
+    say("case: risk features and trees")
+    with MLP(seed=3) as m:
+        X, _, _, _ = m.features(s1, om, raw, axis, F)
+        # special rows: exact ratios 0, 1/2, 1, halves that round to even, zeros
+        Xs = np.zeros((6, 12), np.float32)
+        Xs[0, :6] = [0.0, 0.6, 0.8, 0.6, 0.0, 0.8]
+        Xs[1, :6] = [0.5, 0.5, 0.70710677, 0.25, 0.5, 0.8291562]
+        Xs[2, :6] = [1.0, 0.0, 0.0, 0.0, 0.0, 1.0]
+        Xs[3, :6] = [-0.3, 0.4, -0.8660254, 0.125, -0.375, 0.9185587]
+        Xs[4, :6] = [0.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+        Xs[5, :6] = [1e-30, 3e-30, 1.0, -0.0, 0.0, -1.0]
+        Xa = np.concatenate([X, Xs])
+        Fr, _ = m.risk(Xa, scores=False)
+        Fo = MO.risk_features(Xa)
+        np.testing.assert_array_equal(Fr.view(np.uint32), Fo.view(np.uint32))
+        # trees fitted on a label of the features, then scored on the GPU
+        lab = ((Fo[:, 6 + 2] > 0.5) ^ (Fo[:, 12 + 8 * 4 + 2] < 0.05)).astype(np.uint8)
+        trees = risk_fit(Fo, lab, n_trees=30, depth=5)
+        m.set_risk(trees)
+        _, sc = m.risk(Xa)
+        so = MO.risk_scores(trees, Fo)
+        np.testing.assert_array_equal(sc.view(np.uint32), so.view(np.uint32))
+        assert np.mean((so > 0) == (lab == 1)) > 0.9
+        # the flag: envelope and trust as without trees, and score < cut
+        from anacor_public_amd.surrogate import crystal_axis, make_envelope
+        _, inv, _, _ = m.features(s1, om, raw, axis, F)
+        env = make_envelope(crystal_axis(axis, F), (inv[:, 0].min() - 1e-9,
+                                                    inv[:, 0].max() + 1e-9),
+                            (inv[:, 1].min(), inv[:, 1].max()), (4, 4, 4))
+        m.set_risk(None)
+        m.set_envelope(env)
+        _, _, _, f_env = m.features(s1, om, raw, axis, F, flags=True)
+        cut = float(np.median(so[:len(X)]))
+        m.set_risk(trees, cut)
+        _, _, _, f_risk = m.features(s1, om, raw, axis, F, flags=True)
+        np.testing.assert_array_equal(f_risk, f_env & (so[:len(X)] < cut))
+        assert 0.3 < f_risk.mean() < 0.7
+        with pytest.raises(Exception):
+            bad = dict(trees)
+            bad["feat"] = np.full_like(trees["feat"], 60)
+            m.set_risk(bad, cut)
 
 
 def test_forward_matches_oracle(gpu, refl):
@@ -213,20 +265,22 @@ def _heldout(fitted, bounded):
 
 
 def test_surrogate_heldout_error_bounded(gpu, fitted):
-    """Bounded mode (the default after fit(bound=1e-3)): the resonance bands
-    fitted on the training errors send their reflections to the tracer, and
-    every held-out reflection the network answers is within the stated 1e-3
-    of the tracer."""
+    """Bounded mode (the default after fit(bound=1e-3)): the risk trees fitted
+    on the training errors send the reflections where the network's error is
+    likely to exceed the bound to the tracer (under 10% of them), and every
+    held-out reflection the network answers is within the stated 1e-3 of the
+    tracer."""
     sw, tr, sur, rep = fitted
-    say(f"  {len(rep['bands'])} bands, training fallback {rep['band_train_fallback'] * 100:.1f}%")
-    assert sur.bounded and rep["bands"]
+    say(f"  risk trees: cut {rep['risk_cut']:.3f}, cross-fitted training fallback "
+        f"{rep['risk_train_fallback'] * 100:.1f}%, fitted in {rep['trust_fit_s']:.1f}s")
+    assert sur.bounded and rep["trust"] == "trees" and sur.risk is not None
     r, used = _heldout(fitted, True)
     assert r.max() < 1e-3
-    assert used.mean() > 0.6
+    assert used.mean() > 0.9
 
 
 def test_surrogate_heldout_error_fast(gpu, fitted):
-    """Fast mode (bands off): 99% of held-out reflections within 1e-3 and a
+    """Fast mode (risk trees off): 99% of held-out reflections within 1e-3 and a
     mean below 3e-4.  Its tail is the tracer's own fine oscillation near
     rational direction ratios, which 200k training reflections do not
     resolve; it is measured and reported, not bounded."""
@@ -240,7 +294,7 @@ def test_surrogate_fallback_and_gain(gpu, fitted):
     """Reflections outside the training 2-theta range, or from another
     goniometer, come from the tracer bit for bit; on the mixed inference set
     the fast surrogate is at least 20x faster than tracing every reflection,
-    the bounded one at least 3x."""
+    the bounded one at least 8x."""
     import time
     sw, tr, sur, rep = fitted
     wl = sw.sample
@@ -278,7 +332,7 @@ def test_surrogate_fallback_and_gain(gpu, fitted):
         f"{t_bounded * 1e3:.1f} ms ({t_tr / t_bounded:.1f}x, {used_b.mean() * 100:.1f}% "
         f"answered), tracer {t_tr * 1e3:.1f} ms, fast p99 error {np.quantile(r, 0.99):.2e}")
     assert t_tr / t_fast > 20
-    assert t_tr / t_bounded > 3
+    assert t_tr / t_bounded > 8
     assert np.quantile(r, 0.99) < 1e-3
 
 
@@ -294,3 +348,7 @@ def test_surrogate_state_round_trip(gpu, fitted):
     np.testing.assert_array_equal(ua, ub)
     np.testing.assert_array_equal(a[ua], b[ub])
     assert other.bounded == sur.bounded and other.bands == sur.bands
+    assert other.risk_cut == sur.risk_cut
+    for k in ("feat", "thr", "leaf"):
+        np.testing.assert_array_equal(other.risk[k], sur.risk[k])
+    assert not ua.all()                     # the trees send some rows to the tracer
