@@ -104,8 +104,8 @@ anacor_ctx *anacor_ctx_create(const uint8_t *volume, const int64_t shape[3],
                               int full_iteration, int device);
 void anacor_ctx_destroy(anacor_ctx *ctx);
 
-/* Voxels per workgroup tile (0 = automatic: at most 1,024, 512 in the
- * bricked layouts). */
+/* Voxels per workgroup tile (0 = automatic: at most 1,024 in the raw
+ * layout, 256 in the bricked ones). */
 int anacor_ctx_set_tile(anacor_ctx *ctx, int voxels_per_tile);
 
 /* Launch order of the trace kernel (speed only, never results): reflections
