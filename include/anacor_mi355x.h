@@ -114,8 +114,9 @@ int anacor_ctx_set_tile(anacor_ctx *ctx, int voxels_per_tile);
  * diffracted one in the bricked layouts, else the incident one), and launched
  * in groups of refl_group consecutive reflections, tile by tile within a
  * group, so that the workgroups resident on an XCD share the bricks their
- * rays cross (0: automatic, 128 in the bricked layouts and none in the raw
- * one; 1: no groups, every tile of a reflection in turn). */
+ * rays cross (0: automatic, 2,048 in the bricked layouts and none in the raw
+ * one; 1: no groups, every tile of a reflection in turn).  refl_group at most
+ * 65,536. */
 int anacor_ctx_set_order(anacor_ctx *ctx, int refl_group, int direction_major);
 
 /* The work shape in use: voxel block (bz, by, bx; 0, 0, 0 = Morton), tile
