@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -639,7 +640,9 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
   const bool round = L.bias != 0.0;
   bool unclamped = false;
   // raw layouts only: the bricked walk waits on memory, not on VALU issue,
-  // and measured no faster without the clamp (1000x1000x800 volume)
+  // and measured no faster without the clamp (1000x1000x800 volume, 10k
+  // reflections in launch groups of 2,048: 996.7 ms with it, 1000.2 ms
+  // without)
   if (!GUARD && LAYOUT < kLayoutBrick2) {
     // with the zero shell the walk reads increment incmax + 1 of every ray
     // (limit - 1 for an inclusive bound, limit for an exclusive one)
@@ -1424,14 +1427,62 @@ constexpr int kUnrollRaw = 6;  // label gathers in flight per lane, raw layouts
 // Label gathers in flight per lane.  Raw layouts (256^3 workload, clamp-free
 // forms, 1,024-voxel tiles): 2: 91.9 ms, 3: 83.8, 4: 80.1, 5: 78.8, 6: 78.4,
 // 8: 78.4.  The bricked walk waits on L2/Infinity Cache latency with VALU to
-// spare (1000x1000x800, 1,000 reflections: 164 ms with 4, 140 ms with 8).
+// spare (1000x1000x800, 1,000 reflections: 164 ms with 4, 140 ms with 8;
+// 10k reflections at three workgroups per CU: 892.8 ms with 8, 904.0 with
+// 12, 1,697 with 16, whose code the compiler serialises).
 constexpr int unroll_for(int layout) { return layout >= kLayoutBrick2 ? 8 : kUnrollRaw; }
+
+// Workgroups of the bricked walk resident per CU.  On a volume far larger
+// than the caches the bricked walk is bound by how much of the bricks that its
+// rays in flight cross the L2 holds, and fewer resident workgroups keep that
+// working set smaller (1000x1000x800 volume, 10k reflections, 256-voxel
+// tiles, groups of 2,048: 996.7 ms with 4 workgroups per CU, the register
+// limit; 892.8 ms with 3; 891.7 ms with 2.  The 256^3 volume in bricks: 146,
+// 150 and 170 ms).  The limit is set by the launch's dynamic LDS, sized from
+// the device's LDS per CU so that no further workgroup fits beside them.
+constexpr int kBrickWgPerCu = 3;
+
+// Dynamic LDS bytes that hold launches of `kernel` to `per_cu` workgroups per
+// CU on the current device; 0 (no limit) when the device attributes are not
+// available or the limit does not fit a workgroup's LDS.
+static int lds_pad_for(const void* kernel, int per_cu) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  const auto key = std::make_pair(dev, kernel);
+  const auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int pad = 0, lds_cu = 0, lds_block = 0;
+  hipFuncAttributes fa{};
+  if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) ==
+          hipSuccess &&
+      hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+          hipSuccess &&
+      hipFuncGetAttributes(&fa, kernel) == hipSuccess) {
+    // just over 1 / (per_cu + 1) of the CU's LDS per workgroup, whole KiB: a
+    // further workgroup cannot fit, and the allocation granularity cannot
+    // push out one of the per_cu
+    const int target = ((lds_cu / (per_cu + 1)) & ~1023) + 1024;
+    const int stat = (int)fa.sharedSizeBytes;
+    if (target > stat && target <= lds_block && (int64_t)per_cu * target <= lds_cu)
+      pad = target - stat;
+  } else {
+    (void)hipGetLastError();
+  }
+  cache[key] = pad;
+  return pad;
+}
 
 template <bool GUARD, bool FULL, int LAYOUT, bool F64, bool PATHS, bool L04>
 static void launch_trace(const TraceArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(
-      (trace_kernel<kCap, unroll_for(LAYOUT), GUARD, FULL, LAYOUT, F64, PATHS, L04>),
-      dim3(grid), dim3(kBlock), 0, s, a);
+  auto* k = &trace_kernel<kCap, unroll_for(LAYOUT), GUARD, FULL, LAYOUT, F64, PATHS, L04>;
+  const int pad = LAYOUT >= kLayoutBrick2 ? lds_pad_for((const void*)k, kBrickWgPerCu) : 0;
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), (unsigned)pad, s, a);
 }
 
 using LaunchFn = void (*)(const TraceArgs&, int, hipStream_t);

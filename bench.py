@@ -55,24 +55,24 @@ MLP_FLOPS_PER_ROW = 2 * (12 * 256 + 3 * 256 * 256 + 256)   # 2 x 199,936 MACs
 _HEAD = "plain-256^3-stride2"
 _LARGE10K = "plain-1000x1000x800-100k-first10000"
 TRAFFIC_BYTES_PER_STEP = {
-    (_HEAD, 0, (1, 2, 64), 0, 1): 0.2099,
-    (_HEAD, 0, (1, 2, 64), 512, 1): 0.0961,
-    (_HEAD, 0, (1, 2, 64), 2048, 1): 0.4298,
-    (_HEAD, 0, (1, 1, 128), 0, 1): 0.2099,
-    (_HEAD, 0, (1, 1, 128), 512, 1): 0.0960,
-    (_HEAD, 0, (1, 1, 128), 2048, 1): 0.4299,
-    (_HEAD, 0, (2, 2, 32), 0, 1): 0.2100,
-    (_HEAD, 0, (2, 2, 32), 512, 1): 0.0960,
-    (_HEAD, 0, (2, 2, 32), 2048, 1): 0.4298,
-    (_HEAD, 0, (1, 4, 32), 0, 1): 0.2213,
-    (_HEAD, 0, (1, 4, 32), 512, 1): 0.1054,
+    (_HEAD, 0, (1, 2, 64), 0, 1): 0.2103,
+    (_HEAD, 0, (1, 2, 64), 512, 1): 0.0965,
+    (_HEAD, 0, (1, 2, 64), 2048, 1): 0.4304,
+    (_HEAD, 0, (1, 1, 128), 0, 1): 0.2105,
+    (_HEAD, 0, (1, 1, 128), 512, 1): 0.0964,
+    (_HEAD, 0, (1, 1, 128), 2048, 1): 0.4303,
+    (_HEAD, 0, (2, 2, 32), 0, 1): 0.2104,
+    (_HEAD, 0, (2, 2, 32), 512, 1): 0.0966,
+    (_HEAD, 0, (2, 2, 32), 2048, 1): 0.4302,
+    (_HEAD, 0, (1, 4, 32), 0, 1): 0.2218,
+    (_HEAD, 0, (1, 4, 32), 512, 1): 0.1058,
     (_HEAD, 0, (1, 4, 32), 2048, 1): 0.4544,
-    (_HEAD, 0, (2, 4, 16), 0, 1): 0.2240,
-    (_HEAD, 0, (2, 4, 16), 512, 1): 0.1089,
-    (_HEAD, 0, (2, 4, 16), 2048, 1): 0.4615,
-    (_HEAD, 0, (0, 0, 0), 0, 1): 0.2018,
-    (_HEAD, 0, (0, 0, 0), 512, 1): 0.1124,
-    (_HEAD, 0, (0, 0, 0), 2048, 1): 0.4745,
+    (_HEAD, 0, (2, 4, 16), 0, 1): 0.2245,
+    (_HEAD, 0, (2, 4, 16), 512, 1): 0.1093,
+    (_HEAD, 0, (2, 4, 16), 2048, 1): 0.4621,
+    (_HEAD, 0, (0, 0, 0), 0, 1): 0.2029,
+    (_HEAD, 0, (0, 0, 0), 512, 1): 0.1135,
+    (_HEAD, 0, (0, 0, 0), 2048, 1): 0.4787,
     (_HEAD, 2, (1, 2, 64), 0, 2048): 0.0019,
     (_HEAD, 2, (1, 2, 64), 512, 2048): 0.0013,
     (_HEAD, 2, (1, 2, 64), 2048, 2048): 0.0007,
@@ -91,7 +91,7 @@ TRAFFIC_BYTES_PER_STEP = {
     (_HEAD, 2, (0, 0, 0), 0, 2048): 0.0019,
     (_HEAD, 2, (0, 0, 0), 512, 2048): 0.0013,
     (_HEAD, 2, (0, 0, 0), 2048, 2048): 0.0007,
-    (_LARGE10K, 2, (0, 0, 0), 0, 2048): 0.6564,
+    (_LARGE10K, 2, (0, 0, 0), 0, 2048): 0.5144,
 }
 # The same per label read of ib_kernel (2-bit bricks), over 2.207e10 reads,
 # profiles/ib_kernel_bisection-256cube.md.
