@@ -1439,7 +1439,9 @@ constexpr int unroll_for(int layout) { return layout >= kLayoutBrick2 ? 8 : kUnr
 // tiles, groups of 2,048: 996.7 ms with 4 workgroups per CU, the register
 // limit; 892.8 ms with 3; 891.7 ms with 2.  The 256^3 volume in bricks: 146,
 // 150 and 170 ms).  The limit is set by the launch's dynamic LDS, sized from
-// the device's LDS per CU so that no further workgroup fits beside them.
+// the device's LDS per CU so that no further workgroup fits beside them.  The
+// raw walk, bound by VALU issue on cache-resident volumes, keeps its register
+// limit of five (256^3 workload: 62.4 ms; 65.7 at four, 73.3 at three).
 constexpr int kBrickWgPerCu = 3;
 
 // Dynamic LDS bytes that hold launches of `kernel` to `per_cu` workgroups per
