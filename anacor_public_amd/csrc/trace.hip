@@ -33,7 +33,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <map>
 #include <mutex>
 #include <vector>
 
@@ -1447,18 +1446,24 @@ constexpr int kBrickWgPerCu = 3;
 // Dynamic LDS bytes that hold launches of `kernel` to `per_cu` workgroups per
 // CU on the current device; 0 (no limit) when the device attributes are not
 // available or the limit does not fit a workgroup's LDS.
+// (a fixed table, so that no call allocates, not even host memory)
 static int lds_pad_for(const void* kernel, int per_cu) {
+  struct Entry {
+    int dev;
+    const void* kernel;
+    int pad;
+  };
   static std::mutex mu;
-  static std::map<std::pair<int, const void*>, int> cache;
+  static Entry cache[256];
+  static int used = 0;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) {
     (void)hipGetLastError();
     return 0;
   }
   std::lock_guard<std::mutex> lk(mu);
-  const auto key = std::make_pair(dev, kernel);
-  const auto it = cache.find(key);
-  if (it != cache.end()) return it->second;
+  for (int i = 0; i < used; ++i)
+    if (cache[i].dev == dev && cache[i].kernel == kernel) return cache[i].pad;
   int pad = 0, lds_cu = 0, lds_block = 0;
   hipFuncAttributes fa{};
   if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) ==
@@ -1476,7 +1481,7 @@ static int lds_pad_for(const void* kernel, int per_cu) {
   } else {
     (void)hipGetLastError();
   }
-  cache[key] = pad;
+  if (used < 256) cache[used++] = Entry{dev, kernel, pad};
   return pad;
 }
 
