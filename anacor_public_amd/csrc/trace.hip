@@ -1404,14 +1404,16 @@ static int fit_tile(int64_t n, int64_t R, int vpt) {
 }
 
 // Automatic: at most 4 voxels per lane (1,024 per tile; measured on the 256^3
-// workload: 80 ms at 1,024, 84 ms at 6,144, 108 ms from 8,192 up), 1 in the
+// workload: 80 ms at 1,024, 84 ms at 6,144, 108 ms from 8,192 up), 1.5 in the
 // bricked layouts, where a smaller tile narrows the bundle of rays whose
 // bricks a launch group shares (1000x1000x800, 10k reflections, groups of
-// 2,048, sort led by the diffracted ray: 1,064 ms at 512, 995 ms at 256).
+// 2,048, sort led by the diffracted ray, three workgroups per CU: 939 ms at
+// 512, 864 ms at 384, 894 ms at 256; the same volume with seed 7, 6k
+// reflections: 580, 527, 544 ms; 700x900x900, 80k voxels: 398, 378, 378 ms).
 static bool is_bricked(int layout);
 static int choose_tile(const Context* c, int64_t R, int layout = -1) {
   if (c->tile_voxels > 0) return c->tile_voxels;
-  const int64_t cap = (int64_t)kBlock * (is_bricked(layout) ? 1 : 4);
+  const int64_t cap = is_bricked(layout) ? kBlock + kBlock / 2 : (int64_t)kBlock * 4;
   const int64_t target_blocks = 256 * 16;
   int64_t tiles = (target_blocks + R - 1) / std::max<int64_t>(R, 1);
   tiles = std::max<int64_t>(1, std::min<int64_t>(tiles, (c->N + kBlock - 1) / kBlock));

@@ -105,7 +105,7 @@ anacor_ctx *anacor_ctx_create(const uint8_t *volume, const int64_t shape[3],
 void anacor_ctx_destroy(anacor_ctx *ctx);
 
 /* Voxels per workgroup tile (0 = automatic: at most 1,024 in the raw
- * layout, 256 in the bricked ones). */
+ * layout, 384 in the bricked ones). */
 int anacor_ctx_set_tile(anacor_ctx *ctx, int voxels_per_tile);
 
 /* Launch order of the trace kernel (speed only, never results): reflections
