@@ -7,7 +7,8 @@ Pinned: the split a one-level tree must choose on a separable problem and
 its threshold; the leaf values of one Newton step of the logistic loss,
 recomputed in NumPy from the gradients and Hessians; base = log-odds of the
 weighted label mean; duplicated rows equal to weight 2; no split without a
-gain; the same trees on 1 and 4 host threads; argument errors.  The
+gain; the same trees on 1 and 4 host threads; no split without positive
+labels; argument errors.  The
 features: ratios and distances to p / q for hand-picked vectors, including
 the round-half-even case and a zero component; the scores of a hand-built
 depth-2 ensemble."""
@@ -28,12 +29,12 @@ def sigmoid(z):
 
 def test_one_split_separable():
     rng = np.random.default_rng(1)
-    F = rng.random((400, 3)).astype(np.float32)
+    F = rng.random((200, 3)).astype(np.float32)
     y = (F[:, 1] > 0.3).astype(np.uint8)
     t = fit(F, y, n_trees=1, depth=1, learning_rate=1.0, l2=0.0, min_hess=0.0, n_bins=256)
     assert t["feat"][0, 0] == 1
-    # every distinct value is a cut here (400 < 256 is false: quantile cuts), so
-    # the chosen threshold must separate the classes exactly
+    # fewer distinct values than bins: every value but the largest is a cut,
+    # so the chosen threshold must separate the classes exactly
     thr = t["thr"][0, 0]
     assert np.all(F[y == 0, 1] <= thr) and np.all(F[y == 1, 1] > thr)
     assert t["base"] == pytest.approx(np.log(y.mean() / (1 - y.mean())), rel=1e-6)
@@ -165,3 +166,16 @@ def test_risk_scores_hand_tree():
     #         (0.1, 0.9): 0.1 > 0.1 is false -> 30
     np.testing.assert_array_equal(s, np.float32([-1.5 + 1 + 10, -1.5 + 2 + 30, -1.5 + 3 + 40,
                                                  -1.5 + 4 + 40]))
+
+
+def test_no_positive_labels():
+    """A surrogate whose training errors all stay under the label threshold
+    gives trees without a split and a very negative base: no row is sent to
+    the tracer by them."""
+    rng = np.random.default_rng(5)
+    F = rng.random((500, 6)).astype(np.float32)
+    t = fit(F, np.zeros(500, np.uint8), n_trees=4, depth=3)
+    assert np.all(np.isinf(t["thr"])) and np.all(np.isfinite(t["leaf"]))
+    assert t["base"] < -20
+    s = MO.risk_scores(t, np.pad(F, ((0, 0), (0, 54))))
+    assert np.all(s < -20)
