@@ -660,9 +660,9 @@ def main():
         profile_shapes()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards "
-              "reflections across GPUs by construction but this build runs and reports N=1.",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): reflections "
+              "are independent and would be sharded across GPUs, but this benchmark runs and "
+              "reports N=1.", file=sys.stderr)
         sys.exit(2)
     budget = 0.0 if args.no_cpu_baseline else args.cpu_budget
 
