@@ -60,11 +60,12 @@ EXPORTED = (
     "ib_test", "anacor_ctx_bisection", "anacor_ctx_bisection_rotated", "anacor_probe_gather",
 )
 
-MLP_INPUTS = 12
+MLP_INPUTS = 60
 MLP_WIDTH = 256
-MLP_PARAMS = 200961
+MLP_PARAMS = 213249
 MLP_FEATURES_HARMONIC = 0
 MLP_FEATURES_CHAMBER = 1
+MLP_FEATURES_RESONANCE = 2
 
 PATH_DTYPE = np.dtype([("face", np.int32, 2), ("len", np.int32, 2), ("nseg", np.int32, 2),
                        ("err", np.int32), ("pad", np.int32), ("last", np.int32, (2, 3)),

@@ -1,6 +1,6 @@
-// Learned surrogate of the absorption factor: an MLP 12 -> 256 -> 256 -> 256
-// -> 256 -> 1 (SiLU) from the incident and diffracted directions in the
-// crystal frame to log A_h, trained and evaluated on the GPU by the kernels
+// Learned surrogate of the absorption factor: an MLP 60 -> 256 -> 256 -> 256
+// -> 256 -> 1 (SiLU) from features of the incident and diffracted directions
+// in the crystal frame to log A_h, trained and evaluated on the GPU by the kernels
 // in this file.  The ray tracer (trace.hip) generates its training targets,
 // judges its error and takes every reflection outside its envelope.
 //
@@ -8,12 +8,13 @@
 // BF16 weights and FP32 accumulation.  Activations and back-propagated
 // gradients stay FP32 and enter the MFMA as a BF16 hi + lo pair (two
 // products), so their rounding puts no input-dependent noise on the output;
-// weight gradients use hi/lo on both operands (three products).  The 12 -> 256
-// input layer and the 256 -> 1 output layer are FP32 VALU kernels.  FP32
-// master weights, Adam.
+// weight gradients use hi/lo on both operands (three products).  In training
+// the 60 -> 256 input layer and the 256 -> 1 output layer are FP32 VALU
+// kernels; inference runs the input layer on MFMA as hi/lo pairs of both
+// operands.  FP32 master weights, Adam.
 //
 // Parameter layout (FP32, ANACOR_MLP_PARAMS values):
-//   W1[256][12] b1[256] W2[256][256] b2[256] W3[256][256] b3[256]
+//   W1[256][60] b1[256] W2[256][256] b2[256] W3[256][256] b3[256]
 //   W4[256][256] b4[256] w5[256] b5[1]
 #include <hip/hip_runtime.h>
 
@@ -48,7 +49,8 @@ constexpr int kOffW5 = kOffB4 + kH;
 constexpr int kOffB5 = kOffW5 + kH;
 constexpr int kParams = kOffB5 + 1;
 static_assert(kParams == ANACOR_MLP_PARAMS, "parameter count");
-static_assert(kIn == 12 && kH == 256, "kernels assume 12 inputs and width 256");
+static_assert(kIn == 60 && kH == 256, "kernels assume 60 inputs and width 256");
+constexpr int kInK = 64;          // the input layer's K on MFMA (inputs padded with zeros)
 
 constexpr int kChunk = 1 << 16;   // largest training batch / gradient call
 
@@ -299,18 +301,20 @@ __global__ void split_reduce(const float* __restrict__ ws, int slices, int M, in
 // The whole network for a tile of 64 rows in one workgroup of 4 waves, on
 // MFMA with the units as the M dimension (D[unit][row] = W . H^T): wave w
 // owns units 64 (w % 4) .. + 63 of every layer for all 64 rows.  Weights come
-// from L2 as 16-byte A fragments (BF16 copies; the 12 -> 256 layer as hi/lo
-// pairs padded to K = 32); activations stay in LDS as BF16 hi/lo [row][unit],
+// from L2 as 16-byte A fragments (BF16 copies; the 60 -> 256 layer as hi/lo
+// pairs padded to K = 64); activations stay in LDS as BF16 hi/lo [row][unit],
 // so a lane's 4 consecutive units of one row leave as one 8-byte store.  The
+// inputs are staged as hi/lo in the first 64 columns of the same arrays and
+// overwritten by the first layer's activations once every wave has read
+// them.  The
 // output layer is reduced in registers.  With `flag`/`A` the kernel also
 // writes A = exp(y std + mean) inside the envelope and NaN outside.
 
 constexpr int kHP = kH + 8;   // padded LDS row of activations (bf16)
-constexpr int kXP = 32 + 8;   // padded LDS row of the inputs (bf16, K = 12 padded to 32)
-// BF16 weight block: W2, W3, W4 [256][256], then W1 hi and lo [256][32].
+// BF16 weight block: W2, W3, W4 [256][256], then W1 hi and lo [256][kInK].
 constexpr size_t kWbW1h = (size_t)3 * kH * kH;
-constexpr size_t kWbW1l = kWbW1h + (size_t)kH * 32;
-constexpr size_t kWbSize = kWbW1l + (size_t)kH * 32;
+constexpr size_t kWbW1l = kWbW1h + (size_t)kH * kInK;
+constexpr size_t kWbSize = kWbW1l + (size_t)kH * kInK;
 
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
@@ -321,17 +325,16 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(
   constexpr int ROWS = 64;
   __shared__ __attribute__((aligned(16))) __bf16 Hh[ROWS][kHP];
   __shared__ __attribute__((aligned(16))) __bf16 Hl[ROWS][kHP];
-  __shared__ __attribute__((aligned(16))) __bf16 Xh[ROWS][kXP];
-  __shared__ __attribute__((aligned(16))) __bf16 Xl[ROWS][kXP];
   __shared__ float part[4][ROWS];
+  static_assert(kInK <= kHP && kInK % 32 == 0, "input staging");
   const int t = (int)threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * ROWS;
-  for (int i = t; i < ROWS * 32; i += 256) {   // inputs as hi/lo, k >= 12 zero
-    const int r = i >> 5, k = i & 31;
+  for (int i = t; i < ROWS * kInK; i += 256) {   // inputs as hi/lo, k >= kIn zero
+    const int r = i / kInK, k = i % kInK;
     const float v = (k < kIn && r0 + r < n) ? X[(r0 + r) * kIn + k] : 0.f;
     const __bf16 h = (__bf16)v;
-    Xh[r][k] = h;
-    Xl[r][k] = (__bf16)(v - (float)h);
+    Hh[r][k] = h;
+    Hl[r][k] = (__bf16)(v - (float)h);
   }
   __syncthreads();
   const int lane = t & 63, wave = t >> 6;
@@ -342,17 +345,18 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  {  // layer 1: W1 (hi, lo) x X (hi, lo), three products
+  // layer 1: W1 (hi, lo) x X (hi, lo), three products, K = kInK in steps of 32
+  for (int k0 = 0; k0 < kInK; k0 += 32) {
     bf16x8 wh[4], wl[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      wh[i] = *(const bf16x8*)(Wb + kWbW1h + (size_t)(ub + i * 16 + r16) * 32 + kg);
-      wl[i] = *(const bf16x8*)(Wb + kWbW1l + (size_t)(ub + i * 16 + r16) * 32 + kg);
+      wh[i] = *(const bf16x8*)(Wb + kWbW1h + (size_t)(ub + i * 16 + r16) * kInK + k0 + kg);
+      wl[i] = *(const bf16x8*)(Wb + kWbW1l + (size_t)(ub + i * 16 + r16) * kInK + k0 + kg);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const bf16x8 xh = *(const bf16x8*)&Xh[j * 16 + r16][kg];
-      const bf16x8 xl = *(const bf16x8*)&Xl[j * 16 + r16][kg];
+      const bf16x8 xh = *(const bf16x8*)&Hh[j * 16 + r16][k0 + kg];
+      const bf16x8 xl = *(const bf16x8*)&Hl[j * 16 + r16][k0 + kg];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[i], xh, acc[i][j], 0, 0, 0);
@@ -361,6 +365,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(
       }
     }
   }
+  __syncthreads();   // every wave has read the inputs: the arrays take layer 1's output
   const int offb[4] = {kOffB1, kOffB2, kOffB3, kOffB4};
   for (int l = 0; l < 4; ++l) {
     if (l > 0) {
@@ -538,7 +543,7 @@ __global__ void head_finish(const float* __restrict__ part, int nb, float* __res
 }
 
 // BF16 copies of W2, W3, W4 (round to nearest even) and W1 as hi/lo pairs
-// padded to K = 32 (the inference kernel's layout).
+// padded to K = kInK (the inference kernel's layout).
 __device__ __forceinline__ void store_bf16_copy(int i, float p, __bf16* __restrict__ Wb) {
   int l = -1, j = 0;
   if (i >= kOffW2 && i < kOffB2) { l = 0; j = i - kOffW2; }
@@ -549,8 +554,8 @@ __device__ __forceinline__ void store_bf16_copy(int i, float p, __bf16* __restri
   } else if (i >= kOffW1 && i < kOffB1) {
     const int u = (i - kOffW1) / kIn, k = (i - kOffW1) % kIn;
     const __bf16 h = (__bf16)p;
-    Wb[kWbW1h + (size_t)u * 32 + k] = h;
-    Wb[kWbW1l + (size_t)u * 32 + k] = (__bf16)(p - (float)h);
+    Wb[kWbW1h + (size_t)u * kInK + k] = h;
+    Wb[kWbW1l + (size_t)u * kInK + k] = (__bf16)(p - (float)h);
   }
 }
 
@@ -584,7 +589,8 @@ __global__ void zero_floats(float* __restrict__ p, int n) {
 //
 // The crystal frame of the tracer's setup: M = Rodrigues(omega_axis, omega) F,
 // incident x = M^T raw_xray, diffracted s = M^T s1 (unit vectors here).
-// Network inputs: x (3), s (3), then six more by feature set:
+// Network inputs (ANACOR_MLP_INPUTS per row): x (3), s (3), then six more by
+// feature set:
 //   kFeaturesHarmonic: per vector v, 2 v0 v1, v0^2 - v1^2, 2 v2^2 - 1 (the
 //     sin/cos of twice its azimuth and polar angle, scaled);
 //   kFeaturesChamber: the side of the planes on which the tracer's discrete
@@ -592,14 +598,44 @@ __global__ void zero_floats(float* __restrict__ p, int n) {
 //     sign s2, sign(|s0| - |s1|), sign(|s1| - |s2|).  The floor/round choice
 //     of cal_coord follows the sign of theta, i.e. of v2
 //     (ray_tracing_basic.c:13-50, :428-438), and the absorption factor jumps
-//     across those planes and some of the diagonal ones.
+//     across those planes and some of the diagonal ones;
+//   kFeaturesResonance: the chamber signs, and in [12:60] for each ray and
+//     component pair the ratio t of the smaller to the larger magnitude and
+//     2 |q t - rint(q t)| for q = 2..8: V-shaped kinks at every ratio p / q,
+//     where the voxel steps of all crystal voxels fall into step together
+//     and the traced absorption factor moves faster than elsewhere (the other
+//     sets leave [12:60] zero).
 // Envelope invariants: axial = x . u (u: crystal-frame rotation axis),
 // two_theta = angle(x, s), alpha = azimuth of x about u (basis e1, e2),
 // beta = azimuth of s about x (basis u x x, x (u x x)).  The trust grid
 // splits (alpha, two_theta, beta) into cells; a reflection is inside the
 // envelope when axial and two_theta are in range and its cell is trusted.
 
-enum { kFeaturesHarmonic = 0, kFeaturesChamber = 1 };
+enum { kFeaturesHarmonic = 0, kFeaturesChamber = 1, kFeaturesResonance = 2 };
+static_assert(kFeaturesHarmonic == ANACOR_MLP_FEATURES_HARMONIC &&
+              kFeaturesChamber == ANACOR_MLP_FEATURES_CHAMBER &&
+              kFeaturesResonance == ANACOR_MLP_FEATURES_RESONANCE, "feature set ids");
+
+// Inputs [12:60] of kFeaturesResonance from the FP32 components v[6] (x, s):
+// ray r, pair p (0, 1), (0, 2), (1, 2), k = 3 r + p: o[k] = t, o[6 + 7 k +
+// q - 2] = 2 |q t - rint(q t)| (one rounding per operation, the last two
+// exact).
+__device__ __forceinline__ void resonance_inputs(const float* v, float* o) {
+  constexpr int pi[3] = {0, 0, 1}, pj[3] = {1, 2, 2};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int r = k / 3, p = k % 3;
+    const float a = fabsf(v[3 * r + pi[p]]), b = fabsf(v[3 * r + pj[p]]);
+    const float hi = fmaxf(a, b);
+    const float t = hi > 0.0f ? fminf(a, b) / hi : 0.0f;
+    o[k] = t;
+#pragma unroll
+    for (int q = 2; q <= 8; ++q) {
+      const float u = (float)q * t;
+      o[6 + 7 * k + q - 2] = 2.0f * fabsf(u - rintf(u));
+    }
+  }
+}
 
 // Resonance bands (anacor_band_t): near a small rational ratio of two
 // components of a ray's direction the voxel steps of all crystal voxels fall
@@ -677,10 +713,10 @@ __global__ void features_kernel(const double* __restrict__ s1, const double* __r
     s[b] *= ns;
   }
   if (X) {
-    float* o = X + 12 * i;
+    float* o = X + (int64_t)kIn * i;
     o[0] = (float)x[0]; o[1] = (float)x[1]; o[2] = (float)x[2];
     o[3] = (float)s[0]; o[4] = (float)s[1]; o[5] = (float)s[2];
-    if (g.feature_set == kFeaturesChamber) {
+    if (g.feature_set != kFeaturesHarmonic) {
       o[6] = sgn(x[2]);
       o[7] = sgn(fabs(x[1]) - fabs(x[2]));
       o[8] = sgn(fabs(x[0]) - fabs(x[1]));
@@ -694,6 +730,11 @@ __global__ void features_kernel(const double* __restrict__ s1, const double* __r
       o[9] = (float)(2.0 * s[0] * s[1]);
       o[10] = (float)(s[0] * s[0] - s[1] * s[1]);
       o[11] = (float)(2.0 * s[2] * s[2] - 1.0);
+    }
+    if (g.feature_set == kFeaturesResonance) {
+      resonance_inputs(o, o + 12);
+    } else {
+      for (int k = 12; k < kIn; ++k) o[k] = 0.f;
     }
   }
   const double axial = dot3(x, g.u);
@@ -783,7 +824,7 @@ __device__ __forceinline__ void risk_features(const float* v, float* o, int stri
   }
 }
 
-// One thread per row of network inputs X[n][12]: the features into LDS
+// One thread per row of network inputs X[n][kIn] (x, s in the first six): the features into LDS
 // (column per thread, so the 64 lanes of a wave read 64 consecutive banks),
 // optionally out to feat_out[n][60]; with trees the score (score[n],
 // nullable) and flag[i] cleared when score >= cut (flag nullable).  No thread
@@ -1237,7 +1278,8 @@ int anacor_mlp_set_scale(anacor_mlp* mlp, double y_mean, double y_std) {
 int anacor_mlp_set_features(anacor_mlp* mlp, int feature_set) {
   Model* M = model_of(mlp);
   if (!M || (feature_set != ANACOR_MLP_FEATURES_HARMONIC &&
-             feature_set != ANACOR_MLP_FEATURES_CHAMBER)) {
+             feature_set != ANACOR_MLP_FEATURES_CHAMBER &&
+             feature_set != ANACOR_MLP_FEATURES_RESONANCE)) {
     set_error(ANACOR_ERR_ARG, "mlp set_features: bad arguments");
     return ANACOR_ERR_ARG;
   }

@@ -1,8 +1,10 @@
 """Learned surrogate of the absorption factor for one sample.
 
-An MLP (12 -> 256 x 4 -> 1, SiLU; BF16-MFMA kernels with FP32 accumulation,
-csrc/mlp.hip) maps the incident and diffracted directions of a reflection in
-the crystal frame to log A_h.  It is trained on the GPU on absorption factors
+An MLP (60 -> 256 x 4 -> 1, SiLU; BF16-MFMA kernels with FP32 accumulation,
+csrc/mlp.hip) maps features of the incident and diffracted directions of a
+reflection in the crystal frame (by default the components, the sides of the
+planes where the tracer's rounding switches, and kinks at small rational
+ratios of two components) to log A_h.  It is trained on the GPU on absorption factors
 the ray tracer computes for the same sample, answers only inside the
 envelope it was trained on (the training 2-theta range and the training
 goniometer's rotation circle), and leaves every other reflection to the
@@ -146,7 +148,7 @@ class MLP:
     def risk(self, X, scores: bool = True):
         """(features float32[n][60], scores float32[n] or None) of rows of
         network inputs, on the GPU; scores need trees set."""
-        X = _f32(X).reshape(-1, 12)
+        X = _f32(X).reshape(-1, _lib.MLP_INPUTS)
         Fr = np.empty((len(X), _lib.RISK_FEATURES), dtype=np.float32)
         sc = np.empty(len(X), dtype=np.float32) if scores else None
         self._check(self._lib.anacor_mlp_risk(self.handle, _p(X), len(X), _p(Fr),
@@ -155,14 +157,14 @@ class MLP:
 
     def features(self, s1, omega, raw_xray, omega_axis, F, cells: bool = False,
                  flags: bool = False):
-        """X float32[R][12], invariants float64[R][4] (axial, two_theta, alpha,
+        """X float32[R][MLP_INPUTS], invariants float64[R][4] (axial, two_theta, alpha,
         beta) and, with an envelope set, cell indices and in-envelope flags."""
         s1 = _f64(s1).reshape(-1, 3)
         om = _f64(omega).reshape(-1)
         R = len(om)
         if len(s1) != R:
             raise ValueError("s1 and omega lengths differ")
-        X = np.empty((R, 12), dtype=np.float32)
+        X = np.empty((R, _lib.MLP_INPUTS), dtype=np.float32)
         inv = np.empty((R, 4), dtype=np.float64)
         cl = np.empty(R, dtype=np.int32) if cells else None
         fl = np.empty(R, dtype=np.uint8) if flags else None
@@ -173,14 +175,14 @@ class MLP:
         return X, inv, cl, (fl.astype(bool) if fl is not None else None)
 
     def forward(self, X) -> np.ndarray:
-        X = _f32(X).reshape(-1, 12)
+        X = _f32(X).reshape(-1, _lib.MLP_INPUTS)
         y = np.empty(len(X), dtype=np.float32)
         self._check(self._lib.anacor_mlp_forward(self.handle, _p(X), len(X), _p(y)))
         return y
 
     def gradients(self, X, t):
         """(gradient float32[n_params], loss) of the mean squared error."""
-        X = _f32(X).reshape(-1, 12)
+        X = _f32(X).reshape(-1, _lib.MLP_INPUTS)
         t = _f32(t).reshape(-1)
         g = np.empty(self.n_params, dtype=np.float32)
         loss = ct.c_float(0.0)
@@ -190,7 +192,7 @@ class MLP:
 
     def train(self, X, t, epochs: int, batch: int = 1024, lr: float = 1e-3,
               lr_min: float = 1e-5, seed: int = 0) -> np.ndarray:
-        X = _f32(X).reshape(-1, 12)
+        X = _f32(X).reshape(-1, _lib.MLP_INPUTS)
         t = _f32(t).reshape(-1)
         losses = np.zeros(int(epochs), dtype=np.float32)
         self._check(self._lib.anacor_mlp_train(self.handle, _p(X), _p(t), len(X), int(epochs),
@@ -343,7 +345,7 @@ class Surrogate:
     """MLP surrogate of one sample's A_h with the tracer as fallback and judge."""
 
     def __init__(self, tracer, raw_xray, omega_axis, F, seed: int = 0, device: int = 0,
-                 feature_set: int = _lib.MLP_FEATURES_CHAMBER):
+                 feature_set: int = _lib.MLP_FEATURES_RESONANCE):
         self.tracer = tracer
         self.raw_xray = _f64(raw_xray).reshape(3)
         self.omega_axis = _f64(omega_axis).reshape(3)
@@ -386,7 +388,7 @@ class Surrogate:
             band_target: float = 0.85, band_grow: float = 1.1, risk_trees: int = 200,
             risk_depth: int = 6, risk_folds: int = 5, risk_label: float = 0.5,
             risk_cut: float = 0.9, risk_quantile: float = 0.01,
-            risk_keep: float = 0.3) -> dict:
+            risk_keep: float = 0.3, risk_s1=None, risk_omega=None) -> dict:
         """Trace the training reflections (unless targets are given), train
         with Adam, and set the envelope: the training axial and two-theta
         ranges, and trust only in the (alpha, two_theta, beta) cells holding at
@@ -402,7 +404,10 @@ class Surrogate:
           (risk_folds folds, negatives subsampled to risk_keep and weighted
           back), and the cut is the risk_quantile quantile of those scores
           over the rows whose error exceeds risk_cut * bound; the trees set
-          on the GPU are refitted on every row.
+          on the GPU are refitted on every row.  With risk_s1 / risk_omega
+          (reflections of the training distribution that the network does not
+          train on) the trees learn from the network's errors on those, which
+          are not shrunk by fitting; else from the training rows' errors.
         * trust="bands": resonance bands (fit_bands, target band_target *
           bound, widths grown by band_grow).
         set_bounded(False) lets the network answer every row of its envelope."""
@@ -454,8 +459,18 @@ class Surrogate:
                          "band_train_fallback": float(band_out.mean()),
                          "band_uncovered": int(len(left))}
         elif bound is not None:
-            rep_trust = self._fit_risk(X, res, bound, seed, risk_trees, risk_depth, risk_folds,
+            Xk, rk, n_risk = X, res, 0
+            if risk_s1 is not None:
+                A_r = self.trace(risk_s1, risk_omega)
+                Xk, _, _, _ = self.mlp.features(risk_s1, risk_omega, *self._geom())
+                pred_r = np.exp(self.mlp.forward(Xk).astype(np.float64) * self.y_std +
+                                self.y_mean)
+                rk = np.abs(pred_r - A_r) / A_r
+                n_risk = len(rk)
+            rep_trust = self._fit_risk(Xk, rk, bound, seed, risk_trees, risk_depth, risk_folds,
                                        risk_label, risk_cut, risk_quantile, risk_keep)
+            rep_trust["risk_rows"] = n_risk if n_risk else int(len(res))
+            rep_trust["risk_set"] = "separate" if n_risk else "training"
         t_trust = time.perf_counter() - t2
         trust_tab = ok.astype(np.uint8)
         self.mlp.set_envelope(env, trust_tab)

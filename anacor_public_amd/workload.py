@@ -279,19 +279,23 @@ class SurrogateWorkload:
     infer_outside: np.ndarray      # bool: drawn outside the training 2-theta range
     audit_s1: np.ndarray = None    # a larger held-out set (its own stream) for the error tail
     audit_omega: np.ndarray = None
+    risk_s1: np.ndarray = None     # reflections the risk trees learn the network's errors on
+    risk_omega: np.ndarray = None
 
 
 SURROGATE_SAMPLE = 2   # of batched_specs(): 100^3, crystal 23.2/19.0/10.9, all 20,089 voxels
 
 
-def surrogate_config(seed: int = 5, n_train: int = 200_000, n_heldout: int = 10_000,
+def surrogate_config(seed: int = 5, n_train: int = 3_200_000, n_heldout: int = 10_000,
                      n_infer: int = 1_000_000, outside_fraction: float = 0.01,
-                     sample: int = SURROGATE_SAMPLE, n_audit: int = 0) -> SurrogateWorkload:
+                     sample: int = SURROGATE_SAMPLE, n_audit: int = 0,
+                     n_risk: int = 800_000) -> SurrogateWorkload:
     """Batched sample `sample` (100^3, every crystal voxel traced) with
     n_train training reflections (2theta 5-60 deg), n_heldout held-out ones
     from the same distribution, n_infer inference reflections of which
     outside_fraction lie at 2theta 60-75 deg, outside the training range,
-    and n_audit more held-out reflections for the tail of the error."""
+    n_audit more held-out reflections for the tail of the error, and n_risk
+    reflections of the training distribution for the risk trees."""
     sp = batched_specs()[sample]
     vol = make_volume(VolumeSpec(sp["shape"], sp["cyl_radius"], sp["slab"], sp["semi_axes"],
                                  sp["seed"]))
@@ -313,5 +317,7 @@ def surrogate_config(seed: int = 5, n_train: int = 200_000, n_heldout: int = 10_
                   coords, s1, om)
     as1, aom = (make_reflections(n_audit, seed * 1000 + 6, stream=331) if n_audit
                 else (np.zeros((0, 3)), np.zeros(0)))
+    rs1, rom = (make_reflections(n_risk, seed * 1000 + 7, stream=337) if n_risk
+                else (None, None))
     return SurrogateWorkload(wl, hs1, hom, np.ascontiguousarray(all_s1),
-                             np.ascontiguousarray(all_om), outside, as1, aom)
+                             np.ascontiguousarray(all_om), outside, as1, aom, rs1, rom)

@@ -15,10 +15,11 @@ workload in a "bisection" record of the default run, with the reference's
 ib_test on this host's cores beside it.
 
 The learned surrogate (config "surrogate", and a "surrogate" record in the
-default run) is trained on 200k reflections the tracer computes for batched
-sample 2 (100^3, 20,089 crystal voxels), judged by the tracer on 10k held-out
-reflections, and timed on 1M reflections of which 1% lie outside its training
-2-theta range and go to the tracer.
+default run) is trained on 3.2M reflections the tracer computes for batched
+sample 2 (100^3, 20,089 crystal voxels), its risk trees on the network's
+errors on 800k more, judged by the tracer on 10k held-out reflections and a
+200k audit set, and timed on 1M reflections of which 1% lie outside its
+training 2-theta range and go to the tracer.
 
 Workloads (anacor_public_amd/workload.py):
   single   256^3 volume, 25,323 crystal voxels (stride 2), 20,000 reflections (default)
@@ -43,7 +44,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
-MLP_FLOPS_PER_ROW = 2 * (12 * 256 + 3 * 256 * 256 + 256)   # 2 x 199,936 MACs
+MLP_FLOPS_PER_ROW = 2 * (60 * 256 + 3 * 256 * 256 + 256)   # 2 x 212,224 MACs
 # L2<->fabric bytes per voxel step of trace_kernel, by exact work shape
 # (workload, layout, voxel block, tile, reflection group; tile 0 is the
 # library's automatic choice): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (the
@@ -234,9 +235,10 @@ def gridding_baseline(tracer, sw, geo, A_true, steps: int, n_theta: int = 360,
     return out
 
 
-def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int = 600,
+def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int = 150,
                   cpu_budget: float = 0.0):
-    """Fit the surrogate, judge it on the held-out set (10k reflections) and
+    """Fit the surrogate (risk trees on a separate risk set of 800k
+    reflections), judge it on the held-out set (10k reflections) and
     on an audit set (200k more, its own stream) for the tail of the error,
     time 1M inferences (features, envelope and risk-tree tests, MLP on MFMA,
     tracer fallback) and the plain GPU path on the same sample.  Two
@@ -264,11 +266,13 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
         for _ in range(3):
             tr.run(ps1, pom, *geo)
         plain_rate = 3 * len(pom) / (time.perf_counter() - t1)
-        rep = sur.fit(wl.s1, wl.omega, epochs=epochs, bound=SURROGATE_BOUND)
+        rep = sur.fit(wl.s1, wl.omega, epochs=epochs, bound=SURROGATE_BOUND,
+                      risk_s1=sw.risk_s1, risk_omega=sw.risk_omega)
         log(f"# surrogate fit: trace {rep['trace_s']:.2f}s, train {rep['train_s']:.1f}s "
             f"({epochs} epochs), final loss {rep['final_loss']:.3e}, risk trees "
             f"{rep['trust_fit_s']:.1f}s (cross-fitted fallback "
-            f"{rep['risk_train_fallback'] * 100:.1f}% of training reflections)")
+            f"{rep['risk_train_fallback'] * 100:.1f}% of the {rep['risk_rows']} risk-set "
+            f"reflections)")
         A_true = tr.run(sw.heldout_s1, sw.heldout_omega, *geo)
         A_audit = tr.run(sw.audit_s1, sw.audit_omega, *geo)
 
@@ -314,10 +318,13 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
             "n_voxels": int(len(wl.coords)), "inferences": n_infer,
             "fallback_reflections": bounded["fallback_reflections"],
             "fallback_fraction": bounded["fallback_fraction"],
-            "training_pairs": int(len(wl.omega)), "epochs": epochs,
+            "network_inputs": "60: components, chamber signs, resonance kinks",
+            "training_pairs": int(len(wl.omega)), "risk_pairs": int(rep["risk_rows"]),
+            "epochs": epochs,
             "train_s": rep["train_s"], "train_trace_s": rep["trace_s"],
             "trust": {k: rep[k] for k in ("trust", "risk_trees", "risk_depth", "risk_cut",
-                                          "risk_train_fallback", "risk_train_missed")},
+                                          "risk_set", "risk_rows", "risk_train_fallback",
+                                          "risk_train_missed")},
             "trust_fit_s": rep["trust_fit_s"],
             "heldout": int(len(sw.heldout_omega)),
             "heldout_answered": bounded["heldout_answered"],
@@ -636,7 +643,7 @@ def main():
                     help="skip the 1000x1000x800 record of the default run")
     ap.add_argument("--large-reflections", type=int, default=LARGE_REFLECTIONS,
                     help="reflections of the default run's 1000x1000x800 record (its first R)")
-    ap.add_argument("--epochs", type=int, default=600, help="surrogate training epochs")
+    ap.add_argument("--epochs", type=int, default=150, help="surrogate training epochs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--tile", type=int, default=0)

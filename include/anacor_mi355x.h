@@ -345,21 +345,33 @@ int anacor_reflection_geometry(const double *s1, const double *omega, int64_t R,
 
 /* -------------------------------------------------------- learned surrogate */
 
-/* An MLP 12 -> 256 -> 256 -> 256 -> 256 -> 1 (SiLU) from the incident and
- * diffracted unit vectors in the crystal frame (plus their second harmonics)
- * to log A_h for one sample, trained on the GPU on absorption factors the
- * tracer computes.  No counterpart in the reference; its nearest analogue is
- * the gridding approximation (src/gridding_interpolation.c:98-161).  The
- * 256x256 layers use BF16 weights on MFMA with FP32 accumulation. */
-#define ANACOR_MLP_INPUTS 12
+/* An MLP 60 -> 256 -> 256 -> 256 -> 256 -> 1 (SiLU) from features of the
+ * incident and diffracted unit vectors in the crystal frame to log A_h for
+ * one sample, trained on the GPU on absorption factors the tracer computes.
+ * No counterpart in the reference; its nearest analogue is the gridding
+ * approximation (src/gridding_interpolation.c:98-161).  The 256x256 layers
+ * use BF16 weights on MFMA with FP32 accumulation, the input layer BF16
+ * hi/lo pairs.
+ *
+ * A row of network inputs (float[60]): [0:3] x (incident), [3:6] s
+ * (diffracted), [6:12] six more by feature set, [12:60] resonance inputs
+ * (ANACOR_MLP_FEATURES_RESONANCE; zero in the other sets): for each ray
+ * (x, then s) and component pair (0, 1), (0, 2), (1, 2), k = 3 ray + pair,
+ * [12 + k] the ratio t = min(|v_i|, |v_j|) / max(|v_i|, |v_j|) (0 when both
+ * are 0) and [18 + 7 k + q - 2] = 2 |q t - rint(q t)| for q = 2..8, all in
+ * FP32 from the FP32 components. */
+#define ANACOR_MLP_INPUTS 60
 #define ANACOR_MLP_WIDTH  256
-#define ANACOR_MLP_PARAMS 200961   /* FP32 master parameters, layout in mlp.hip */
+#define ANACOR_MLP_PARAMS 213249   /* FP32 master parameters, layout in mlp.hip */
 
-/* Network input sets (anacor_mlp_set_features): six more inputs after the two
+/* Network input sets (anacor_mlp_set_features): inputs [6:12] after the two
  * unit vectors x (incident) and s (diffracted), crystal frame. */
-#define ANACOR_MLP_FEATURES_HARMONIC 0  /* per vector: 2 v0 v1, v0^2 - v1^2, 2 v2^2 - 1 */
-#define ANACOR_MLP_FEATURES_CHAMBER  1  /* +-1 sides of the planes where the tracer's
-                                           discrete rules switch (sign of theta, diagonals) */
+#define ANACOR_MLP_FEATURES_HARMONIC  0  /* per vector: 2 v0 v1, v0^2 - v1^2, 2 v2^2 - 1 */
+#define ANACOR_MLP_FEATURES_CHAMBER   1  /* +-1 sides of the planes where the tracer's
+                                            discrete rules switch (sign of theta, diagonals) */
+#define ANACOR_MLP_FEATURES_RESONANCE 2  /* the chamber signs, and the resonance inputs
+                                            [12:60]: kinks where two components of a ray
+                                            stand in a ratio p / q, q <= 8 */
 
 /* Where the network may answer.  Invariants of a reflection: axial = x . axis,
  * two_theta = angle(x, s), alpha = atan2(x . e2, x . e1) (azimuth of the
@@ -440,7 +452,7 @@ int anacor_mlp_set_risk(anacor_mlp *mlp, int n_trees, int depth, const uint8_t *
 
 /* The 60 resonance features (out[n][60]) and the trees' scores (scores[n],
  * the trees set by anacor_mlp_set_risk; nullable) of n rows of network
- * inputs x[n][12], on the GPU. */
+ * inputs x[n][ANACOR_MLP_INPUTS] (only x[:, 0:6] is read), on the GPU. */
 int anacor_mlp_risk(anacor_mlp *mlp, const float *x, int64_t n, float *features,
                     float *scores);
 
@@ -458,7 +470,7 @@ int anacor_risk_fit(const float *features, const uint8_t *labels, const float *w
                     float l2, float min_hess, int n_bins, int threads, uint8_t *feat,
                     float *thr, float *leaf, float *base);
 
-/* Network inputs of R reflections (features[R][12]) and invariants[R][4] =
+/* Network inputs of R reflections (features[R][ANACOR_MLP_INPUTS]) and invariants[R][4] =
  * (axial, two_theta, alpha, beta); with an envelope set also the cell index
  * and the in-envelope flag.  Every output is nullable; without an envelope
  * the axis is F^T omega_axis. */

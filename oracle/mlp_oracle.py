@@ -7,11 +7,13 @@ this module is the checker for the GPU kernels of anacor_public_amd/csrc/mlp.hip
 * ``features`` -- the network inputs of a reflection: the crystal-frame
   incident and diffracted unit vectors, rotated as the tracer's setup does
   (Rodrigues matrix of utils/utils_os.py:26-43, ``(R(omega) F)^T v`` as
-  ray_tracing.c:397-405 applies it), plus their second harmonics;
+  ray_tracing.c:397-405 applies it), plus their second harmonics or chamber
+  signs, and (``resonance_inputs``) the kinks at small rational component
+  ratios;
 * ``risk_features`` / ``risk_scores`` -- the resonance features of the
   surrogate's risk trees and the trees' FP32 scores, the checker of
   mlp.hip's risk_kernel;
-* ``forward`` / ``backward`` -- MLP 12 -> 256 x 4 -> 1 with SiLU and the mean
+* ``forward`` / ``backward`` -- MLP 60 -> 256 x 4 -> 1 with SiLU and the mean
   squared error, in FP64; with ``bf16=True`` the three 256x256 weight matrices
   are first rounded to BF16 (round to nearest even) exactly as the GPU uses
   them, so the GPU's remaining differences are FP32 accumulation only.
@@ -26,14 +28,14 @@ from __future__ import annotations
 
 import numpy as np
 
-N_IN = 12
+N_IN = 60
 WIDTH = 256
-# parameter layout: W1[256][12] b1 W2[256][256] b2 W3 b3 W4 b4 w5[256] b5[1]
+# parameter layout: W1[256][60] b1 W2[256][256] b2 W3 b3 W4 b4 w5[256] b5[1]
 _SHAPES = (("W1", (WIDTH, N_IN)), ("b1", (WIDTH,)), ("W2", (WIDTH, WIDTH)), ("b2", (WIDTH,)),
            ("W3", (WIDTH, WIDTH)), ("b3", (WIDTH,)), ("W4", (WIDTH, WIDTH)), ("b4", (WIDTH,)),
            ("w5", (WIDTH,)), ("b5", (1,)))
 N_PARAMS = sum(int(np.prod(s)) for _, s in _SHAPES)
-assert N_PARAMS == 200961
+assert N_PARAMS == 213249
 
 
 def offsets():
@@ -78,7 +80,7 @@ def silu_grad(z):
 
 
 def forward(params, X, bf16: bool = True):
-    """y[n] and the cache for ``backward``.  X: [n][12]."""
+    """y[n] and the cache for ``backward``.  X: [n][60]."""
     p = unpack(params)
     W = [p["W2"], p["W3"], p["W4"]]
     if bf16:
@@ -167,17 +169,44 @@ def chamber(x, s):
                      np.sign(a(s[:, 0]) - a(s[:, 1])), np.sign(a(s[:, 1]) - a(s[:, 2]))], axis=1)
 
 
+def resonance_inputs(V) -> np.ndarray:
+    """Inputs [12:60] of feature set 2 from rows of FP32 components V[n][6]
+    (x, then s), in FP32: for ray r and pair (0, 1), (0, 2), (1, 2), k = 3 r +
+    pair, the ratio t = min(|v_i|, |v_j|) / max(|v_i|, |v_j|) (0 when both are
+    0) at [k] and 2 |q t - rint(q t)| at [6 + 7 k + q - 2] for q = 2..8."""
+    V = np.asarray(V, dtype=np.float32)[:, 0:6]
+    out = np.empty((V.shape[0], 48), dtype=np.float32)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        for k in range(6):
+            r, p = divmod(k, 3)
+            i, j = _RISK_PAIRS[p]
+            a = np.abs(V[:, 3 * r + i])
+            b = np.abs(V[:, 3 * r + j])
+            hi = np.maximum(a, b)
+            t = np.where(hi > 0, np.minimum(a, b) / np.where(hi > 0, hi, np.float32(1)),
+                         np.float32(0)).astype(np.float32)
+            out[:, k] = t
+            for q in range(2, 9):
+                u = np.float32(q) * t
+                out[:, 6 + 7 * k + q - 2] = np.float32(2) * np.abs(u - np.rint(u))
+    return out
+
+
 def features(s1, omega, raw_xray, omega_axis, F, axis_c=None, feature_set: int = 0):
-    """Network inputs [R][12] and the envelope invariants [R][2] (axial, cos2t);
+    """Network inputs [R][60] and the envelope invariants [R][2] (axial, cos2t);
     axis_c defaults to the unit F^T omega_axis.  feature_set 0: second
-    harmonics, 1: chamber signs."""
+    harmonics, 1: chamber signs, 2: chamber signs and the resonance inputs
+    of the FP32 components (zero in [12:60] for sets 0 and 1)."""
     x, s = crystal_frame(s1, omega, raw_xray, omega_axis, F)
     if axis_c is None:
         u = np.asarray(F, dtype=np.float64).reshape(3, 3).T @ np.asarray(omega_axis, float)
         axis_c = u / np.linalg.norm(u)
     extra = (np.concatenate([harmonics(x), harmonics(s)], axis=1) if feature_set == 0
              else chamber(x, s))
-    X = np.concatenate([x, s, extra], axis=1)
+    comps = np.concatenate([x, s], axis=1)
+    res = (resonance_inputs(comps.astype(np.float32)).astype(np.float64) if feature_set == 2
+           else np.zeros((len(x), 48)))
+    X = np.concatenate([comps, extra, res], axis=1)
     inv = np.stack([x @ np.asarray(axis_c, float), np.sum(x * s, axis=1)], axis=1)
     return X, inv
 

@@ -21,7 +21,7 @@ def _params(seed, scale=1.0):
 def test_backward_matches_central_differences():
     rng = np.random.default_rng(1)
     P = _params(2)
-    X = rng.normal(size=(6, 12))
+    X = rng.normal(size=(6, MO.N_IN))
     t = rng.normal(size=6)
     y, cache = MO.forward(P, X, bf16=False)
     g, loss = MO.backward(cache, y, t)
@@ -155,3 +155,55 @@ def test_band_mask_closed_forms():
     assert MO.band_mask(x, s, [(0, 0, 1, 0.0, 1e-9)]).all()      # x0 = 0: ratio 0 exactly
     assert MO.band_mask(x, s, [(0, 0, 2, 0.0, 1e-9)]).all()      # 0 / 0 counts as ratio 0
     assert not MO.band_mask(x, s, [(1, 0, 1, 0.5, 0.0)]).any()   # zero width: nothing
+
+
+def test_resonance_inputs_hand_values():
+    """Feature set 2's inputs [12:60] for hand-picked component rows: the
+    ratios, and 2 |q t - rint(q t)| with its zeros at t = p / q, the
+    round-half-even cases (t q = 1.5 -> 2, 4.5 -> 4) and both components 0."""
+    f32 = np.float32
+    V = np.zeros((3, 6), np.float32)
+    V[0] = [0.0, 0.6, 0.8, 0.36, 0.48, 0.8]
+    V[1] = [0.5, -0.5, 0.70710677, 0.0, 0.0, 1.0]
+    V[2] = [0.0, 0.0, 0.0, 1.0, 0.0, 0.0]
+    R = MO.resonance_inputs(V)
+    assert R.dtype == np.float32 and R.shape == (3, 48)
+    # row 0 ratios: x (0, 1) and (0, 2) -> 0, (1, 2) -> 0.6 / 0.8; s likewise
+    t = [f32(0), f32(0), f32(0.6) / f32(0.8), f32(0.36) / f32(0.48), f32(0.36) / f32(0.8),
+         f32(0.48) / f32(0.8)]
+    np.testing.assert_array_equal(R[0, :6], t)
+    for k in range(6):
+        for q in range(2, 9):
+            u = f32(q) * t[k]
+            assert R[0, 6 + 7 * k + q - 2] == f32(2) * abs(u - np.rint(u))
+    # t = 0.75: q t = 1.5 (rounds to 2: 1.0), 2.25 (0.5), 3 (0), 3.75 (0.5),
+    # 4.5 (rounds to 4: 1.0), 5.25 (0.5), 6 (0)
+    assert R[0, 2] == pytest.approx(0.75, abs=1e-7)
+    np.testing.assert_allclose(R[0, 6 + 7 * 2: 6 + 7 * 3], [1.0, 0.5, 0, 0.5, 1.0, 0.5, 0],
+                               atol=1e-6)
+    # t = 0: every distance 0; x (0, 1) of row 1 is exactly 1: distance 0 too
+    assert np.all(R[0, 6:6 + 14] == 0)
+    assert R[1, 0] == 1.0 and np.all(R[1, 6:13] == 0)
+    # s (0, 1) of row 1: both components zero -> t = 0
+    assert R[1, 3] == 0.0 and np.all(R[1, 6 + 21:6 + 28] == 0)
+    assert np.all(R[2, 0:3] == 0)
+
+
+def test_feature_set_layout():
+    """Sets 0 and 1 leave [12:60] zero; set 2 carries the chamber signs of set
+    1 and the resonance inputs of its own FP32 components."""
+    from anacor_public_amd import workload as W
+    s1, om = W.make_reflections(300, 13)
+    F = W.random_rotation(5).reshape(-1)
+    X0, _ = MO.features(s1, om, W.RAW_XRAY, W.OMEGA_AXIS, F, feature_set=0)
+    X1, _ = MO.features(s1, om, W.RAW_XRAY, W.OMEGA_AXIS, F, feature_set=1)
+    X2, _ = MO.features(s1, om, W.RAW_XRAY, W.OMEGA_AXIS, F, feature_set=2)
+    assert X0.shape == X1.shape == X2.shape == (300, MO.N_IN)
+    assert np.all(X0[:, 12:] == 0) and np.all(X1[:, 12:] == 0)
+    np.testing.assert_array_equal(X2[:, :12], X1[:, :12])
+    np.testing.assert_array_equal(X2[:, 12:], MO.resonance_inputs(X2[:, :6]))
+    # the ratio of (i, j) is symmetric in the two components and sign-blind
+    flip = X2[:, :6].astype(np.float32).copy()
+    flip[:, [3, 4]] = -flip[:, [4, 3]]
+    R = MO.resonance_inputs(flip)
+    np.testing.assert_array_equal(R[:, 3], X2[:, 12 + 3])
