@@ -20,9 +20,10 @@ this module is the checker for the GPU kernels of anacor_public_amd/csrc/mlp.hip
 
 Pinned by tests/test_oracle_mlp.py: central finite differences of the loss
 (FP64, ``bf16=False``) for every parameter block, BF16 rounding against
-torch's conversion and hand-worked tie cases, SiLU closed forms, and the
+torch's conversion and hand-worked tie cases, SiLU closed forms, the
 feature identities (unit norms, harmonic identities, rotation invariance of
-x . s).  Never imported by the product path.
+x . s), and hand-worked values of the resonance inputs (zeros at t = p / q,
+round-half-even cases, zero components).  Never imported by the product path.
 """
 from __future__ import annotations
 
@@ -264,7 +265,7 @@ _RISK_PAIRS = ((0, 1), (0, 2), (1, 2))
 
 
 def risk_features(X) -> np.ndarray:
-    """The 60 resonance features of rows of network inputs X[n][12] (x = X[:, 0:3],
+    """The 60 resonance features of rows of network inputs X[n][>= 6] (x = X[:, 0:3],
     s = X[:, 3:6]) in FP32, the operation chain of mlp.hip risk_features:
     the six components, per ray and pair (i, j) the ratio
     t = min(|v_i|, |v_j|) / max(|v_i|, |v_j|), and |q t - rint(q t)| * RN(1/q)
