@@ -8,17 +8,13 @@
 // (no FMA contraction) -fno-fast-math; gcc folds the reference's pow(v, 2)
 // into v*v, which is written out here.
 #include "geometry.h"
+#include "host_pool.h"
 
 #include <cmath>
-#include <condition_variable>
 #include <cstdint>
 #include <cstring>
-#include <functional>
-#include <mutex>
 #include <thread>
 #include <vector>
-
-#include <unistd.h>
 
 namespace anacor {
 namespace {
@@ -296,96 +292,6 @@ void rotate_one(const double* s1, double omega, const double* raw_xray,
   mul3(tr, s1, sv, 1);
 }
 
-// Worker threads kept for the process's lifetime: starting and joining 16
-// threads per call cost about 0.6 ms, most of the setup of a 20,000-reflection
-// call.  run() splits [0, R) into `parts` contiguous chunks; the caller runs
-// chunk 0 and worker w chunk w + 1; it returns when every worker has seen the
-// job, so the job may live on the caller's stack.  One job at a time (callers
-// from several threads queue on run_mu).  The pool is never destroyed: its
-// threads sleep on the condition variable until the process ends.
-class WorkerPool {
- public:
-  using Body = std::function<void(int64_t, int64_t)>;
-
-  void run(int64_t R, int parts, const Body& body) {
-    std::lock_guard<std::mutex> serial(run_mu_);
-    const int64_t chunk = (R + parts - 1) / parts;
-    int workers;
-    {
-      std::unique_lock<std::mutex> lk(mu_);
-      while ((int)threads_.size() < parts - 1) {
-        const int w = (int)threads_.size();
-        threads_.emplace_back([this, w] { loop(w); });
-      }
-      workers = (int)threads_.size();
-      body_ = &body;
-      R_ = R;
-      chunk_ = chunk;
-      parts_ = parts;
-      acks_ = 0;
-      ++gen_;
-    }
-    cv_.notify_all();
-    body((int64_t)0, chunk < R ? chunk : R);
-    std::unique_lock<std::mutex> lk(mu_);
-    done_cv_.wait(lk, [&] { return acks_ == workers; });
-    body_ = nullptr;
-  }
-
- private:
-  void loop(int w) {
-    uint64_t seen = 0;
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      seen = gen_ - 1;   // created inside run(): take part in the current job
-    }
-    for (;;) {
-      const Body* body;
-      int64_t lo, hi;
-      {
-        std::unique_lock<std::mutex> lk(mu_);
-        cv_.wait(lk, [&] { return gen_ != seen; });
-        seen = gen_;
-        body = body_;
-        lo = (int64_t)(w + 1) * chunk_;
-        hi = lo + chunk_ < R_ ? lo + chunk_ : R_;
-        if (w + 1 >= parts_) lo = hi;
-      }
-      if (lo < hi) (*body)(lo, hi);
-      {
-        std::lock_guard<std::mutex> lk(mu_);
-        ++acks_;
-      }
-      done_cv_.notify_one();
-    }
-  }
-
-  std::mutex run_mu_;
-  std::mutex mu_;
-  std::condition_variable cv_, done_cv_;
-  std::vector<std::thread> threads_;
-  uint64_t gen_ = 0;
-  const Body* body_ = nullptr;
-  int64_t R_ = 0, chunk_ = 0;
-  int parts_ = 0;
-  int acks_ = 0;
-};
-
-// The process's pool (a child after fork() starts its own: the parent's
-// threads do not exist there).
-WorkerPool& pool() {
-  static std::mutex mu;
-  static WorkerPool* p = nullptr;
-  static pid_t owner = 0;
-  std::lock_guard<std::mutex> lk(mu);
-  const pid_t me = getpid();
-  if (!p || owner != me) {
-    p = new WorkerPool();   // never destroyed (see above)
-    owner = me;
-  }
-  return *p;
-}
-
 // Run body(lo, hi) over [0, R) on up to n_threads threads (at most 64).
 template <class Body>
 void parallel_range(int64_t R, int n_threads, Body body) {
@@ -396,7 +302,7 @@ void parallel_range(int64_t R, int n_threads, Body body) {
     body((int64_t)0, R);
     return;
   }
-  pool().run(R, nt, WorkerPool::Body(body));
+  host_pool().run(R, nt, WorkerPool::Body(body));
 }
 
 }  // namespace

@@ -11,8 +11,9 @@
 // among those whose children both carry a Hessian of at least min_hess (ties:
 // the lower feature, then the lower cut); without a gain above the rounding of
 // its terms every row goes left.  Leaves get -learning_rate G / (H + l2).  Histograms are summed
-// per feature by one thread in row order, so the trees do not depend on the
-// number of threads.
+// per feature by one thread in row order, and the per-node totals over a fixed
+// split of the rows into kRowChunks chunks added in chunk order, so the trees
+// do not depend on the number of threads.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "errors.h"
+#include "host_pool.h"
 #include "../../include/anacor_mi355x.h"
 
 namespace {
@@ -49,21 +51,22 @@ std::vector<float> quantile_cuts(std::vector<float> v, int n_bins) {
   return cuts;
 }
 
+// fn(i) for i in [0, count) on up to `threads` threads (the library's
+// persistent host workers), in contiguous runs of i.
 template <class Fn>
 void parallel_for(int count, int threads, Fn fn) {
   if (threads <= 1 || count <= 1) {
     for (int i = 0; i < count; ++i) fn(i);
     return;
   }
-  std::vector<std::thread> pool;
-  const int t = std::min(threads, count);
-  pool.reserve((size_t)t);
-  for (int w = 0; w < t; ++w)
-    pool.emplace_back([&, w] {
-      for (int i = w; i < count; i += t) fn(i);
-    });
-  for (auto& th : pool) th.join();
+  anacor::host_pool().run(count, std::min(threads, count), [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; ++i) fn((int)i);
+  });
 }
+
+// Rows are split into this many contiguous chunks (fewer for small n) for the
+// per-row passes; sums over rows are taken per chunk and then in chunk order.
+constexpr int kRowChunks = 64;
 
 struct Split {
   double gain = 0.0;
@@ -142,14 +145,26 @@ extern "C" int anacor_risk_fit(const float* features, const uint8_t* labels,
   std::vector<GH> tot;                          // per node totals
   std::vector<uint32_t> act;                    // rows whose histogram is summed
 
+  const int chunks = (int)std::min<size_t>((size_t)kRowChunks, (N + 4095) / 4096);
+  const size_t cs = (N + (size_t)chunks - 1) / (size_t)chunks;
+  auto rows_of = [&](int k, size_t& lo, size_t& hi) {
+    lo = std::min(N, (size_t)k * cs);
+    hi = std::min(N, lo + cs);
+  };
+  std::vector<std::vector<GH>> part((size_t)chunks);      // per chunk, per node sums
+  std::vector<std::vector<uint32_t>> act_part((size_t)chunks);
   for (int t = 0; t < n_trees; ++t) {
-    for (size_t i = 0; i < N; ++i) {
-      const double w = weights ? (double)weights[i] : 1.0;
-      const double p = 1.0 / (1.0 + std::exp(-score[i]));
-      gh[i].g = w * (p - (double)labels[i]);
-      gh[i].h = w * std::max(p * (1.0 - p), 1e-16);
-      node[i] = 0;
-    }
+    parallel_for(chunks, threads, [&](int k) {
+      size_t lo, hi;
+      rows_of(k, lo, hi);
+      for (size_t i = lo; i < hi; ++i) {
+        const double w = weights ? (double)weights[i] : 1.0;
+        const double p = 1.0 / (1.0 + std::exp(-score[i]));
+        gh[i].g = w * (p - (double)labels[i]);
+        gh[i].h = w * std::max(p * (1.0 - p), 1e-16);
+        node[i] = 0;
+      }
+    });
     uint8_t* tf = feat + (size_t)t * n_int;
     float* tt = thr + (size_t)t * n_int;
     float* tl = leaf + (size_t)t * n_leaf;
@@ -157,11 +172,26 @@ extern "C" int anacor_risk_fit(const float* features, const uint8_t* labels,
       const int nodes = 1 << d;
       hist.resize((size_t)nodes * F * NB);
       tot.assign((size_t)nodes, GH{0.0, 0.0});
+      parallel_for(chunks, threads, [&](int k) {
+        size_t lo, hi;
+        rows_of(k, lo, hi);
+        std::vector<GH>& pt = part[(size_t)k];
+        std::vector<uint32_t>& ak = act_part[(size_t)k];
+        pt.assign((size_t)nodes, GH{0.0, 0.0});
+        ak.clear();
+        for (size_t i = lo; i < hi; ++i) {
+          pt[(size_t)node[i]].g += gh[i].g;
+          pt[(size_t)node[i]].h += gh[i].h;
+          if (d == 0 || !(node[i] & 1)) ak.push_back((uint32_t)i);
+        }
+      });
       act.clear();
-      for (size_t i = 0; i < N; ++i) {
-        tot[(size_t)node[i]].g += gh[i].g;
-        tot[(size_t)node[i]].h += gh[i].h;
-        if (d == 0 || !(node[i] & 1)) act.push_back((uint32_t)i);
+      for (int k = 0; k < chunks; ++k) {
+        for (int j = 0; j < nodes; ++j) {
+          tot[(size_t)j].g += part[(size_t)k][(size_t)j].g;
+          tot[(size_t)j].h += part[(size_t)k][(size_t)j].h;
+        }
+        act.insert(act.end(), act_part[(size_t)k].begin(), act_part[(size_t)k].end());
       }
       // Per feature, one task: the histograms of the level (level 0: the
       // root's; below: the left children's summed, the right ones as parent
@@ -233,24 +263,43 @@ extern "C" int anacor_risk_fit(const float* features, const uint8_t* labels,
           tt[g] = std::numeric_limits<float>::infinity();
         }
       }
-      for (size_t i = 0; i < N; ++i) {
-        const int j = node[i];
-        const Split& s = chosen[(size_t)j];
-        const bool right = s.feature >= 0 && (int)code[(size_t)s.feature * N + i] > s.cut;
-        node[i] = 2 * j + (right ? 1 : 0);
-      }
+      parallel_for(chunks, threads, [&](int k) {
+        size_t lo, hi;
+        rows_of(k, lo, hi);
+        for (size_t i = lo; i < hi; ++i) {
+          const int j = node[i];
+          const Split& s = chosen[(size_t)j];
+          const bool right = s.feature >= 0 && (int)code[(size_t)s.feature * N + i] > s.cut;
+          node[i] = 2 * j + (right ? 1 : 0);
+        }
+      });
       std::swap(prev, hist);
     }
     std::vector<GH> lv((size_t)n_leaf, GH{0.0, 0.0});
-    for (size_t i = 0; i < N; ++i) {
-      lv[(size_t)node[i]].g += gh[i].g;
-      lv[(size_t)node[i]].h += gh[i].h;
-    }
+    parallel_for(chunks, threads, [&](int k) {
+      size_t lo, hi;
+      rows_of(k, lo, hi);
+      std::vector<GH>& pt = part[(size_t)k];
+      pt.assign((size_t)n_leaf, GH{0.0, 0.0});
+      for (size_t i = lo; i < hi; ++i) {
+        pt[(size_t)node[i]].g += gh[i].g;
+        pt[(size_t)node[i]].h += gh[i].h;
+      }
+    });
+    for (int k = 0; k < chunks; ++k)
+      for (int j = 0; j < n_leaf; ++j) {
+        lv[(size_t)j].g += part[(size_t)k][(size_t)j].g;
+        lv[(size_t)j].h += part[(size_t)k][(size_t)j].h;
+      }
     for (int j = 0; j < n_leaf; ++j)
       tl[j] = lv[(size_t)j].h > 0.0
                   ? (float)(-(double)learning_rate * lv[(size_t)j].g / (lv[(size_t)j].h + l2))
                   : 0.f;
-    for (size_t i = 0; i < N; ++i) score[i] += (double)tl[node[i]];
+    parallel_for(chunks, threads, [&](int k) {
+      size_t lo, hi;
+      rows_of(k, lo, hi);
+      for (size_t i = lo; i < hi; ++i) score[i] += (double)tl[node[i]];
+    });
   }
   return ANACOR_OK;
 }
