@@ -99,6 +99,8 @@ TRAFFIC_BYTES_PER_STEP = {
 IB_TRAFFIC_BYTES_PER_READ = {"plain-256^3-stride2": 0.0024}
 METRIC = "absorption factors/sec (rays/s) at 1 GPU, plain vs surrogate at stated rel. error"
 SURROGATE_BOUND = 1e-3      # the stated relative error bound of the surrogate
+SURROGATE_BATCH = 4096      # Adam mini-batch of the 3.2M-pair fit (learning rate below)
+SURROGATE_LR = 2e-3
 LARGE_REFLECTIONS = 10000   # the default run's 1000x1000x800 record: its first 10k reflections
 
 
@@ -235,7 +237,70 @@ def gridding_baseline(tracer, sw, geo, A_true, steps: int, n_theta: int = 360,
     return out
 
 
-def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int = 150,
+def surrogate_mode(sur, sw, A_true, A_audit, plain_rate, steps, warmup, n_infer,
+                   bounded: bool, label: str = "") -> dict:
+    """One operating point of a fitted surrogate: held-out and audit errors
+    against the tracer, then `steps` timed predict calls on the inference set
+    (tracer fallback included)."""
+    sur.set_bounded(bounded)
+    A_sur, used = sur.predict(sw.heldout_s1, sw.heldout_omega)
+    err = rel_stats(A_sur[used], A_true[used])
+    A_a, used_a = sur.predict(sw.audit_s1, sw.audit_omega, fallback=False)
+    ra = np.abs(A_a[used_a] - A_audit[used_a]) / A_audit[used_a]
+    audit = {"reflections": int(len(used_a)), "answered": int(used_a.sum()),
+             "rel_error": rel_stats(A_a[used_a], A_audit[used_a]),
+             "above_bound": int(np.sum(ra > SURROGATE_BOUND))}
+    for _ in range(warmup):
+        sur.predict(sw.infer_s1, sw.infer_omega)
+    mlp_ms, fb = [], 0
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        _, u = sur.predict(sw.infer_s1, sw.infer_omega)
+        mlp_ms.append(sur.mlp.stats()["mlp_ms"])
+        fb = int((~u).sum())
+    elapsed = time.perf_counter() - t2
+    value = n_infer * steps / elapsed
+    out = {"value": value, "ms_per_step": 1e3 * elapsed / steps,
+           "heldout_answered": int(used.sum()), "heldout_rel_error": err,
+           "audit": audit, "fallback_reflections": fb,
+           "fallback_fraction": fb / n_infer,
+           "speedup_vs_plain_gpu": value / plain_rate,
+           "mlp_ms": float(np.mean(mlp_ms))}
+    name = ("bounded" if bounded else "fast") + label
+    log(f"# surrogate {name}: held-out ({out['heldout_answered']}/{len(used)} answered) "
+        f"{err}; audit {audit}; {value:.3e} A_h/s ({out['speedup_vs_plain_gpu']:.1f}x "
+        f"the plain GPU path's {plain_rate:.3e}), fallback {fb}")
+    return out
+
+
+def surrogate_named_config(tr, sw, geo, A_true, A_audit, plain_rate, steps, warmup,
+                           n_infer, n_train: int = 200_000, epochs: int = 600) -> dict:
+    """The surrogate in the configuration the workload names: 200k training
+    pairs (the first of the same training stream), 12 inputs (the two unit
+    vectors and the six chamber signs), 600 epochs, risk trees learned on the
+    training rows' errors; judged and timed on the same held-out, audit and
+    inference sets."""
+    from anacor_public_amd import _lib
+    from anacor_public_amd.surrogate import Surrogate
+    wl = sw.sample
+    with Surrogate(tr, *geo, feature_set=_lib.MLP_FEATURES_CHAMBER) as sur:
+        rep = sur.fit(wl.s1[:n_train], wl.omega[:n_train], epochs=epochs,
+                      bound=SURROGATE_BOUND)
+        log(f"# surrogate (named configuration) fit: train {rep['train_s']:.1f}s ({epochs} "
+            f"epochs on {n_train}), risk trees {rep['trust_fit_s']:.1f}s")
+        args = (sur, sw, A_true, A_audit, plain_rate, steps, warmup, n_infer)
+        fast = surrogate_mode(*args, False, " (named configuration)")
+        bounded = surrogate_mode(*args, True, " (named configuration)")
+    keys = ("value", "ms_per_step", "heldout_answered", "heldout_rel_error", "audit",
+            "fallback_reflections", "fallback_fraction", "speedup_vs_plain_gpu")
+    return {"training_pairs": n_train, "network_inputs": "12: components, chamber signs",
+            "epochs": epochs, "train_s": rep["train_s"],
+            "trust": {k: rep[k] for k in ("trust", "risk_cut", "risk_set", "risk_rows",
+                                          "risk_train_fallback")},
+            "bounded": {k: bounded[k] for k in keys}, "fast": {k: fast[k] for k in keys}}
+
+
+def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int = 300,
                   cpu_budget: float = 0.0):
     """Fit the surrogate (risk trees on a separate risk set of 800k
     reflections), judge it on the held-out set (10k reflections) and
@@ -266,8 +331,8 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
         for _ in range(3):
             tr.run(ps1, pom, *geo)
         plain_rate = 3 * len(pom) / (time.perf_counter() - t1)
-        rep = sur.fit(wl.s1, wl.omega, epochs=epochs, bound=SURROGATE_BOUND,
-                      risk_s1=sw.risk_s1, risk_omega=sw.risk_omega)
+        rep = sur.fit(wl.s1, wl.omega, epochs=epochs, batch=SURROGATE_BATCH, lr=SURROGATE_LR,
+                      bound=SURROGATE_BOUND, risk_s1=sw.risk_s1, risk_omega=sw.risk_omega)
         log(f"# surrogate fit: trace {rep['trace_s']:.2f}s, train {rep['train_s']:.1f}s "
             f"({epochs} epochs), final loss {rep['final_loss']:.3e}, risk trees "
             f"{rep['trust_fit_s']:.1f}s (cross-fitted fallback "
@@ -276,39 +341,9 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
         A_true = tr.run(sw.heldout_s1, sw.heldout_omega, *geo)
         A_audit = tr.run(sw.audit_s1, sw.audit_omega, *geo)
 
-        def mode(bounded: bool) -> dict:
-            sur.set_bounded(bounded)
-            A_sur, used = sur.predict(sw.heldout_s1, sw.heldout_omega)
-            err = rel_stats(A_sur[used], A_true[used])
-            A_a, used_a = sur.predict(sw.audit_s1, sw.audit_omega, fallback=False)
-            ra = np.abs(A_a[used_a] - A_audit[used_a]) / A_audit[used_a]
-            audit = {"reflections": int(len(used_a)), "answered": int(used_a.sum()),
-                     "rel_error": rel_stats(A_a[used_a], A_audit[used_a]),
-                     "above_bound": int(np.sum(ra > SURROGATE_BOUND))}
-            for _ in range(warmup):
-                sur.predict(sw.infer_s1, sw.infer_omega)
-            mlp_ms, fb = [], 0
-            t2 = time.perf_counter()
-            for _ in range(steps):
-                _, u = sur.predict(sw.infer_s1, sw.infer_omega)
-                mlp_ms.append(sur.mlp.stats()["mlp_ms"])
-                fb = int((~u).sum())
-            elapsed = time.perf_counter() - t2
-            value = n_infer * steps / elapsed
-            out = {"value": value, "ms_per_step": 1e3 * elapsed / steps,
-                   "heldout_answered": int(used.sum()), "heldout_rel_error": err,
-                   "audit": audit, "fallback_reflections": fb,
-                   "fallback_fraction": fb / n_infer,
-                   "speedup_vs_plain_gpu": value / plain_rate,
-                   "mlp_ms": float(np.mean(mlp_ms))}
-            name = "bounded" if bounded else "fast"
-            log(f"# surrogate {name}: held-out ({out['heldout_answered']}/{len(used)} answered) "
-                f"{err}; audit {audit}; {value:.3e} A_h/s ({out['speedup_vs_plain_gpu']:.1f}x "
-                f"the plain GPU path's {plain_rate:.3e}), fallback {fb}")
-            return out
-
-        fast = mode(False)
-        bounded = mode(True)
+        args = (sur, sw, A_true, A_audit, plain_rate, steps, warmup, n_infer)
+        fast = surrogate_mode(*args, False)
+        bounded = surrogate_mode(*args, True)
         mms = bounded["mlp_ms"]
         achieved = n_infer * MLP_FLOPS_PER_ROW / (mms * 1e-3) / 1e12
         rec = {
@@ -320,7 +355,7 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
             "fallback_fraction": bounded["fallback_fraction"],
             "network_inputs": "60: components, chamber signs, resonance kinks",
             "training_pairs": int(len(wl.omega)), "risk_pairs": int(rep["risk_rows"]),
-            "epochs": epochs,
+            "epochs": epochs, "batch": SURROGATE_BATCH, "lr": SURROGATE_LR,
             "train_s": rep["train_s"], "train_trace_s": rep["trace_s"],
             "trust": {k: rep[k] for k in ("trust", "risk_trees", "risk_depth", "risk_cut",
                                           "risk_set", "risk_rows", "risk_train_fallback",
@@ -341,6 +376,8 @@ def run_surrogate(steps: int, warmup: int, n_infer: int = 1_000_000, epochs: int
                          "traffic": None, "kernel": "MLP forward (HIP events)",
                          "kernel_ms": mms, "flops_per_launch": n_infer * MLP_FLOPS_PER_ROW},
         }
+        rec["named_config"] = surrogate_named_config(tr, sw, geo, A_true, A_audit, plain_rate,
+                                                     steps, warmup, n_infer)
         # the reference's own approximation on the same sample: gridding maps
         rec["gridding_baseline"] = gridding_baseline(tr, sw, geo, A_true, steps)
         if cpu_budget > 0:
@@ -643,7 +680,7 @@ def main():
                     help="skip the 1000x1000x800 record of the default run")
     ap.add_argument("--large-reflections", type=int, default=LARGE_REFLECTIONS,
                     help="reflections of the default run's 1000x1000x800 record (its first R)")
-    ap.add_argument("--epochs", type=int, default=150, help="surrogate training epochs")
+    ap.add_argument("--epochs", type=int, default=300, help="surrogate training epochs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--tile", type=int, default=0)

@@ -241,7 +241,8 @@ def test_training_fits_a_smooth_map(gpu):
 @pytest.fixture(scope="module")
 def fitted():
     """The surrogate of batched sample 2 as the benchmark fits it: 3.2M traced
-    training reflections, 150 epochs, risk trees learned on 800k more."""
+    training reflections, 300 epochs of batch 4096 at lr 2e-3, risk trees
+    learned on 800k more."""
     from anacor_public_amd import workload as W
     from anacor_public_amd.ray_tracing import Tracer
     from anacor_public_amd.surrogate import Surrogate
@@ -250,7 +251,8 @@ def fitted():
     say(f"case: fit surrogate of {wl.name} on {len(wl.omega)} reflections")
     tr = Tracer.for_workload(wl)
     sur = Surrogate(tr, wl.raw_xray, wl.omega_axis, wl.F)
-    rep = sur.fit(wl.s1, wl.omega, epochs=150, risk_s1=sw.risk_s1, risk_omega=sw.risk_omega)
+    rep = sur.fit(wl.s1, wl.omega, epochs=300, batch=4096, lr=2e-3, risk_s1=sw.risk_s1,
+                  risk_omega=sw.risk_omega)
     say(f"  trained in {rep['train_s']:.1f}s, loss {rep['final_loss']:.3e}")
     yield sw, tr, sur, rep
     sur.close()
