@@ -638,14 +638,13 @@ __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, cons
                                          int last_zyx[3], double out[4]) {
   const bool round = L.bias != 0.0;
   bool unclamped = false;
-  // raw layouts only: the bricked walk waits on memory, not on VALU issue,
-  // and measured no faster without the clamp (1000x1000x800 volume, 10k
-  // reflections in launch groups of 2,048: 996.7 ms with it, 1000.2 ms
-  // without)
-  if (!GUARD && LAYOUT < kLayoutBrick2) {
+  // every layout: steps past a ray's end (the unrolled tail) are not taken,
+  // and their loads stay range-checked
+  if (!GUARD) {
     // with the zero shell the walk reads increment incmax + 1 of every ray
-    // (limit - 1 for an inclusive bound, limit for an exclusive one)
-    const int last = FULL ? L.limit - 1 : L.incmax + 1;
+    // (limit - 1 for an inclusive bound, limit for an exclusive one); without
+    // it (full_iteration, bricked layouts) it takes increments below limit
+    const int last = (FULL || LAYOUT >= kLayoutBrick2) ? L.limit - 1 : L.incmax + 1;
     const bool in_range = L.limit <= 0 || (round ? lane_in_range<true>(L, last)
                                                  : lane_in_range<false>(L, last));
     unclamped = __all(in_range) != 0;
