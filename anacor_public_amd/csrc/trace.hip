@@ -632,15 +632,18 @@ __device__ __forceinline__ bool lane_in_range(const RayLane& L, int last) {
 // the stepped coordinates.  The rounding is fixed per ray and almost always
 // the same across a wave; the clamp is dropped only when no lane of the
 // wave needs it (a wave vote), so that branch never diverges.
-template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool L04>
+// BRICK_FREE: the bricked layouts take the clamp-free forms too (steps past a
+// ray's end, the unrolled tail, are not taken and their loads stay
+// range-checked).  The FP64-mean and absorption-map kernels leave it off: in
+// the FP64-mean kernel the two more inlined forms made the launch 50% slower
+// (1,294 against 864 ms on 1000x1000x800), while the FP32 one gained 1%.
+template <int CAP, int U, bool GUARD, bool FULL, int LAYOUT, bool L04, bool BRICK_FREE = false>
 __device__ __forceinline__ void walk_ray(const RayLane& L, const Volume& V, const double vs[3],
                                          lds_u32* seg, int& len_out, int& nseg_out,
                                          int last_zyx[3], double out[4]) {
   const bool round = L.bias != 0.0;
   bool unclamped = false;
-  // every layout: steps past a ray's end (the unrolled tail) are not taken,
-  // and their loads stay range-checked
-  if (!GUARD) {
+  if (!GUARD && (BRICK_FREE || LAYOUT < kLayoutBrick2)) {
     // with the zero shell the walk reads increment incmax + 1 of every ray
     // (limit - 1 for an inclusive bound, limit for an exclusive one); without
     // it (full_iteration, bricked layouts) it takes increments below limit
@@ -788,8 +791,8 @@ __device__ __forceinline__ void trace_tile(const SampleView& S, const ReflRule* 
         verr = ANACOR_RAY_ERR_ASSERT;
         continue;
       }
-      walk_ray<CAP, U, GUARD, FULL, LAYOUT, L04>(L, S.V, S.vs, seg, lens[k], nsegs[k], lasts[k],
-                                            Lr[k]);
+      walk_ray<CAP, U, GUARD, FULL, LAYOUT, L04, !F64>(L, S.V, S.vs, seg, lens[k], nsegs[k],
+                                                      lasts[k], Lr[k]);
       acc.steps += (unsigned long long)lens[k];
     }
     acc.err |= verr;

@@ -92,7 +92,7 @@ TRAFFIC_BYTES_PER_STEP = {
     (_HEAD, 2, (0, 0, 0), 0, 2048): 0.0015,
     (_HEAD, 2, (0, 0, 0), 512, 2048): 0.0013,
     (_HEAD, 2, (0, 0, 0), 2048, 2048): 0.0007,
-    (_LARGE10K, 2, (0, 0, 0), 0, 2048): 0.6992,
+    (_LARGE10K, 2, (0, 0, 0), 0, 2048): 0.6935,
 }
 # The same per label read of ib_kernel (2-bit bricks), over 2.207e10 reads,
 # profiles/ib_kernel_bisection-256cube.md.
