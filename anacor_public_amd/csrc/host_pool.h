@@ -16,25 +16,32 @@ namespace anacor {
 // Worker threads kept for the process's lifetime: starting and joining 16
 // threads per call cost about 0.6 ms, most of the setup of a 20,000-reflection
 // call.  run() splits [0, R) into `parts` contiguous chunks; the caller runs
-// chunk 0 and worker w chunk w + 1; it returns when every worker has seen the
-// job, so the job may live on the caller's stack.  One job at a time (callers
-// from several threads queue on run_mu).  The pool is never destroyed: its
-// threads sleep on the condition variable until the process ends.
+// chunk 0 and worker w chunk w + 1; it returns when those workers have
+// finished their chunks (the other workers wake, read the job's sizes under
+// the lock and sleep again, never touching the body), so the job may live on
+// the caller's stack.  One job at a time (callers from several threads queue
+// on run_mu).  The pool is never destroyed: its threads sleep on the
+// condition variable until the process ends.
 class WorkerPool {
  public:
   using Body = std::function<void(int64_t, int64_t)>;
 
   void run(int64_t R, int parts, const Body& body) {
     std::lock_guard<std::mutex> serial(run_mu_);
-    const int64_t chunk = (R + parts - 1) / parts;
-    int workers;
+    int64_t chunk;
     {
       std::unique_lock<std::mutex> lk(mu_);
-      while ((int)threads_.size() < parts - 1) {
-        const int w = (int)threads_.size();
-        threads_.emplace_back([this, w] { loop(w); });
+      try {
+        while ((int)threads_.size() < parts - 1) {
+          const int w = (int)threads_.size();
+          threads_.emplace_back([this, w] { loop(w); });
+        }
+      } catch (...) {
+        // no more threads: the job is split over the ones there are
       }
-      workers = (int)threads_.size();
+      const int workers = (int)threads_.size();
+      if (parts > workers + 1) parts = workers + 1;
+      chunk = (R + parts - 1) / parts;
       body_ = &body;
       R_ = R;
       chunk_ = chunk;
@@ -45,7 +52,7 @@ class WorkerPool {
     cv_.notify_all();
     body((int64_t)0, chunk < R ? chunk : R);
     std::unique_lock<std::mutex> lk(mu_);
-    done_cv_.wait(lk, [&] { return acks_ == workers; });
+    done_cv_.wait(lk, [&] { return acks_ == parts - 1; });
     body_ = nullptr;
   }
 
@@ -63,10 +70,10 @@ class WorkerPool {
         std::unique_lock<std::mutex> lk(mu_);
         cv_.wait(lk, [&] { return gen_ != seen; });
         seen = gen_;
+        if (w + 1 >= parts_) continue;   // no chunk of this job
         body = body_;
         lo = (int64_t)(w + 1) * chunk_;
         hi = lo + chunk_ < R_ ? lo + chunk_ : R_;
-        if (w + 1 >= parts_) lo = hi;
       }
       if (lo < hi) (*body)(lo, hi);
       {
