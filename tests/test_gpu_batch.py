@@ -57,6 +57,25 @@ def test_batch_matches_unbatched_and_oracle(gpu, oracle, samples):
     assert st["voxel_steps"] == total_steps
 
 
+def test_batch_with_an_empty_sample(gpu, samples):
+    """A sample without reflections gives an empty array; every other sample's
+    result is the same bit for bit as in the full batch."""
+    from anacor_public_amd.ray_tracing import Batch
+    say("case: batch with sample 1 empty")
+    s1 = [w.s1 for w in samples]
+    om = [w.omega for w in samples]
+    geo = (samples[0].raw_xray, samples[0].omega_axis, samples[0].F)
+    s1e, ome = list(s1), list(om)
+    s1e[1], ome[1] = np.zeros((0, 3)), np.zeros(0)
+    with Batch(samples) as b:
+        full = b.run(s1, om, *geo)
+        part = b.run(s1e, ome, *geo)
+    assert part[1].shape == (0,)
+    for i in range(len(samples)):
+        if i != 1:
+            np.testing.assert_array_equal(bits(part[i]), bits(full[i]), err_msg=f"sample {i}")
+
+
 def test_repeated_runs_with_eager_work_between(gpu, samples):
     from anacor_public_amd.ray_tracing import Batch, Tracer
     say("case: repeated batch runs")

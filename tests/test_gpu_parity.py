@@ -192,6 +192,21 @@ def test_empty_voxel_list_gives_nan(gpu, golden_small):
         assert np.all(np.isnan(t.run_workload(wl)))
 
 
+def test_zero_reflections(gpu, golden_small):
+    """No reflections: every trace entry returns an empty result and no error."""
+    say("case: zero reflections")
+    wl = workload_from(golden_small)
+    s1, om = np.zeros((0, 3)), np.zeros(0)
+    geo = (wl.raw_xray, wl.omega_axis, wl.F)
+    with tracer(wl) as t:
+        assert t.run(s1, om, *geo, strict=True).shape == (0,)
+        assert t.paths(s1, om, *geo).shape == (0, len(wl.coords))
+        assert t.bisection(s1, om, *geo, strict=True).shape == (0,)
+        # and the context still traces afterwards
+        A = t.run(wl.s1[:2], wl.omega[:2], *geo)
+    assert np.all(np.isfinite(A))
+
+
 def test_single_gpu_config_parity_and_properties(gpu, oracle):
     """256^3 volume, stride-2 crystal voxels: paths bit-exact on 6 reflections,
     A_h within 1e-6 of the oracle on 96, then on all 20,000 reflections the
