@@ -61,3 +61,25 @@ def test_order_argument_checks(gpu):
         for bad in ((-1, 0), (1, 2), (1, -2)):
             with pytest.raises(_lib.AnacorError):
                 t.set_order(*bad)
+
+
+@pytest.mark.parametrize("layout_name", ["RAW", "BRICK2"])
+def test_two_part_call_equals_one_part_calls(gpu, layout_name):
+    """A call of 8,192 reflections or more runs in two halves (the second
+    half set up on the host while the first is traced); at a fixed tile its
+    results equal, bit for bit, those of calls below that size, which run in
+    one part."""
+    from anacor_public_amd import _lib
+    from anacor_public_amd import workload as W
+    from anacor_public_amd.ray_tracing import Tracer
+    wl = W.small_config(R=12001)            # odd: halves of 6,000 and 6,001
+    geo = (wl.raw_xray, wl.omega_axis, wl.F)
+    say(f"case: two-part call, {layout_name}, R = {len(wl.omega)}")
+    with Tracer.for_workload(wl) as t:
+        t.set_layout(getattr(_lib, f"LAYOUT_{layout_name}"))
+        t.set_tile(256)
+        whole = t.run_workload(wl)
+        parts = np.concatenate([t.run(wl.s1[i:i + 4000], wl.omega[i:i + 4000], *geo)
+                                for i in range(0, len(wl.omega), 4000)])
+    np.testing.assert_array_equal(bits(whole), bits(parts))
+    assert np.mean(np.isfinite(whole)) > 0.99
