@@ -2146,9 +2146,9 @@ static int ctx_run(Context* c, const double* s1, const double* omega, int64_t R,
   HIP_TRY(c->h_rules.reserve((size_t)std::max<int64_t>(R, 1)));
   const int threads = host_threads(n_threads);
   // large calls without path records in two parts: the host sets up and
-  // sorts the second half while the GPU traces the first (on 20,000
-  // reflections about 0.4 ms of a 64 ms step)
-  const int64_t first = (!paths && R >= kSplitMinReflections) ? R / 2 : R;
+  // sorts the last three quarters while the GPU traces the first quarter
+  // (which holds thousands of workgroups: the call's tile is the whole call's)
+  const int64_t first = (!paths && R >= kSplitMinReflections) ? R / 4 : R;
   host_setup(s1, omega, first, raw_xray, omega_axis, F, max_dim(c), c->h_rules.data(), nullptr,
              threads);
   c->last_host_ms = ms_since(t0);

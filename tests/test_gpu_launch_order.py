@@ -65,14 +65,14 @@ def test_order_argument_checks(gpu):
 
 @pytest.mark.parametrize("layout_name", ["RAW", "BRICK2"])
 def test_two_part_call_equals_one_part_calls(gpu, layout_name):
-    """A call of 8,192 reflections or more runs in two halves (the second
-    half set up on the host while the first is traced); at a fixed tile its
-    results equal, bit for bit, those of calls below that size, which run in
-    one part."""
+    """A call of 8,192 reflections or more runs in two parts (the last three
+    quarters set up on the host while the first quarter is traced); at a
+    fixed tile its results equal, bit for bit, those of calls below that
+    size, which run in one part."""
     from anacor_public_amd import _lib
     from anacor_public_amd import workload as W
     from anacor_public_amd.ray_tracing import Tracer
-    wl = W.small_config(R=12001)            # odd: halves of 6,000 and 6,001
+    wl = W.small_config(R=12001)            # parts of 3,000 and 9,001
     geo = (wl.raw_xray, wl.omega_axis, wl.F)
     say(f"case: two-part call, {layout_name}, R = {len(wl.omega)}")
     with Tracer.for_workload(wl) as t:
