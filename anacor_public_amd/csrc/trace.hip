@@ -3000,6 +3000,7 @@ struct IbArgs {
   unsigned long long* oob;     // of which outside the volume
   double* lengths;             // [R][N][2][4] in the caller's voxel order, or null
   const int* order;
+  int pos0;                    // processing position of the launch's first block row
 };
 
 // Label at integer-valued FP64 position (z, y, x); 0 outside the volume.
@@ -3134,8 +3135,9 @@ void ib_kernel(IbArgs a) {
   __shared__ ReduceLds red;
   const int lb = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
   if (lb >= a.n_blocks) return;
-  const int pos = lb / a.tiles;
-  const int tile = lb - pos * a.tiles;
+  const int row = lb / a.tiles;
+  const int tile = lb - row * a.tiles;
+  const int pos = a.pos0 + row;
   const int refl = a.order[pos];
   const Volume V = make_volume(a.vol, a.Z, a.Y, a.X, a.vol_bytes, a.cz, a.cy);
   const int k = (int)threadIdx.x & 1;
@@ -3198,7 +3200,9 @@ void ib_kernel(IbArgs a) {
     }
   }
   if (oob) atomicAdd(a.oob, (unsigned long long)oob);
-  reduce_tile<F64, false>(acc, red, &a.err[refl], a.reads, a.partial32 + lb, a.partial64 + lb);
+  const size_t slot = (size_t)pos * a.tiles + tile;   // finish_kernel reads [position][tile]
+  reduce_tile<F64, false>(acc, red, &a.err[refl], a.reads, a.partial32 + slot,
+                          a.partial64 + slot);
 }
 
 template <int LAYOUT, bool F64, bool LENGTHS>
@@ -3247,7 +3251,12 @@ static int report_ray_errors(Context* c, const double* out, int64_t R) {
 
 // Bisection traversal of R reflections whose rules the caller has written
 // into c->h_ib (pinned).
-static int ib_run(Context* c, int64_t R, int num_cls, int flags, double* out, double* lengths) {
+// With first in (0, R) and setup_rest set (no lengths): only the rules of
+// reflections [0, first) are ready; setup_rest() fills the rest into
+// h_ib + first while the GPU bisects the first part, and each part is sorted
+// by direction on its own (as run_rules does for the walk).
+static int ib_run(Context* c, int64_t R, int num_cls, int flags, double* out, double* lengths,
+                  int64_t first = -1, const std::function<int()>& setup_rest = {}) {
   const bool f64 = (flags & ANACOR_FLAG_F64_MEAN) != 0;
   c->last_oob = 0;
   c->last_steps = 0;
@@ -3274,9 +3283,10 @@ static int ib_run(Context* c, int64_t R, int num_cls, int flags, double* out, do
   }
   if (!c->d_oob) HIP_TRY(hipMalloc(&c->d_oob, sizeof(unsigned long long)));
   const IbRule* rules = c->h_ib.data();
-  order_by_direction(c, R, [rules](int64_t i, int k) { return rules[i].ray[k].dir; }, 0);
+  const int64_t RA = (!lengths && setup_rest && first > 0 && first < R) ? first : R;
+  order_by_direction(c, RA, [rules](int64_t i, int k) { return rules[i].ray[k].dir; }, 0);
   HIP_TRY(c->h_order_pin.reserve((size_t)R));
-  for (int64_t i = 0; i < R; ++i) c->h_order_pin[(size_t)i] = c->h_order[(size_t)i];
+  for (int64_t i = 0; i < RA; ++i) c->h_order_pin[(size_t)i] = c->h_order[(size_t)i];
   // Automatic layout: bricks whenever they fit.  The bisection's gathers are
   // scattered along the ray, so a 4x or 2x smaller volume keeps more of it in
   // the L2s; the walk's choice (raw up to 64 MB) does not carry over.
@@ -3297,9 +3307,9 @@ static int ib_run(Context* c, int64_t R, int num_cls, int flags, double* out, do
   if (lengths) HIP_TRY(hipMalloc(&len_buf.p, (size_t)R * c->N * 8 * sizeof(double)));
   double* const d_len = (double*)len_buf.p;
 
-  HIP_TRY(hipMemcpyAsync(c->d_ib, c->h_ib.data(), (size_t)R * sizeof(IbRule),
+  HIP_TRY(hipMemcpyAsync(c->d_ib, c->h_ib.data(), (size_t)RA * sizeof(IbRule),
                          hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order_pin.data(), (size_t)R * sizeof(int),
+  HIP_TRY(hipMemcpyAsync(c->d_order, c->h_order_pin.data(), (size_t)RA * sizeof(int),
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemsetAsync(c->d_err, 0, (size_t)R * sizeof(unsigned), c->stream));
   HIP_TRY(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
@@ -3331,11 +3341,39 @@ static int ib_run(Context* c, int64_t R, int num_cls, int flags, double* out, do
   a.oob = c->d_oob;
   a.lengths = d_len;
   a.order = c->d_order;
-  const int grid = (int)((nblk + 7) / 8 * 8);
+  auto launch_rows = [&](int64_t lo, int64_t hi) -> int {
+    a.pos0 = (int)lo;
+    a.n_blocks = (int)((hi - lo) * tiles);
+    dispatch_ib(a, (int)((a.n_blocks + 7) / 8 * 8), layout, f64, lengths != nullptr, c->stream);
+    HIP_TRY(hipGetLastError());
+    return ANACOR_OK;
+  };
 
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  dispatch_ib(a, grid, layout, f64, lengths != nullptr, c->stream);
-  HIP_TRY(hipGetLastError());
+  rc = launch_rows(0, RA);
+  if (rc) return rc;
+  if (RA < R) {
+    // the second part: its setup and direction order on the host while the
+    // GPU bisects the first; its uploads beside that launch, which reads
+    // positions and reflections below RA only
+    rc = setup_rest();
+    if (rc) return rc;
+    const int64_t n = R - RA;
+    std::vector<int> ord((size_t)n);
+    std::vector<uint64_t> key((size_t)n), tmp((size_t)n);
+    direction_order(n, [&](int64_t i, int k) { return rules[RA + i].ray[k].dir; }, ord.data(),
+                    key.data(), tmp.data(), 0);
+    for (int64_t i = 0; i < n; ++i)
+      c->h_order_pin[(size_t)(RA + i)] = (int)RA + ord[(size_t)i];
+    HIP_TRY(hipMemcpyAsync(c->d_ib + RA, c->h_ib.data() + RA, (size_t)n * sizeof(IbRule),
+                           hipMemcpyHostToDevice, c->copy_stream));
+    HIP_TRY(hipMemcpyAsync(c->d_order + RA, c->h_order_pin.data() + RA, (size_t)n * sizeof(int),
+                           hipMemcpyHostToDevice, c->copy_stream));
+    HIP_TRY(hipEventRecord(c->copied, c->copy_stream));
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->copied, 0));
+    rc = launch_rows(RA, R);
+    if (rc) return rc;
+  }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   hipLaunchKernelGGL(finish_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, c->stream,
                      (const float2*)c->d_partial, (const double*)c->d_partial, c->d_err,
@@ -3749,9 +3787,20 @@ int anacor_ctx_bisection(anacor_ctx* ctx, const double* s1, const double* omega,
   HIP_TRY(hipSetDevice(c->device));
   const auto t0 = std::chrono::steady_clock::now();
   HIP_TRY(c->h_ib.reserve((size_t)std::max<int64_t>(R, 1)));
-  ib_setup(s1, omega, R, raw_xray, omega_axis, F, c->h_ib.data(), host_threads(0));
+  const int threads = host_threads(0);
+  // large calls in two parts, as anacor_ctx_run's
+  const int64_t first = R >= kSplitMinReflections ? R / 4 : R;
+  ib_setup(s1, omega, first, raw_xray, omega_axis, F, c->h_ib.data(), threads);
   c->last_host_ms = ms_since(t0);
-  return ib_run(c, R, num_cls, flags, out, nullptr);
+  if (first == R) return ib_run(c, R, num_cls, flags, out, nullptr);
+  auto setup_rest = [&]() -> int {
+    const auto t1 = std::chrono::steady_clock::now();
+    ib_setup(s1 + 3 * first, omega + first, R - first, raw_xray, omega_axis, F,
+             c->h_ib.data() + first, threads);
+    c->last_host_ms += ms_since(t1);
+    return ANACOR_OK;
+  };
+  return ib_run(c, R, num_cls, flags, out, nullptr, first, setup_rest);
 }
 
 int anacor_ctx_bisection_rotated(anacor_ctx* ctx, const double* rotated_s1, const double* xray,

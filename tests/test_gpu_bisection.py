@@ -132,3 +132,29 @@ def test_fuzz_cases(gpu, oracle):
         for i in np.flatnonzero(ok):
             _, _, Lo = oracle.ib_test(wl, wl.coords, rs1[i], rxr[i], 4, lengths=True)
             np.testing.assert_array_equal(L[i], Lo)
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+@pytest.mark.parametrize("layout_name", ["RAW", "BRICK2"])
+def test_two_part_call_equals_one_part_calls(gpu, layout_name):
+    """A bisection call of 8,192 reflections or more runs in two parts (the
+    host sets up the last three quarters while the GPU bisects the first);
+    at a fixed tile its means equal, bit for bit, those of calls below that
+    size, which run in one part."""
+    from anacor_public_amd import _lib
+    from anacor_public_amd import workload as W
+    from anacor_public_amd.ray_tracing import Tracer
+    wl = W.small_config(R=12001)
+    geo = (wl.raw_xray, wl.omega_axis, wl.F)
+    say(f"case: two-part bisection call, {layout_name}, R = {len(wl.omega)}")
+    with Tracer.for_workload(wl) as t:
+        t.set_layout(getattr(_lib, f"LAYOUT_{layout_name}"))
+        t.set_tile(256)
+        whole = t.bisection(wl.s1, wl.omega, *geo)
+        parts = np.concatenate([t.bisection(wl.s1[i:i + 4000], wl.omega[i:i + 4000], *geo)
+                                for i in range(0, len(wl.omega), 4000)])
+    np.testing.assert_array_equal(bits(whole), bits(parts))
+    assert np.mean(np.isfinite(whole)) > 0.99
