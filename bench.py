@@ -19,7 +19,9 @@ default run) is trained on 3.2M reflections the tracer computes for batched
 sample 2 (100^3, 20,089 crystal voxels), its risk trees on the network's
 errors on 800k more, judged by the tracer on 10k held-out reflections and a
 200k audit set, and timed on 1M reflections of which 1% lie outside its
-training 2-theta range and go to the tracer.
+training 2-theta range and go to the tracer.  Its "named_config" record is
+the same surrogate in the configuration the workload names (200k training
+reflections, 12 inputs, lr 1e-3), fitted and timed on the same sets.
 
 Workloads (anacor_public_amd/workload.py):
   single   256^3 volume, 25,323 crystal voxels (stride 2), 20,000 reflections (default)
