@@ -843,8 +843,27 @@ __global__ __launch_bounds__(256) void risk_kernel(const float* __restrict__ X, 
   if (feat_out)
     for (int k = 0; k < kRiskF; ++k) feat_out[i * kRiskF + k] = col[k * 256];
   if (r.n_trees <= 0) return;
+  // eight trees walked side by side (eight independent chains of node loads
+  // in flight; 1M rows of 200 depth-6 trees: 0.79 ms against 1.62 ms one
+  // tree at a time, 1.00 ms with four, 0.77 ms with sixteen), their leaves
+  // added in tree order as one at a time would
+  constexpr int kT = 8;
   float s = r.base;
-  for (int t = 0; t < r.n_trees; ++t) {
+  int t = 0;
+  for (; t + kT <= r.n_trees; t += kT) {
+    int j[kT] = {};
+    for (int d = 0; d < r.depth; ++d) {
+      RiskNode e[kT];
+#pragma unroll
+      for (int q = 0; q < kT; ++q) e[q] = r.nodes[(size_t)(t + q) * r.n_int + j[q]];
+#pragma unroll
+      for (int q = 0; q < kT; ++q)
+        j[q] = 2 * j[q] + 1 + (col[e[q].feat * 256] > e[q].thr ? 1 : 0);
+    }
+#pragma unroll
+    for (int q = 0; q < kT; ++q) s += r.leaf[(size_t)(t + q) * (r.n_int + 1) + (j[q] - r.n_int)];
+  }
+  for (; t < r.n_trees; ++t) {
     const RiskNode* nd = r.nodes + (size_t)t * r.n_int;
     int j = 0;
     for (int d = 0; d < r.depth; ++d) {
