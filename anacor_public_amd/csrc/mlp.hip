@@ -318,11 +318,13 @@ constexpr size_t kWbSize = kWbW1l + (size_t)kH * kInK;
 
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
+template <int ROWS>
 __global__ __launch_bounds__(256) void mlp_infer_kernel(
     const float* __restrict__ X, int64_t n, const float* __restrict__ P,
     const __bf16* __restrict__ Wb, const uint8_t* __restrict__ flag, double mean, double stdv,
     float* __restrict__ y, double* __restrict__ A) {
-  constexpr int ROWS = 64;
+  constexpr int RT = ROWS / 16;   // 16-row MFMA tiles
+  static_assert(ROWS % 16 == 0 && ROWS <= 256, "row tile");
   __shared__ __attribute__((aligned(16))) __bf16 Hh[ROWS][kHP];
   __shared__ __attribute__((aligned(16))) __bf16 Hl[ROWS][kHP];
   __shared__ float part[4][ROWS];
@@ -340,11 +342,11 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(
   const int lane = t & 63, wave = t >> 6;
   const int ub = wave * 64;                     // this wave's units
   const int r16 = lane & 15, kg = (lane >> 4) * 8, q4 = (lane >> 4) * 4;
-  floatx4 acc[4][4];
+  floatx4 acc[4][RT];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < RT; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
   // layer 1: W1 (hi, lo) x X (hi, lo), three products, K = kInK in steps of 32
   for (int k0 = 0; k0 < kInK; k0 += 32) {
     bf16x8 wh[4], wl[4];
@@ -354,7 +356,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(
       wl[i] = *(const bf16x8*)(Wb + kWbW1l + (size_t)(ub + i * 16 + r16) * kInK + k0 + kg);
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < RT; ++j) {
       const bf16x8 xh = *(const bf16x8*)&Hh[j * 16 + r16][k0 + kg];
       const bf16x8 xl = *(const bf16x8*)&Hl[j * 16 + r16][k0 + kg];
 #pragma unroll
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < RT; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
       const __bf16* W = Wb + (size_t)(l - 1) * kH * kH;
       for (int k0 = 0; k0 < kH; k0 += 32) {
         bf16x8 w[4];
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(
         for (int i = 0; i < 4; ++i)
           w[i] = *(const bf16x8*)(W + (size_t)(ub + i * 16 + r16) * kH + k0 + kg);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < RT; ++j) {
           const bf16x8 hh = *(const bf16x8*)&Hh[j * 16 + r16][k0 + kg];
           const bf16x8 hl = *(const bf16x8*)&Hl[j * 16 + r16][k0 + kg];
 #pragma unroll
@@ -398,7 +400,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(
       const int u = ub + i * 16 + q4;
       const float4 bias = *(const float4*)(P + offb[l] + u);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < RT; ++j) {
         const int row = j * 16 + r16;
         bf16x4 hh, hl;
 #pragma unroll
@@ -416,7 +418,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(
   }
   // output layer: per row, this wave's 64 units of SiLU(z4) . w5
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < RT; ++j) {
     float v = 0.f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -971,13 +973,16 @@ static void refresh_bf16(Model* M) {
                      M->P.p, M->Wb.p);
 }
 
+// Rows per inference workgroup (LDS: 2 x ROWS x 264 BF16 + 4 x ROWS floats).
+constexpr int kInferRows = 64;
+
 // Inference over n rows of device features X: y[n] and, with flag/A, the
 // absorption factors (fused inference kernel, one launch).
 static void infer(Model* M, const float* X, int64_t n, const uint8_t* flag, float* y,
                   double* A) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(mlp_infer_kernel, dim3(nblocks(n, 64)), dim3(256), 0, M->stream, X, n,
-                     M->P.p, M->Wb.p, flag, M->y_mean, M->y_std, y, A);
+  hipLaunchKernelGGL(mlp_infer_kernel<kInferRows>, dim3(nblocks(n, kInferRows)), dim3(256), 0,
+                     M->stream, X, n, M->P.p, M->Wb.p, flag, M->y_mean, M->y_std, y, A);
 }
 
 // K slices of a weight-gradient GEMM with K = n rows.
