@@ -53,7 +53,12 @@ def test_gather_probe_rejects_bad_arguments():
 
 
 def _bits(a):
-    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+    """Bit patterns, every NaN mapped to one: IEEE 754 fixes no sign or payload
+    for a NaN that arithmetic produces, and compilers may legally move a
+    negation across it (-(a/b) as (-a)/b), as the clang host build does on
+    the delta-table entries of faces a ray cannot leave through."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return np.where(np.isnan(a), np.nan, a).view(np.uint64)
 
 
 def _oracle_d_table(oracle, g):
