@@ -168,6 +168,27 @@ def test_risk_kernel_matches_oracle(gpu, refl):
             m.set_risk(bad, cut)
 
 
+@pytest.mark.parametrize("n_trees,depth", [(1, 3), (7, 4), (8, 6), (17, 2)])
+def test_risk_scores_any_tree_count(gpu, refl, n_trees, depth):
+    """The risk kernel walks eight trees side by side and the rest one at a
+    time; for tree counts below, at and past a multiple of eight its scores
+    equal the oracle's bit for bit (leaves added in tree order)."""
+    from anacor_public_amd.surrogate import MLP, risk_fit
+    s1, om, F = refl
+    raw = np.array([0.0, 0.0, -1.0])
+    axis = np.array([1.0, 0.0, 0.0])
+    say(f"case: risk scores, {n_trees} trees of depth {depth}")
+    with MLP(seed=3) as m:
+        X, _, _, _ = m.features(s1, om, raw, axis, F)
+        Fo = MO.risk_features(X)
+        lab = (Fo[:, 6 + 1] > np.median(Fo[:, 6 + 1])).astype(np.uint8)
+        trees = risk_fit(Fo, lab, n_trees=n_trees, depth=depth)
+        m.set_risk(trees)
+        _, sc = m.risk(X)
+        so = MO.risk_scores(trees, Fo)
+        np.testing.assert_array_equal(sc.view(np.uint32), so.view(np.uint32))
+
+
 def test_forward_matches_oracle(gpu, refl):
     from anacor_public_amd import workload as W
     from anacor_public_amd.surrogate import MLP
